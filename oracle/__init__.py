@@ -1,0 +1,46 @@
+"""CPU oracle for the ERA / difference-map iteration loop.
+
+TEST INFRASTRUCTURE ONLY.  This package is a Python 3 / NumPy restatement of
+the reference's ``ptychography/era.py`` and ``ptychography/dm.py`` (the
+reference itself is Python 2 and imports mpi4py at module scope, so it cannot
+be imported or run here).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
+as the timed CPU baseline -- never as part of the shipped GPU path.
+
+Every function cites the reference ``file:line`` it follows.  The MPI layer of
+the reference is restated for one rank (``size == 1``): every ``Allreduce`` /
+``reduce`` / ``bcast`` / ``scatter`` is the identity.
+
+Pinning status (what checks each function against something other than its
+own formula; the tests live in ``tests/test_oracle_*.py``):
+
+* ``multiroll``                 -- nested ``np.roll`` and a brute-force index loop.
+* ``make_exits``                -- brute-force index loop; periodic-roll formula.
+* ``psup_O`` / ``psup_P``       -- brute-force loops, the adjoint identity, the
+                                   exact-data fixed point.
+* ``make_P_heatmap`` / ``make_O_heatmap`` -- adjoint applied to |P|^2 / |O|^2.
+* ``pmod_1`` / ``Pmod_1``       -- direct O(n^2) DFT on small frames, closed-form
+                                   modulus invariants, idempotence.
+* ``pmod_7`` / ``Pmod_7``       -- closed-form invariants (masked pixels -> 0,
+                                   background scaled by the same factor).
+* ``Pmod_P``                    -- modulus invariant on the probe.
+* ``center_of_mass``            -- ``scipy.ndimage.center_of_mass``.
+* ``ERA`` (methods 1-6)         -- monotone eMod for method 1, exact-data fixed
+                                   point, an independent pure-loop ERA on tiny
+                                   inputs (direct DFT, explicit loops).
+* ``DM`` (methods 1-6)          -- exact-data fixed point, an independent
+                                   pure-loop DM on tiny inputs.
+* ``numpy.fft`` itself           -- parity unpinned against the reference's own
+                                   (unpinned) numpy version; pinned here by the
+                                   direct DFT and closed forms.
+"""
+
+from .era import (ERA, preamble, pmod_1, Pmod_1, pmod_7, Pmod_7, Pmod_P, psup_O,
+                  psup_P, make_P_heatmap, make_O_heatmap, make_exits, multiroll,
+                  center_of_mass, probe_centre_shift)
+from .dm import DM, model_error_1
+
+__all__ = ["ERA", "DM", "preamble", "pmod_1", "Pmod_1", "pmod_7", "Pmod_7",
+           "Pmod_P", "psup_O", "psup_P", "make_P_heatmap", "make_O_heatmap",
+           "make_exits", "multiroll", "center_of_mass", "probe_centre_shift",
+           "model_error_1"]

@@ -1,0 +1,392 @@
+"""NumPy restatement of the reference ERA solver and its projection helpers.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  Follows
+``ptychography/era.py`` of andyofmelbourne/Ptychography for one MPI rank.
+Arrays stay in the caller's precision (complex128 for float64 intensities,
+complex64 for float32, exactly as the reference's ``preamble`` decides).
+
+Build additions (not in the reference, marked where they occur):
+
+* ``info['eSup']`` -- the support/consistency error per iteration,
+  ``sqrt(M*K * sum_n |psi'_n - P*O_{i+1,n}|^2 / I_norm)``: the legacy class's
+  ``calc_eSup`` (``ptychography/Ptychography_2dsample_2dprobe_farfield.py:282-286``,
+  unitary FFT) rescaled by Parseval to era.py's unnormalised FFT convention.
+* ``callback(i, O, P)`` -- called at the end of every iteration so tests can
+  compare iterates; ``None`` by default.
+* ``rng`` -- where the reference draws unseeded ``np.random.random`` numbers
+  (default probe, default background) this uses ``np.random`` too, unless a
+  ``numpy.random.Generator``-like object with ``.random(shape)`` is given.
+"""
+
+import numpy as np
+
+
+# ---------------------------------------------------------------------------
+# index helpers
+# ---------------------------------------------------------------------------
+
+def multiroll(x, shift):
+    """Periodic roll of every axis of ``x`` by ``shift`` (era.py:631-740).
+
+    ``y[i, j, ...] = x[(i - s0) mod n0, (j - s1) mod n1, ...]``; identical to
+    nested ``np.roll`` calls, one per axis.
+    """
+    x = np.asarray(x)
+    shift = tuple(int(s) for s in shift)
+    if len(shift) != x.ndim:
+        raise ValueError("multiroll needs one shift per axis")
+    return np.roll(x, shift, axis=tuple(range(x.ndim)))
+
+
+def make_exits(O, P, R, exits=None):
+    """Exit waves ``psi_n = roll(O, r_n)[:M, :K] * P`` (era.py:426-433).
+
+    This keeps the reference's cost model: every frame rolls the whole object.
+    """
+    if exits is None:
+        exits = np.empty((len(R),) + P.shape, dtype=P.dtype)
+    M, K = P.shape
+    for n in range(len(R)):
+        exits[n] = multiroll(O, (R[n][0], R[n][1]))[:M, :K] * P
+    return exits
+
+
+def make_P_heatmap(P, R, shape):
+    """Sum of |P|^2 placed at every frame window of the object (era.py:402-410)."""
+    M, K = P.shape
+    heat = np.zeros(shape, dtype=P.real.dtype)
+    p2 = (P.conj() * P).real
+    for r in R:
+        heat[-r[0]:M - r[0], -r[1]:K - r[1]] += p2
+    return heat
+
+
+def make_O_heatmap(O, R, shape):
+    """Sum over frames of the |O|^2 window, cropped to the probe shape (era.py:412-417)."""
+    heat = np.zeros(O.shape, dtype=O.real.dtype)
+    o2 = (O * O.conj()).real
+    for r in R:
+        heat += multiroll(o2, (r[0], r[1]))
+    return heat[:shape[0], :shape[1]]
+
+
+# ---------------------------------------------------------------------------
+# consistency (overlap) projections
+# ---------------------------------------------------------------------------
+
+def psup_O(exits, P, R, O_shape, P_heatmap=None, alpha=1.0e-10):
+    """Object update ``O = sum_n window_n^T(conj(P) psi_n) / (H + alpha)`` (era.py:336-369).
+
+    ``H`` (the P heatmap) is recomputed only when ``P_heatmap is None``.
+    """
+    M, K = P.shape
+    numer = np.zeros(O_shape, P.dtype)
+    if P_heatmap is None:
+        P_heatmap = make_P_heatmap(P, R, O_shape)
+    Pc = P.conj()
+    for r, ex in zip(R, exits):
+        numer[-r[0]:M - r[0], -r[1]:K - r[1]] += ex * Pc
+    return numer / (P_heatmap + alpha), P_heatmap
+
+
+def psup_P(exits, O, R, O_heatmap=None, alpha=1.0e-10):
+    """Probe update ``P = sum_n conj(O_n) psi_n / (sum_n |O_n|^2 + alpha)`` (era.py:371-400)."""
+    M, K = exits[0].shape
+    numer = np.zeros((M, K), exits.dtype)
+    if O_heatmap is None:
+        O_heatmap = np.ascontiguousarray(make_O_heatmap(O, R, (M, K)))
+    Oc = O.conj()
+    for r, ex in zip(R, exits):
+        numer += ex * Oc[-r[0]:M - r[0], -r[1]:K - r[1]]
+    return numer / (O_heatmap + alpha), O_heatmap
+
+
+# ---------------------------------------------------------------------------
+# modulus projections
+# ---------------------------------------------------------------------------
+
+def _mask_is_array(mask):
+    return isinstance(mask, np.ndarray) and mask.ndim >= 1
+
+
+def Pmod_1(amp, exits, mask=1, alpha=1.0e-10, eMod_calc=False):
+    """Fourier-space modulus replacement, masked pixels kept (era.py:560-572).
+
+    ``Mg = |psi_hat| + alpha``; ``eMod = sum mask (Mg - amp)^2``;
+    ``psi_hat *= amp / Mg`` except where ``mask == 0`` (factor 1).
+    """
+    Mg = np.sqrt((exits.conj() * exits).real) + alpha
+    eMod = np.sum((Mg - amp) ** 2 * mask) if eMod_calc else None
+    fac = amp / Mg
+    if _mask_is_array(mask):
+        bad = np.where(mask == 0)
+        if len(bad[0]) > 0:
+            fac[:, bad[0], bad[1]] = 1.
+    exits *= fac
+    return exits, eMod
+
+
+def pmod_1(amp, exits, mask=1, alpha=1.0e-10, eMod_calc=False):
+    """FFT -> Pmod_1 -> inverse FFT over the last two axes (era.py:550-558)."""
+    ex = np.fft.fftn(exits, axes=(-2, -1))
+    ex, eMod = Pmod_1(amp, ex, mask, alpha, eMod_calc)
+    ex = np.fft.ifftn(ex, axes=(-2, -1))
+    if eMod_calc:
+        return ex, eMod
+    return ex
+
+
+def Pmod_7(amp, background, exits, mask=1, alpha=1.0e-10, eMod_calc=False):
+    """Modulus replacement with an incoherent background (era.py:583-595).
+
+    ``Mg = sqrt(|psi_hat|^2 + b^2 + alpha)``; ``fac = mask * amp / Mg``.  The
+    reference's ``if mask is 1`` test never fires once the preamble has turned
+    the mask into an array, so masked pixels get factor 0 (psi_hat and b are
+    zeroed there).  ``background`` is scaled in place.
+    """
+    Mg = np.sqrt((exits.conj() * exits).real + background ** 2 + alpha)
+    eMod = np.sum((Mg - amp) ** 2 * mask) if eMod_calc else None
+    fac = mask * amp / Mg
+    exits *= fac
+    background *= fac
+    return exits, background, eMod
+
+
+def pmod_7(amp, background, exits, mask=1, alpha=1.0e-10, eMod_calc=False):
+    """FFT -> Pmod_7 -> inverse FFT (era.py:574-581)."""
+    ex = np.fft.fftn(exits, axes=(-2, -1))
+    ex, background, eMod = Pmod_7(amp, background, ex, mask, alpha, eMod_calc)
+    ex = np.fft.ifftn(ex, axes=(-2, -1))
+    if eMod_calc:
+        return ex, background, eMod
+    return ex, background
+
+
+def Pmod_P(amp, P, mask=1, alpha=1.0e-10):
+    """Far-field modulus constraint on the probe itself (era.py:597-610)."""
+    Pf = np.fft.fftn(P, axes=(-2, -1))
+    fac = amp / (np.sqrt((Pf.conj() * Pf).real) + alpha)
+    if _mask_is_array(mask):
+        bad = np.where(mask == 0)
+        if len(bad[0]) > 0:
+            fac[bad[0], bad[1]] = 1.
+    Pf *= fac
+    return np.fft.ifftn(Pf, axes=(-2, -1))
+
+
+# ---------------------------------------------------------------------------
+# probe centring (era.py:172-186)
+# ---------------------------------------------------------------------------
+
+def center_of_mass(a):
+    """Intensity-weighted centroid per axis (what ``scipy.ndimage.center_of_mass`` returns)."""
+    a = np.asarray(a, dtype=np.float64)
+    tot = a.sum()
+    rows = np.arange(a.shape[0], dtype=np.float64)
+    cols = np.arange(a.shape[1], dtype=np.float64)
+    return (float((a.sum(axis=1) * rows).sum() / tot),
+            float((a.sum(axis=0) * cols).sum() / tot))
+
+
+def probe_centre_shift(P):
+    """``cm = rint(centre_of_mass(|P|^2)) - shape // 2`` (era.py:175-176; Python 2 ``/``)."""
+    a = (P.conj() * P).real
+    cm = np.rint(np.array(center_of_mass(a))).astype(np.int64)
+    return cm - np.array(a.shape, dtype=np.int64) // 2
+
+
+# ---------------------------------------------------------------------------
+# preamble (era.py:435-548)
+# ---------------------------------------------------------------------------
+
+def _random(rng, shape):
+    if rng is None:
+        return np.random.random(shape)
+    return rng.random(shape)
+
+
+def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype,
+             rng=None):
+    """Resolve method/update/dtype, default O and P, amplitudes, mask and R.
+
+    ``R`` is shifted IN PLACE so that each column's maximum is 0 (era.py:504-505).
+    Returns a dict with the same quantities the reference's tuple carries.
+    """
+    if method is None:
+        if O is None and P is None:
+            method = 3
+        elif O is None:
+            method = 1
+        elif P is None:
+            method = 2
+        else:
+            raise ValueError("method=None needs O or P to be None (the reference "
+                             "leaves 'update' undefined here)")
+        if background is not None:
+            method += 3
+    if method in (1, 4):
+        update = 'O'
+    elif method in (2, 5):
+        update = 'P'
+    elif method in (3, 6):
+        update = 'OP'
+    else:
+        raise ValueError("method must be one of 1..6")
+
+    if isinstance(OP_iters, (int, np.integer)):
+        OP_iters = (int(OP_iters), 1)
+
+    if dtype is None:
+        dtype = 'single' if I.dtype == np.float32 else 'double'
+    if dtype == 'single':
+        r_dtype, c_dtype = np.float32, np.complex64
+    elif dtype == 'double':
+        r_dtype, c_dtype = np.float64, np.complex128
+    else:
+        raise ValueError("dtype must be None, 'single' or 'double'")
+
+    if O is None:
+        shape = (I.shape[1] + R[:, 0].max() - R[:, 0].min(),
+                 I.shape[2] + R[:, 1].max() - R[:, 1].min())
+        O = np.ones(shape, dtype=c_dtype)
+    if P is None:
+        P = _random(rng, I[0].shape) + 1J * _random(rng, I[0].shape)
+    P = P.astype(c_dtype)
+    O = O.astype(c_dtype)
+
+    I_norm = np.sum(mask * I)
+    amp = np.fft.ifftshift(np.sqrt(I).astype(r_dtype), axes=(-2, -1))
+    mask = np.asarray(mask)
+    if mask.ndim > 0:
+        mask = np.fft.ifftshift(mask)
+
+    R[:, 0] -= R[:, 0].max()
+    R[:, 1] -= R[:, 1].max()
+
+    exits = make_exits(O, P, R)
+
+    if method in (4, 5, 6):
+        if background is None:
+            background = (_random(rng, exits.shape).astype(r_dtype) + 0.1).astype(r_dtype)
+        else:
+            tmp = np.empty(exits.shape, dtype=r_dtype)
+            tmp[:] = np.sqrt(np.fft.ifftshift(background))
+            background = tmp
+    # outside methods 4-6 a passed background is left untouched; the reference
+    # only uses it for info['background'] (era.py:311-314)
+
+    Pamp = np.fft.fftn(P, axes=(-2, -1))
+    Pamp = np.sqrt((Pamp.conj() * Pamp).real)
+    return dict(method=method, update=update, dtype=r_dtype, c_dtype=c_dtype,
+                OP_iters=OP_iters, O=O, P=P, amp=amp, Pamp=Pamp,
+                background=background, R=R, mask=mask, I_norm=I_norm,
+                N=len(I), exits=exits)
+
+
+def _state(update, O, P):
+    if update == 'O':
+        return O
+    if update == 'P':
+        return P
+    return np.hstack((O.ravel(), P.ravel()))
+
+
+def _econ(bak, cur):
+    d = bak - cur
+    return np.sqrt(np.sum((d * d.conj()).real) / np.sum((cur * cur.conj()).real))
+
+
+def _centre(P, O):
+    cm = probe_centre_shift(P)
+    return multiroll(P, -cm), multiroll(O, -cm), cm
+
+
+# ---------------------------------------------------------------------------
+# ERA (era.py:13-333)
+# ---------------------------------------------------------------------------
+
+def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
+        Pmod_probe=False, probe_centering=False, hardware='cpu', alpha=1.0e-10,
+        dtype=None, full_output=True, callback=None, rng=None):
+    """Error-reduction ptychography, one-rank restatement of era.py:13-333."""
+    st = preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha,
+                  dtype, rng)
+    method, update, OP_iters = st['method'], st['update'], st['OP_iters']
+    O, P, amp, Pamp = st['O'], st['P'], st['amp'], st['Pamp']
+    background, R, mask = st['background'], st['R'], st['mask']
+    I_norm, exits = st['I_norm'], st['exits']
+    MK = P.shape[0] * P.shape[1]
+
+    P_heatmap = None
+    O_heatmap = None
+    eMods, eCons, eSups = [], [], []
+    bg_mode = method in (4, 5, 6)
+
+    for i in range(iters):
+        bak = _state(update, O, P).copy()
+
+        # modulus projection (era.py:160 / :230)
+        if bg_mode:
+            exits, background, eMod = pmod_7(amp, background, exits, mask,
+                                             alpha=alpha, eMod_calc=True)
+        else:
+            exits, eMod = pmod_1(amp, exits, mask, alpha=alpha, eMod_calc=True)
+
+        # consistency projection (era.py:163-189 / :233-259)
+        if update == 'O':
+            O, P_heatmap = psup_O(exits, P, R, O.shape, P_heatmap, alpha)
+        elif update == 'P':
+            P, O_heatmap = psup_P(exits, O, R, O_heatmap, alpha)
+        else:
+            if i % OP_iters[1] == 0:
+                for _ in range(OP_iters[0]):
+                    O, P_heatmap = psup_O(exits, P, R, O.shape, None, alpha)
+                    P, O_heatmap = psup_P(exits, O, R, None, alpha)
+                if probe_centering:
+                    P, O, _ = _centre(P, O)
+                    P_heatmap = O_heatmap = None
+            else:
+                O, P_heatmap = psup_O(exits, P, R, O.shape, P_heatmap, alpha)
+
+        # far-field probe modulus (era.py:193-194)
+        if Pmod_probe is not None and i < Pmod_probe:
+            P = Pmod_P(Pamp, P, mask, alpha)
+
+        # frame-averaged background (era.py:265-270)
+        if bg_mode:
+            background[:] = np.mean(background, axis=0)
+
+        psi_mod = exits
+        exits = make_exits(O, P, R)
+
+        # build addition: support error on the frames just projected
+        d = psi_mod - exits
+        eSups.append(float(np.sqrt(MK * np.sum((d * d.conj()).real) / I_norm)))
+
+        eCons.append(_econ(bak, _state(update, O, P)))
+        eMods.append(np.sqrt(eMod / I_norm))
+        if callback is not None:
+            callback(i, O, P)
+
+    if not full_output:
+        if update == 'O':
+            return O
+        if update == 'P':
+            return P
+        return O, P
+
+    info = {}
+    info['I'] = np.fft.fftshift(np.abs(np.fft.fftn(exits, axes=(-2, -1))) ** 2,
+                                axes=(-2, -1))
+    info['eMod'] = eMods
+    info['eCon'] = eCons
+    info['eSup'] = eSups
+    info['heatmap'] = P_heatmap
+    if background is not None:
+        b = background[0] if background.ndim == 3 else background
+        info['background'] = np.fft.fftshift(b) ** 2
+    if update == 'O':
+        return O, info
+    if update == 'P':
+        return P, info
+    return O, P, info
