@@ -1,0 +1,224 @@
+"""Benchmark: ERA iterations/s at 4096 positions x 256^2 frames on one MI355X.
+
+Workload ("era-4096x256"): the production-shape scan -- 64 x 64 raster of
+4096 positions, 40 px step with +-3 px jitter, 256 x 256 frames, object
+2782 x 2782 (smoothed random amplitude/phase), pupil radius 64 px with defocus,
+smooth background of 1 % of the mean intensity, Poisson noise at 1e8 photons
+per frame, central beamstop masked.  Synthetic data, generated on the GPU by
+the library's own frame-formation / FFT kernels plus torch.poisson.
+
+A step is one ERA iteration (method 1: object update with the known probe)
+over all 4096 frames, with every input resident in HBM.  The same run also
+times the difference map with background refinement (method 4) at the same
+shape, reported under "extra".
+
+Prints ONE JSON line (rank 0, N=1).
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# compulsory bytes (per frame pixel, per object pixel) of one ERA method-1
+# iteration: amp read 4 + psi' write 8 + psi' read 8; object read/write 16 + FP64 heatmap 8
+ERA1_BYTES_PX = 20
+OBJ_BYTES_PX = 24
+# DM method 4 (f, b, amp through two modulus steps + two overlap reads + metric amp read)
+DM4_BYTES_PX = 76
+
+# algorithmic bytes per frame pixel of each stage's launch (ERA method 1)
+STAGE_BYTES_PX = {
+    "rows_fwd": 8,    # write the row spectrum (object window and probe come from cache)
+    "cols": 20,       # read spectrum 8 + amplitude 4, write 8
+    "rows_inv": 16,   # read 8, write psi' 8
+    "gather": 8,      # read psi' 8 (+ object pixel terms below)
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--dm-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=48,
+                    help="frames in the bounded CPU-baseline sample")
+    ap.add_argument("--grid", type=int, default=64, help="raster side (64 -> 4096 positions)")
+    return ap.parse_args()
+
+
+def make_scan_gpu(grid, dev):
+    """C3 recipe at full size, intensities formed on the GPU."""
+    import torch
+    from ptychography_amd import sim
+    from ptychography_amd.engine import Engine
+
+    spec = sim.RECIPES["c3"].with_(grid=(grid, grid))
+    rng = np.random.default_rng(spec.seed)
+    M = spec.frame
+    off = sim.make_positions(spec.grid, spec.step, spec.jitter, rng)
+    U = int(off[:, 0].max()) + M
+    V = int(off[:, 1].max()) + M
+    O = sim.make_object((U, V), rng, spec.amp_range, spec.sigma)
+    P = sim.make_probe(M, spec.pupil_radius, spec.defocus, spec.zernike_rms, rng)
+    R = (-off).astype(np.int32)
+    N = len(R)
+    gen = Engine(N, M, M, U, V, R, device=dev)
+    O_t = torch.from_numpy(O.astype(np.complex64)).to(dev)
+    P_t = torch.from_numpy(P.astype(np.complex64)).to(dev)
+    gen.set_state(O_t, P_t)
+    I = gen.intensity()
+    gen.close()
+    del O_t
+    scale = spec.photons / float(I.sum(dtype=torch.float64).item() / N)
+    I *= scale
+    P = P * np.sqrt(scale)
+    B = sim._rescale(sim._gauss_smooth(rng.random((M, M)), 2.0 * spec.sigma), 0.5, 1.5)
+    B *= spec.background_frac * float(I.mean().item()) / B.mean()
+    I += torch.from_numpy(B.astype(np.float32)).to(dev)[None]
+    torch.manual_seed(spec.seed)
+    I = torch.poisson(I)
+    mask = sim.make_mask(M, spec.beamstop_radius, 0.0, rng)
+    return dict(I=I, R=R, P=P, O=O, B=B, mask=mask, N=N, M=M, U=U, V=V, spec=spec)
+
+
+def bind(eng, scan, dev, O0, P0):
+    import torch
+    mask_c = torch.from_numpy(scan["mask"].astype(np.uint8)).to(dev)
+    mask_s = torch.from_numpy(np.fft.ifftshift(scan["mask"]).astype(np.uint8)).to(dev)
+    amp, I_norm = eng.prepare_amp(scan["I"], mask_c)
+    eng.set_data(amp, mask_s, I_norm)
+    O_t = torch.from_numpy(O0.astype(np.complex64)).to(dev)
+    P_t = torch.from_numpy(P0.astype(np.complex64)).to(dev)
+    eng.set_state(O_t, P_t)
+    return O_t, P_t, amp
+
+
+def cpu_baseline(scan, nframes):
+    """Oracle ERA (method 1, complex128, reference-faithful make_exits) on the first
+    `nframes` frames with the full-size object; per-iteration time scaled to all frames."""
+    import oracle
+    I = scan["I"][:nframes].double().cpu().numpy()
+    R = scan["R"][:nframes].astype(np.int64)
+    O0 = np.ones((scan["U"], scan["V"]), np.complex128)
+    stamps = []
+    t0 = time.perf_counter()
+    oracle.ERA(I, R.copy(), scan["P"].astype(np.complex128), O0, 2, method=1,
+               mask=scan["mask"], full_output=False,
+               callback=lambda i, O, P: stamps.append(time.perf_counter()))
+    per_iter = stamps[-1] - stamps[-2]
+    total = time.perf_counter() - t0
+    per_iter_full = per_iter * scan["N"] / nframes
+    return {"value": 1.0 / per_iter_full, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle.ERA method 1, complex128, {nframes} of {scan['N']} frames "
+                       f"(256^2) with the full {scan['U']}x{scan['V']} object, 2 iterations "
+                       f"({total:.1f} s wall); second iteration {per_iter:.3f} s scaled by "
+                       f"{scan['N']}/{nframes}; host {platform.processor() or platform.machine()}")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding "
+                                   "is a design note only (--gpus must be 1)"}))
+        sys.exit(2)
+    import torch
+    from ptychography_amd.engine import Engine
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+
+    t_gen = time.perf_counter()
+    scan = make_scan_gpu(args.grid, dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    N, M, U, V = scan["N"], scan["M"], scan["U"], scan["V"]
+    px = N * M * M
+
+    # ---- ERA method 1 (headline) ----
+    eng = Engine(N, M, M, U, V, scan["R"], device=dev)
+    O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
+    eng.run(0, 1, args.warmup, iter0=0, begin=True)
+    eng.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    metrics = eng.run(0, 1, args.steps, iter0=args.warmup)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = eng.profile_read()
+    eng.profile(False)
+    eng.close()
+    del amp, O_t, P_t
+
+    its = args.steps / dt
+    stage_avg = {k: (ms / n if n else 0.0) for k, (ms, n) in prof.items()}
+    dom = max((k for k in prof if k in STAGE_BYTES_PX), key=lambda k: prof[k][0])
+    dom_bytes = STAGE_BYTES_PX[dom] * px + (OBJ_BYTES_PX * U * V if dom == "gather" else 0)
+    dom_gbs = dom_bytes / (stage_avg[dom] * 1e-3) / 1e9
+    iter_bytes = ERA1_BYTES_PX * px + OBJ_BYTES_PX * U * V
+
+    # ---- DM with background refinement (method 4), same shape ----
+    dm = None
+    if args.dm_steps > 0:
+        eng = Engine(N, M, M, U, V, scan["R"], device=dev)
+        O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
+        b0 = np.sqrt(np.fft.ifftshift(scan["B"])).astype(np.float32)
+        eng.set_background(torch.from_numpy(b0), per_frame=False)
+        eng.run(1, 4, 2, iter0=0, begin=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dm_metrics = eng.run(1, 4, args.dm_steps, iter0=2)
+        torch.cuda.synchronize()
+        dm_dt = time.perf_counter() - t1
+        eng.close()
+        del amp, O_t, P_t
+        dm_its = args.dm_steps / dm_dt
+        dm = {"dm_bg_iters_per_s": dm_its, "dm_bg_ms_per_step": 1e3 * dm_dt / args.dm_steps,
+              "dm_bg_roofline_frac": (DM4_BYTES_PX * px + OBJ_BYTES_PX * U * V) * dm_its / (HBM_PEAK_GBS * 1e9),
+              "dm_bg_final_eMod": float(dm_metrics[-1, 0])}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scan, args.cpu_frames)
+
+    out = {
+        "metric": "ERA/DM iters/s (4096x256^2 frames)",
+        "value": its,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded smoothed-random object, defocused pupil probe, Poisson noise)",
+        "config": {"workload": f"era-method1-{N}x{M}x{M}-object{U}x{V}", "frames": N,
+                   "frame": [M, M], "object": [U, V], "method": 1, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "kernel_ms": stage_avg[dom],
+                     "algorithmic_bytes_per_launch": dom_bytes},
+        "iteration_roofline_frac": iter_bytes * its / (HBM_PEAK_GBS * 1e9),
+        "stage_ms_per_launch": stage_avg,
+        "final_eMod": float(metrics[-1, 0]),
+        "gen_s": t_gen,
+        "cpu_baseline": cpu,
+        "extra": dm,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/*
+ * ptyhip -- C ABI of the MI355X (gfx950) ERA / difference-map engine.
+ *
+ * The reference (andyofmelbourne/Ptychography) has no native layer: its hot
+ * path is the Python pair ERA / DM (ptychography/era.py:13, ptychography/dm.py:13)
+ * and the NumPy helpers they call.  Each entry point below replaces one of
+ * those pieces; the Python facade ptychography_amd.ERA / .DM keeps the
+ * reference's signature and binds this library through ctypes.
+ *
+ * Conventions
+ *   - Every function returns int status: PTY_OK (0) or a negative code; the
+ *     message of the last failure on the calling thread is pty_last_error().
+ *   - Pointers named *_dev are device pointers (for example from torch
+ *     tensors on a ROCm device); *_host are host pointers.
+ *   - Complex data is interleaved float32 (pty_cfloat).  Frame stacks are
+ *     [N][M][K] row-major, the object [U][V], the probe [M][K].
+ *   - Amplitudes and masks are in the reference's ifftshifted order (DC at
+ *     [0][0], ptychography/era.py:497-499).
+ *   - A context is not thread-safe; all its work is ordered on one stream.
+ */
+#ifndef PTYHIP_H
+#define PTYHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pty_ctx pty_ctx;
+typedef struct { float re, im; } pty_cfloat;
+
+enum {
+  PTY_OK = 0,
+  PTY_EINVAL = -1,   /* bad argument: shape, pointer, positions outside the object */
+  PTY_ENOMEM = -2,   /* device allocation failed */
+  PTY_EHIP = -3,     /* a HIP runtime call or kernel launch failed */
+  PTY_ESTATE = -4,   /* call order: data / state not bound yet */
+  PTY_EUNSUP = -5    /* unsupported frame size */
+};
+
+/* Algorithm switches of one run (the keyword arguments of ERA / DM). */
+typedef struct pty_opts {
+  int32_t algorithm;          /* 0 = ERA (era.py:13), 1 = DM (dm.py:13) */
+  int32_t method;             /* 1..6 as resolved by the preamble (era.py:440-456) */
+  int32_t op_n, op_every;     /* OP_iters = (op_n, op_every) (era.py:458-459) */
+  int64_t pmod_probe_until;   /* Pmod_P while iteration < this; 0 = never (era.py:193) */
+  int32_t probe_centering;    /* era.py:172-186, dm.py:65-82 */
+  float alpha;                /* division regulariser (era.py:90) */
+} pty_opts;
+
+/* 1 if M or K = L is a supported frame side (16, 32, 64, 96, 128, 192, 256). */
+int pty_supported_frame(int32_t L);
+const char* pty_last_error(void);
+
+/* Geometry and workspace.  R_host holds the reference's R (N x 2, int32) AFTER
+ * the preamble's shift (era.py:504-505): R <= 0, frame n sees
+ * O[-R[n,0] : M - R[n,0], -R[n,1] : K - R[n,1]].  Every window must lie inside
+ * the U x V object.  The context allocates its scratch on the current device. */
+int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_t V,
+               const int32_t* R_host);
+int pty_destroy(pty_ctx* ctx);
+/* Run all work of this context on `stream` (a hipStream_t; NULL = default). */
+int pty_set_stream(pty_ctx* ctx, void* stream);
+
+/* amp_dev = ifftshift(sqrt(I)) from centred intensities I_dev (float32, N x M x K),
+ * and I_norm = sum(mask * I) with the centred mask (era.py:495-498).
+ * mask_centred_dev may be NULL (all good).  Synchronous (returns I_norm). */
+int pty_prepare_amp(pty_ctx* ctx, const float* I_dev, const uint8_t* mask_centred_dev,
+                    float* amp_dev, double* I_norm_out);
+
+/* Bind the modulus data: amplitudes (N x M x K float32, ifftshifted), the
+ * ifftshifted mask (M x K, 1 = good; NULL = all good) and I_norm. */
+int pty_set_data(pty_ctx* ctx, const float* amp_dev, const uint8_t* mask_dev, double I_norm);
+
+/* Bind the iterate: object (U x V) and probe (M x K), updated in place. */
+int pty_set_state(pty_ctx* ctx, pty_cfloat* O_dev, pty_cfloat* P_dev);
+
+/* Background state for methods 4-6: b = sqrt(ifftshift(B)) or the random
+ * start (era.py:539-544).  per_frame = 1: N x M x K values, 0: one M x K frame
+ * shared by all frames.  Copied into the context. */
+int pty_set_background(pty_ctx* ctx, const float* b_dev, int32_t per_frame);
+
+/* Far-field probe modulus |FFT(P)| for Pmod_P (era.py:546-547).  NULL: compute
+ * it now from the bound probe. */
+int pty_set_pamp(pty_ctx* ctx, const float* pamp_dev);
+
+/* Run `iters` iterations with global iteration numbers iter0, iter0+1, ...
+ * (they drive the OP_iters schedule and the Pmod_probe window).  begin = 1
+ * starts a new solve (DM: f = make_exits(O, P), eCon reference = O/P;
+ * heatmaps invalidated).  metrics_host receives iters x 3 doubles
+ * (eMod, eCon, eSup) per iteration; may be NULL.  Blocks until done. */
+int pty_run(pty_ctx* ctx, const pty_opts* opts, int64_t iter0, int32_t iters, int32_t begin,
+            double* metrics_host);
+
+/* fftshift(|fftn(exits)|^2) of the final exits into I_dev (N x M x K float32):
+ * ERA: make_exits(O, P) (era.py:293-295); DM: make_exits(Os, Ps) (dm.py:296-299). */
+int pty_intensity(pty_ctx* ctx, float* I_dev);
+
+/* Copy the cached P heatmap (U x V float64) to H_dev; *valid = 0 if none is
+ * cached (the reference's info['heatmap'] is then None). */
+int pty_heatmap(pty_ctx* ctx, double* H_dev, int32_t* valid);
+
+/* Current frame-averaged background (M x K float32, ifftshifted). */
+int pty_background(pty_ctx* ctx, float* b_dev);
+
+/* ---- per-stage timing (HIP events on the context stream) ---- */
+enum {
+  PTY_STAGE_ROWS_FWD = 0,  /* frame formation + forward row FFT */
+  PTY_STAGE_COLS = 1,      /* column FFT + modulus step + inverse column FFT */
+  PTY_STAGE_ROWS_INV = 2,  /* inverse row FFT (+ DM update) */
+  PTY_STAGE_GATHER = 3,    /* object update (overlap gather) */
+  PTY_STAGE_PROBE = 4,     /* probe update (frame reduction) */
+  PTY_STAGE_REDUCE = 5,    /* metric / background reductions */
+  PTY_STAGE_MISC = 6,      /* copies, centring, far-field probe constraint */
+  PTY_NSTAGE = 7
+};
+/* enable != 0 starts recording an event pair around every stage of pty_run
+ * and clears the totals; 0 stops. */
+int pty_profile(pty_ctx* ctx, int32_t enable);
+/* Totals since pty_profile(ctx, 1): milliseconds and launch counts per stage
+ * (arrays of PTY_NSTAGE). */
+int pty_profile_read(pty_ctx* ctx, double* ms_host, int64_t* count_host);
+
+/* ---- building blocks (used by the tests; same kernels as pty_run) ---- */
+
+/* In-place batched 2-D FFT of nframes M x K frames: forward unnormalised,
+ * inverse scaled by 1/(M K) (numpy.fft.fftn / ifftn over the last two axes). */
+int pty_fft2(pty_ctx* ctx, pty_cfloat* data_dev, int32_t nframes, int32_t inverse);
+
+/* exits = make_exits(O, P, R) (era.py:426-433). */
+int pty_make_exits(pty_ctx* ctx, const pty_cfloat* O_dev, const pty_cfloat* P_dev,
+                   pty_cfloat* exits_dev);
+
+/* exits = pmod_1(amp, exits, mask, alpha, eMod_calc=True) in place (era.py:550-572)
+ * with the bound data; *emod_host = the un-normalised eMod sum. */
+int pty_pmod1(pty_ctx* ctx, pty_cfloat* exits_dev, float alpha, double* emod_host);
+
+/* O_out = psup_O(exits, P, R, (U, V), H, alpha) (era.py:336-369).  fresh = 1
+ * recomputes the heatmap into H_dev (make_P_heatmap, era.py:402-410); fresh = 0
+ * reads it. */
+int pty_psup_O(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* P_dev,
+               pty_cfloat* O_out_dev, double* H_dev, int32_t fresh, float alpha);
+
+/* P_out = psup_P(exits, O, R, G, alpha) (era.py:371-400); G = O heatmap (M x K
+ * float64, make_O_heatmap, era.py:412-417), recomputed when fresh = 1. */
+int pty_psup_P(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* O_dev,
+               pty_cfloat* P_out_dev, double* G_dev, int32_t fresh, float alpha);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTYHIP_H */

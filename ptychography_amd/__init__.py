@@ -1,0 +1,12 @@
+"""MI355X-native ERA / difference-map ptychography.
+
+Drop-in for the ``Ptychography.ERA`` / ``Ptychography.DM`` call surface of
+andyofmelbourne/Ptychography (``__init__.py:3-4`` there), computed by
+hand-written gfx950 HIP kernels in ``libptyhip.so`` through a ctypes C ABI.
+"""
+
+from .era import ERA
+from .dm import DM
+from . import sim
+
+__all__ = ["ERA", "DM", "sim"]

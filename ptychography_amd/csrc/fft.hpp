@@ -1,0 +1,211 @@
+// Batched complex-to-complex FFT building blocks for gfx950.
+//
+// A line of L = R1*R2 points is transformed in two Stockham passes:
+//   pass 1: R2 butterflies of radix R1 on x[j + r*R2]              (registers)
+//   exchange through LDS (one write, one read, padded rows)
+//   pass 2: R1 butterflies of radix R2 on mid[j + r*R1] * W_L^(j*r) (registers)
+// giving X[j + r*R1] in natural order.  T threads share a line; every thread
+// keeps E = L/T points in registers.  The radix-R1/R2 DFTs are unrolled
+// register codelets built from radix 2, 3 and 4 with compile-time twiddles
+// (double-precision constants rounded once to float).  Pass-2 twiddles come
+// from a table of exp(-2 pi i k / L) computed in double on the host.
+//
+// Register layouts (t = thread within the line, q = butterfly slot):
+//   layoutA<Ra,Rb>: e = q*Ra + r  <->  element (t + q*T) + r*Rb   (q < Rb/T, r < Ra)
+//   layoutB<Ra,Rb>: e = q*Rb + r  <->  element (t + q*T) + r*Ra   (q < Ra/T, r < Rb)
+// The forward transform uses (Ra,Rb) = (R1,R2) and maps layoutA -> layoutB.
+// The inverse uses (R2,R1); its input layout is the forward's output layout,
+// so a forward -> pointwise -> inverse chain needs no extra data movement.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <utility>
+
+namespace pty {
+
+__device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 operator-(float2 a) { return make_float2(-a.x, -a.y); }
+__device__ __forceinline__ float2 operator*(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float norm2(float2 a) { return a.x * a.x + a.y * a.y; }
+
+// multiply by -i (forward sign) or +i (inverse sign)
+template <bool INV>
+__device__ __forceinline__ float2 mul_negi(float2 a) {
+  if constexpr (INV) return make_float2(-a.y, a.x);
+  else return make_float2(a.y, -a.x);
+}
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// cos / sin of 2*pi*k/48, k = 0..47 (every codelet radix divides 48)
+struct Tw48 {
+  static constexpr double c[48] = {
+      1.0, 0.991444861373810411145, 0.96592582628906828675, 0.923879532511286756128,
+      0.866025403784438646764, 0.79335334029123516458, 0.707106781186547524401,
+      0.608761429008720639416, 0.5, 0.382683432365089771728, 0.258819045102520762349,
+      0.130526192220051591548, 0.0, -0.130526192220051591548, -0.258819045102520762349,
+      -0.382683432365089771728, -0.5, -0.608761429008720639416, -0.707106781186547524401,
+      -0.79335334029123516458, -0.866025403784438646764, -0.923879532511286756128,
+      -0.96592582628906828675, -0.991444861373810411145, -1.0, -0.991444861373810411145,
+      -0.96592582628906828675, -0.923879532511286756128, -0.866025403784438646764,
+      -0.79335334029123516458, -0.707106781186547524401, -0.608761429008720639416, -0.5,
+      -0.382683432365089771728, -0.258819045102520762349, -0.130526192220051591548, 0.0,
+      0.130526192220051591548, 0.258819045102520762349, 0.382683432365089771728, 0.5,
+      0.608761429008720639416, 0.707106781186547524401, 0.79335334029123516458,
+      0.866025403784438646764, 0.923879532511286756128, 0.96592582628906828675,
+      0.991444861373810411145};
+  static constexpr double s(int k) { return c[(k + 36) % 48]; }  // sin(x) = cos(x - pi/2)
+};
+
+// a * W_R^m, W_R = exp(-2 pi i / R) (forward) or its conjugate (inverse)
+template <int R, int m, bool INV>
+__device__ __forceinline__ float2 twiddle(float2 a) {
+  constexpr int mm = ((m % R) + R) % R;
+  if constexpr (mm == 0) {
+    return a;
+  } else if constexpr (2 * mm == R) {
+    return -a;
+  } else if constexpr (4 * mm == R) {
+    return mul_negi<INV>(a);
+  } else if constexpr (4 * mm == 3 * R) {
+    return mul_negi<!INV>(a);
+  } else {
+    constexpr int k = mm * (48 / R);
+    constexpr float wc = (float)Tw48::c[k];
+    constexpr float ws = INV ? (float)Tw48::s(k) : -(float)Tw48::s(k);
+    return make_float2(a.x * wc - a.y * ws, a.x * ws + a.y * wc);
+  }
+}
+
+template <int R> struct Split;
+template <> struct Split<6>  { static constexpr int A = 2, B = 3; };
+template <> struct Split<8>  { static constexpr int A = 2, B = 4; };
+template <> struct Split<12> { static constexpr int A = 4, B = 3; };
+template <> struct Split<16> { static constexpr int A = 4, B = 4; };
+template <> struct Split<24> { static constexpr int A = 8, B = 3; };
+
+// In-register DFT of R points v[0..R-1] (natural order in, natural order out).
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    float2 a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if constexpr (R == 3) {
+    constexpr float h = 0.866025403784438646764f;  // sqrt(3)/2
+    float2 s = v[1] + v[2], d = v[1] - v[2];
+    float2 a = v[0];
+    v[0] = a + s;
+    float2 m = make_float2(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
+    // -i*h*d (forward) / +i*h*d (inverse)
+    float2 r = mul_negi<INV>(d) * h;
+    v[1] = m + r;
+    v[2] = m - r;
+  } else if constexpr (R == 4) {
+    float2 t0 = v[0] + v[2], t1 = v[0] - v[2];
+    float2 t2 = v[1] + v[3], t3 = mul_negi<INV>(v[1] - v[3]);
+    v[0] = t0 + t2;
+    v[2] = t0 - t2;
+    v[1] = t1 + t3;
+    v[3] = t1 - t3;
+  } else {
+    // R = A*B: x[B*n1 + n2] -> A-point DFTs over n1 -> twiddle W_R^(n2*k1)
+    //          -> B-point DFTs over n2 -> X[k1 + A*k2]
+    constexpr int A = Split<R>::A, B = Split<R>::B;
+    float2 y[R];
+    static_for<0, B>([&](auto n2c) {
+      constexpr int n2 = decltype(n2c)::value;
+      float2 s[A];
+      static_for<0, A>([&](auto n1c) { s[decltype(n1c)::value] = v[B * decltype(n1c)::value + n2]; });
+      dft<A, INV>(s);
+      static_for<0, A>([&](auto k1c) {
+        constexpr int k1 = decltype(k1c)::value;
+        y[n2 * A + k1] = twiddle<R, n2 * k1, INV>(s[k1]);
+      });
+    });
+    static_for<0, A>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      float2 s[B];
+      static_for<0, B>([&](auto n2c) { s[decltype(n2c)::value] = y[decltype(n2c)::value * A + k1]; });
+      dft<B, INV>(s);
+      static_for<0, B>([&](auto k2c) { v[k1 + A * decltype(k2c)::value] = s[decltype(k2c)::value]; });
+    });
+  }
+}
+
+// Line-FFT plan for one length: L = R1 * R2, T = threads per line.
+template <int L> struct Plan;
+template <> struct Plan<16>  { static constexpr int R1 = 4,  R2 = 4,  T = 4; };
+template <> struct Plan<32>  { static constexpr int R1 = 8,  R2 = 4,  T = 4; };
+template <> struct Plan<48>  { static constexpr int R1 = 12, R2 = 4,  T = 4; };
+template <> struct Plan<64>  { static constexpr int R1 = 8,  R2 = 8,  T = 8; };
+template <> struct Plan<96>  { static constexpr int R1 = 12, R2 = 8,  T = 4; };
+template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  T = 8; };
+template <> struct Plan<192> { static constexpr int R1 = 24, R2 = 8,  T = 8; };
+template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, T = 16; };
+
+// Element index held in register slot e by thread t, for layoutA / layoutB.
+template <int Ra, int Rb, int T>
+__device__ __forceinline__ int idxA(int e, int t) { return (t + (e / Ra) * T) + (e % Ra) * Rb; }
+template <int Ra, int Rb, int T>
+__device__ __forceinline__ int idxB(int e, int t) { return (t + (e / Rb) * T) + (e % Rb) * Ra; }
+
+// LDS words (float2) one line needs for its exchange buffer (both orders).
+template <int L>
+struct LineLds {
+  static constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2;
+  static constexpr int a = R2 * (R1 + 1), b = R1 * (R2 + 1);
+  static constexpr int words = (a > b ? a : b) | 1;  // odd stride between lines
+};
+
+// One two-pass transform of a line held in registers v[E] (layoutA<Ra,Rb> in,
+// layoutB<Ra,Rb> out).  `mid` is this line's LDS exchange area, `tw` the
+// table exp(-2 pi i k / L).  Contains one __syncthreads (between the passes);
+// the caller must put a barrier between two calls that reuse `mid`.
+template <int L, int Ra, int Rb, int T, bool INV>
+__device__ __forceinline__ void line_fft(float2* v, float2* mid, const float2* __restrict__ tw, int t) {
+  static_assert(Ra * Rb == L, "bad factorisation");
+  static_assert(Ra % T == 0 && Rb % T == 0, "T must divide both radices");
+  constexpr int E = L / T;
+  constexpr int NQ1 = Rb / T;  // pass-1 butterflies per thread
+  constexpr int NQ2 = Ra / T;  // pass-2 butterflies per thread
+  constexpr int PAD = Ra + 1;
+  static_for<0, NQ1>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    dft<Ra, INV>(v + q * Ra);
+    const int j = t + q * T;  // row of the exchange matrix
+#pragma unroll
+    for (int r = 0; r < Ra; ++r) mid[j * PAD + r] = v[q * Ra + r];
+  });
+  __syncthreads();
+  static_for<0, NQ2>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const int j = t + q * T;
+#pragma unroll
+    for (int r = 0; r < Rb; ++r) {
+      float2 a = mid[r * PAD + j];
+      float2 w = tw[j * r];
+      if constexpr (INV) w.y = -w.y;
+      v[q * Rb + r] = (r == 0) ? a : cmul(a, w);
+    }
+    dft<Rb, INV>(v + q * Rb);
+  });
+  static_assert(NQ1 * Ra == E && NQ2 * Rb == E, "layout size mismatch");
+}
+
+}  // namespace pty

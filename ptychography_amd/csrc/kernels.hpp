@@ -1,0 +1,719 @@
+// Device kernels of the ERA / DM iteration (gfx950, wave64).
+//
+// Frame stacks are [N][M][K] complex64, row-major; the object is [U][V];
+// the probe [M][K].  Frame n sees the object window starting at off[n] =
+// (y, x) >= 0 (the reference's R negated, ptychography/era.py:356).
+//
+// A 2-D frame transform is split into a row stage and a column stage; each
+// stage keeps whole lines in registers and exchanges once through LDS
+// (fft.hpp).  The per-pixel work of each ERA / DM step is fused into those
+// stages:
+//   rows forward : frame formation  psi = O_win * P  (or 2 O_win P - f, or f)
+//   columns      : forward column FFT -> modulus / error / background update
+//                  -> inverse column FFT (with FP64 error partials)
+//   rows inverse : inverse row FFT, optionally the DM update f += ...
+// Reductions are deterministic: every work-group writes its own partial, and
+// partials are summed in a fixed order by a single work-group.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fft.hpp"
+
+namespace pty {
+
+constexpr int WG = 256;
+
+struct Geom {
+  int N, M, K, U, V;
+  const int2* __restrict__ off;  // (y, x) window offset per frame
+};
+
+// ---------------------------------------------------------------------------
+// block reduction of doubles (wave64 shuffles, then LDS across waves)
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red /*[4*NV]*/) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[w * NV + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < nw; ++i) s += red[i * NV + k];
+      v[k] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row stage, forward
+// ---------------------------------------------------------------------------
+enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2 };
+
+// X[n, i, :] = FFT_row( source(n, i, :) ) for rows [0, nrows) of the stack.
+//   SRC_OP: O[y_n + i, x_n + c] * P[i, c]
+//   SRC_DM: 2 O[..] P[i, c] - F[n, i, c]
+//   SRC_F : F[n, i, c]                 (X may alias F)
+template <int L, int SRC>
+__global__ __launch_bounds__(WG, 4) void k_rows_fwd(Geom g, const float2* O, const float2* P,
+                                                 const float2* F, float2* X,
+                                                 const float2* __restrict__ tw, int nrows) {
+  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
+  constexpr int W = LineLds<L>::words;
+  __shared__ float2 lds[LPW * W];
+  const int tid = threadIdx.x, line = tid / T, t = tid % T;
+  const int row = blockIdx.x * LPW + line;
+  const bool valid = row < nrows;
+  const int M = g.M;
+  const int n = valid ? row / M : 0, i = valid ? row % M : 0;
+  float2 v[E];
+  if (valid) {
+    const size_t rbase = (size_t)row * L;
+    if constexpr (SRC == SRC_F) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = F[rbase + idxA<R1, R2, T>(e, t)];
+    } else {
+      const int2 o = g.off[n];
+      const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
+      const float2* Prow = P + (size_t)i * L;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = idxA<R1, R2, T>(e, t);
+        float2 psi = cmul(Orow[c], Prow[c]);
+        if constexpr (SRC == SRC_DM) {
+          float2 f = F[rbase + c];
+          psi = make_float2(2.f * psi.x - f.x, 2.f * psi.y - f.y);
+        }
+        v[e] = psi;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
+  }
+  line_fft<L, R1, R2, T, false>(v, lds + line * W, tw, t);
+  if (valid) {
+    const size_t rbase = (size_t)row * L;
+#pragma unroll
+    for (int e = 0; e < E; ++e) X[rbase + idxB<R1, R2, T>(e, t)] = v[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row stage, inverse
+// ---------------------------------------------------------------------------
+enum RowDst { DST_STORE = 0, DST_DMUPD = 1 };
+
+// y = scale * IFFT_row(Y[n, i, :]) (unnormalised inverse, times scale), then
+//   DST_STORE: Out[n, i, c] = y                         (Out may alias Y)
+//   DST_DMUPD: Out[n, i, c] += y - O[..] P[i, c]        (DM: f - Ps f + Pm(..))
+//              partial[block] = sum |y - O P|^2          (DM step size)
+template <int L, int DST>
+__global__ __launch_bounds__(WG, 4) void k_rows_inv(Geom g, const float2* Y, float2* Out,
+                                                 const float2* O, const float2* P, float scale,
+                                                 double* partial, const float2* __restrict__ tw,
+                                                 int nrows) {
+  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
+  constexpr int W = LineLds<L>::words;
+  __shared__ float2 lds[LPW * W];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, line = tid / T, t = tid % T;
+  const int row = blockIdx.x * LPW + line;
+  const bool valid = row < nrows;
+  const size_t rbase = (size_t)row * L;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    v[e] = valid ? Y[rbase + idxA<R2, R1, T>(e, t)] : make_float2(0.f, 0.f);
+  line_fft<L, R2, R1, T, true>(v, lds + line * W, tw, t);
+  double acc[1] = {0.0};
+  if (valid) {
+    if constexpr (DST == DST_STORE) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) Out[rbase + idxB<R2, R1, T>(e, t)] = v[e] * scale;
+    } else {
+      const int M = g.M, n = row / M, i = row % M;
+      const int2 o = g.off[n];
+      const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
+      const float2* Prow = P + (size_t)i * L;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = idxB<R2, R1, T>(e, t);
+        const float2 ex = cmul(Orow[c], Prow[c]);
+        const float2 y = v[e] * scale;
+        const float2 d = y - ex;
+        acc[0] += (double)norm2(d);
+        Out[rbase + c] = Out[rbase + c] + d;
+      }
+    }
+  }
+  if constexpr (DST == DST_DMUPD) {
+    block_sum<1>(acc, red);
+    if (tid == 0) partial[blockIdx.x] = acc[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column stage
+// ---------------------------------------------------------------------------
+enum ColMode {
+  COL_ERA1 = 0,   // Pmod_1 with eMod (era.py:560-572) + sum |psi_hat'|^2
+  COL_DM1,        // Pmod_1 without eMod
+  COL_ERR1,       // model_error_1, no background (dm.py:337-341); forward only
+  COL_ERR7,       // model_error_1 with mean background bbar; forward only
+  COL_INTENS,     // fftshift(|psi_hat|^2) -> outI; forward only
+  COL_PMODP,      // Pmod_P factor (era.py:597-610) with amp = |FFT P0|
+  COL_ERAB,       // Pmod_7 with eMod (era.py:583-595); sums b*fac over frames
+  COL_DMB_LEAD,   // Pmod_7 on (f, b_n), b_n updated in place; sums new b_n
+  COL_DMB_UPD,    // DM background step: b0 = 2 bbar - b_n, Pmod_7, b_n += b0' - bbar
+  COL_FWD,        // plain forward column FFT (2-D FFT building block)
+  COL_INV,        // plain inverse column FFT (input in spectrum order), times scale
+};
+
+struct ColArgs {
+  float2* X;
+  const float* amp;
+  long amp_stride;       // elements between frames of amp (0 = one shared frame)
+  const uint8_t* mask;   // [M][K], 1 = good, ifftshifted; null = all good
+  float alpha;
+  float scale;           // applied to the spectrum before the inverse (1/(M K))
+  float* bg;             // background b (per frame or shared)
+  long bg_stride;
+  const float* bbar;     // [M][K] frame-mean background
+  float* outI;           // [N][M][K] intensities (COL_INTENS)
+  double* part;          // [gridDim.y * gridDim.x] eMod / error partials
+  double* part2;         // [gridDim.y * gridDim.x] sum |psi_hat'|^2 partials
+  double* partb;         // [gridDim.y][M*K] background sums over the chunk's frames
+  int nframes, chunk;
+  const float2* __restrict__ tw;
+};
+
+template <int MODE> struct ColTraits {
+  static constexpr bool fwd = MODE != COL_INV;
+  static constexpr bool inv = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP ||
+                              MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
+                              MODE == COL_DMB_UPD || MODE == COL_INV;
+  static constexpr bool store_fwd = MODE == COL_FWD;
+  static constexpr bool bsum = MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD;
+  static constexpr bool err = MODE == COL_ERA1 || MODE == COL_ERR1 || MODE == COL_ERR7 ||
+                              MODE == COL_ERAB;
+};
+
+// Uniform part of the row index held in register slot e (layoutA / layoutB);
+// the per-thread part is t for both layouts.
+template <int Ra, int Rb, int T>
+__device__ __forceinline__ constexpr int urowA(int e) { return (e / Ra) * T + (e % Ra) * Rb; }
+template <int Ra, int Rb, int T>
+__device__ __forceinline__ constexpr int urowB(int e) { return (e / Rb) * T + (e % Rb) * Ra; }
+
+template <int L, int MODE>
+__global__ __launch_bounds__(WG, 3) void k_cols(Geom g, ColArgs a) {
+  using TR = ColTraits<MODE>;
+  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = WG / T;
+  constexpr int W = LineLds<L>::words;
+  __shared__ float2 lds[CPW * W];
+  __shared__ double red[8];
+  const int tid = threadIdx.x, l = tid % CPW, t = tid / CPW;
+  const int K = g.K;
+  const int c = blockIdx.x * CPW + l;
+  const bool cv = c < K;
+  const int n0 = blockIdx.y * a.chunk;
+  const int n1 = min(a.nframes, n0 + a.chunk);
+  // element (row t + u, column c) of a frame is at [u*K + toff]: u*K is uniform
+  const uint32_t toff = (uint32_t)(t * K + c);
+  float2* mid = lds + l * W;
+  double acc[2] = {0.0, 0.0};
+  double bacc[TR::bsum ? E : 1];
+#pragma unroll
+  for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0.0;
+
+  for (int n = n0; n < n1; ++n) {
+    // opaque, wave-uniform frame index: keeps the frame base in SGPRs instead
+    // of per-element 64-bit induction variables
+    const int nu = __builtin_amdgcn_readfirstlane(n);
+    float2* Xn = a.X + (size_t)nu * L * K;
+    float2 v[E];
+    if (cv) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int u = TR::fwd ? urowA<R1, R2, T>(e) : urowB<R1, R2, T>(e);
+        v[e] = (Xn + (size_t)u * K)[toff];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
+    }
+    if constexpr (TR::fwd) line_fft<L, R1, R2, T, false>(v, mid, a.tw, t);
+
+    if constexpr (MODE != COL_FWD && MODE != COL_INV) {
+      if (cv) {
+        const float* ampn = a.amp ? a.amp + (size_t)nu * a.amp_stride : nullptr;
+        float* bgn = a.bg ? a.bg + (size_t)nu * a.bg_stride : nullptr;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int u = urowB<R1, R2, T>(e);
+          const size_t uo = (size_t)u * K;
+          const float2 z = v[e];
+          const float z2 = norm2(z);
+          if constexpr (MODE == COL_INTENS) {
+            const int r = t + u;
+            const int rs = (r + L / 2) % L, cs = (c + K / 2) % K;
+            a.outI[(size_t)nu * L * K + (size_t)rs * K + cs] = z2;
+          } else {
+            const float m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;
+            const float am = (ampn + uo)[toff];
+            if constexpr (MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP) {
+              const float mg = sqrtf(z2) + a.alpha;
+              if constexpr (MODE == COL_ERA1) {
+                const float d = mg - am;
+                acc[0] += (double)(m * d * d);
+              }
+              const float fac = (m != 0.f) ? am / mg : 1.f;
+              const float2 zn = z * fac;
+              if constexpr (MODE == COL_ERA1) acc[1] += (double)norm2(zn);
+              v[e] = zn * a.scale;
+            } else if constexpr (MODE == COL_ERR1) {
+              const float d = sqrtf(z2) - am;
+              acc[0] += (double)(m * d * d);
+            } else if constexpr (MODE == COL_ERR7) {
+              const float b = (a.bbar + uo)[toff];
+              const float d = sqrtf(z2 + b * b) - am;
+              acc[0] += (double)(m * d * d);
+            } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
+              float* bp = (bgn + uo) + toff;
+              const float b = *bp;
+              const float mg = sqrtf(z2 + b * b + a.alpha);
+              if constexpr (MODE == COL_ERAB) {
+                const float d = mg - am;
+                acc[0] += (double)(m * d * d);
+              }
+              const float fac = m * am / mg;
+              const float bn = b * fac;
+              bacc[e] += (double)bn;
+              if constexpr (MODE == COL_DMB_LEAD) *bp = bn;
+              const float2 zn = z * fac;
+              if constexpr (MODE == COL_ERAB) acc[1] += (double)norm2(zn);
+              v[e] = zn * a.scale;
+            } else if constexpr (MODE == COL_DMB_UPD) {
+              float* bp = (bgn + uo) + toff;
+              const float b = *bp;
+              const float bb = (a.bbar + uo)[toff];
+              const float b0 = 2.f * bb - b;
+              const float mg = sqrtf(z2 + b0 * b0 + a.alpha);
+              const float fac = m * am / mg;
+              const float bn = (b - bb) + b0 * fac;
+              *bp = bn;
+              bacc[e] += (double)bn;
+              v[e] = z * (fac * a.scale);
+            }
+          }
+        }
+      }
+    }
+
+    if constexpr (TR::store_fwd) {
+      if (cv) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) (Xn + (size_t)urowB<R1, R2, T>(e) * K)[toff] = v[e];
+      }
+    }
+    if constexpr (TR::inv) {
+      if constexpr (TR::fwd) __syncthreads();
+      if constexpr (MODE == COL_INV) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
+      }
+      line_fft<L, R2, R1, T, true>(v, mid, a.tw, t);
+      if (cv) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) (Xn + (size_t)urowA<R1, R2, T>(e) * K)[toff] = v[e];
+      }
+    }
+    __syncthreads();
+  }
+
+  if constexpr (TR::bsum) {
+    if (cv) {
+      double* pb = a.partb + (size_t)blockIdx.y * L * K;
+#pragma unroll
+      for (int e = 0; e < E; ++e) (pb + (size_t)urowB<R1, R2, T>(e) * K)[toff] = bacc[e];
+    }
+  }
+  if constexpr (TR::err) {
+    block_sum<2>(acc, red);
+    if (tid == 0) {
+      const int b = blockIdx.y * gridDim.x + blockIdx.x;
+      if (a.part) a.part[b] = acc[0];
+      if (a.part2) a.part2[b] = acc[1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// deterministic overlap gather: object update (psup_O, era.py:336-369)
+// ---------------------------------------------------------------------------
+constexpr int TILE_H = 16, TILE_W = 64;
+
+struct GatherArgs {
+  const float2* S;         // frames psi' (or f): [N][M][K]
+  const float2* P;         // probe [M][K]
+  const float2* Oold;      // previous object, for the fused eCon (null = skip)
+  float2* Oout;            // new object [U][V] (may alias Oold)
+  double* H;               // P heatmap [U][V], FP64
+  int fresh;               // 1: accumulate sum |P|^2 here instead of reading H
+  int storeH;              // with fresh: write the new heatmap to H
+  float alpha;
+  const int* __restrict__ tile_ptr;     // CSR over object tiles
+  const int* __restrict__ tile_frames;  // ascending frame ids per tile
+  int tiles_x;
+  double* part_econ;       // [2][nblocks]  sum |Oold - O|^2, sum |O|^2
+  double* part_esup;       // [2][nblocks]  sum Re(conj(O) num), sum |O|^2 H
+};
+
+// Each object pixel sums conj(P) * S_n over the frames whose window covers it,
+// in ascending frame order, then divides by (sum |P|^2 + alpha) in FP64.
+__global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x, tx = tid & (TILE_W - 1), ty = tid / TILE_W;
+  constexpr int PY = TILE_H / (WG / TILE_W);  // pixels per thread (rows)
+  const int tile = blockIdx.x;
+  const int u0 = (tile / a.tiles_x) * TILE_H, v = (tile % a.tiles_x) * TILE_W + tx;
+  const int M = g.M, K = g.K;
+  float2 acc[PY];
+  double hacc[PY];
+#pragma unroll
+  for (int k = 0; k < PY; ++k) { acc[k] = make_float2(0.f, 0.f); hacc[k] = 0.0; }
+  const int b0 = a.tile_ptr[tile], b1 = a.tile_ptr[tile + 1];
+  for (int b = b0; b < b1; ++b) {
+    const int n = a.tile_frames[b];
+    const int2 o = g.off[n];
+    const int j = v - o.y;
+    if ((unsigned)j >= (unsigned)K) continue;
+    const float2* Sn = a.S + (size_t)n * M * K + j;
+    const float2* Pj = a.P + j;
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+      const int i = u0 + ty + k * (WG / TILE_W) - o.x;
+      if ((unsigned)i < (unsigned)M) {
+        const float2 p = Pj[(size_t)i * K];
+        const float2 s = Sn[(size_t)i * K];
+        acc[k] = acc[k] + cmulc(s, p);
+        if (a.fresh) hacc[k] += (double)norm2(p);
+      }
+    }
+  }
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < PY; ++k) {
+    const int u = u0 + ty + k * (WG / TILE_W);
+    if (u < g.U && v < g.V) {
+      const size_t pix = (size_t)u * g.V + v;
+      const double h = a.fresh ? hacc[k] : a.H[pix];
+      if (a.fresh && a.storeH) a.H[pix] = h;
+      const double den = h + (double)a.alpha;
+      const double ox = (double)acc[k].x / den, oy = (double)acc[k].y / den;
+      const float2 on = make_float2((float)ox, (float)oy);
+      if (a.Oold) {
+        const float2 od = a.Oold[pix];
+        const double dx = (double)od.x - (double)on.x, dy = (double)od.y - (double)on.y;
+        r[0] += dx * dx + dy * dy;
+        r[1] += (double)on.x * on.x + (double)on.y * on.y;
+      }
+      if (a.part_esup) {
+        r[2] += (double)on.x * acc[k].x + (double)on.y * acc[k].y;
+        r[3] += ((double)on.x * on.x + (double)on.y * on.y) * h;
+      }
+      a.Oout[pix] = on;
+    }
+  }
+  if (a.part_econ || a.part_esup) {
+    block_sum<4>(r, red);
+    if (tid == 0) {
+      const int nb = gridDim.x;
+      if (a.part_econ) { a.part_econ[blockIdx.x] = r[0]; a.part_econ[nb + blockIdx.x] = r[1]; }
+      if (a.part_esup) { a.part_esup[blockIdx.x] = r[2]; a.part_esup[nb + blockIdx.x] = r[3]; }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// probe update (psup_P, era.py:371-400): reduction over frames in two stages
+// ---------------------------------------------------------------------------
+struct ProbeArgs {
+  const float2* S;   // frames [N][M][K]
+  const float2* O;   // object [U][V]
+  double2* pnum;     // [nchunks][M*K]  sum conj(O_n) S_n
+  double* pg;        // [nchunks][M*K]  sum |O_n|^2 (fresh only)
+  int fresh;
+  int chunk;
+};
+
+__global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
+  const int MK = g.M * g.K;
+  const int p = blockIdx.x * WG + threadIdx.x;
+  if (p >= MK) return;
+  const int i = p / g.K, j = p % g.K;
+  const int n0 = blockIdx.y * a.chunk, n1 = min(g.N, n0 + a.chunk);
+  double nx = 0.0, ny = 0.0, gg = 0.0;
+  for (int n = n0; n < n1; ++n) {
+    const int2 o = g.off[n];
+    const float2 ov = a.O[(size_t)(o.x + i) * g.V + o.y + j];
+    const float2 s = a.S[(size_t)n * MK + p];
+    const float2 q = cmulc(s, ov);
+    nx += (double)q.x;
+    ny += (double)q.y;
+    if (a.fresh) gg += (double)norm2(ov);
+  }
+  a.pnum[(size_t)blockIdx.y * MK + p] = make_double2(nx, ny);
+  if (a.fresh) a.pg[(size_t)blockIdx.y * MK + p] = gg;
+}
+
+struct ProbeFinalArgs {
+  const double2* pnum;
+  const double* pg;
+  int nchunks;
+  int fresh, storeG;
+  double* G;          // O heatmap [M][K]
+  float alpha;
+  const float2* Pold; // for the fused eCon (null = skip)
+  float2* Pout;
+  double* part_econ;  // [2][nblocks]
+};
+
+__global__ __launch_bounds__(WG) void k_probe_final(Geom g, ProbeFinalArgs a) {
+  __shared__ double red[8];
+  const int MK = g.M * g.K;
+  const int p = blockIdx.x * WG + threadIdx.x;
+  double r[2] = {0.0, 0.0};
+  if (p < MK) {
+    double nx = 0.0, ny = 0.0, gg = 0.0;
+    for (int c = 0; c < a.nchunks; ++c) {
+      const double2 q = a.pnum[(size_t)c * MK + p];
+      nx += q.x;
+      ny += q.y;
+      if (a.fresh) gg += a.pg[(size_t)c * MK + p];
+    }
+    if (!a.fresh) gg = a.G[p];
+    else if (a.storeG) a.G[p] = gg;
+    const double den = gg + (double)a.alpha;
+    const float2 pn = make_float2((float)(nx / den), (float)(ny / den));
+    if (a.Pold) {
+      const float2 po = a.Pold[p];
+      const double dx = (double)po.x - pn.x, dy = (double)po.y - pn.y;
+      r[0] = dx * dx + dy * dy;
+      r[1] = (double)pn.x * pn.x + (double)pn.y * pn.y;
+    }
+    a.Pout[p] = pn;
+  }
+  if (a.part_econ) {
+    block_sum<2>(r, red);
+    if (threadIdx.x == 0) {
+      a.part_econ[blockIdx.x] = r[0];
+      a.part_econ[gridDim.x + blockIdx.x] = r[1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+// out[n, i, j] = O[y_n + i, x_n + j] * P[i, j]   (make_exits, era.py:426-433)
+__global__ __launch_bounds__(WG) void k_make_exits(Geom g, const float2* O, const float2* P,
+                                                   float2* out) {
+  const size_t MK = (size_t)g.M * g.K, total = MK * g.N;
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const int n = (int)(idx / MK);
+    const int p = (int)(idx % MK), i = p / g.K, j = p % g.K;
+    const int2 o = g.off[n];
+    out[idx] = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[p]);
+  }
+}
+
+// partials of sum |psi'_n - O_n P|^2 (support error on an explicit pass)
+__global__ __launch_bounds__(WG) void k_esup(Geom g, const float2* S, const float2* O,
+                                             const float2* P, double* part) {
+  __shared__ double red[4];
+  const size_t MK = (size_t)g.M * g.K, total = MK * g.N;
+  double r[1] = {0.0};
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const int n = (int)(idx / MK);
+    const int p = (int)(idx % MK), i = p / g.K, j = p % g.K;
+    const int2 o = g.off[n];
+    const float2 d = S[idx] - cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[p]);
+    r[0] += (double)norm2(d);
+  }
+  block_sum<1>(r, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+// partials of sum |bak - cur|^2 and sum |cur|^2 (convergence error)
+__global__ __launch_bounds__(WG) void k_econ(const float2* cur, const float2* bak, size_t count,
+                                             double* part) {
+  __shared__ double red[8];
+  double r[2] = {0.0, 0.0};
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < count;
+       idx += (size_t)gridDim.x * WG) {
+    const float2 c = cur[idx], b = bak[idx];
+    const double dx = (double)b.x - c.x, dy = (double)b.y - c.y;
+    r[0] += dx * dx + dy * dy;
+    r[1] += (double)c.x * c.x + (double)c.y * c.y;
+  }
+  block_sum<2>(r, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = r[0];
+    part[gridDim.x + blockIdx.x] = r[1];
+  }
+}
+
+// centre of mass of |P|^2 -> cm = rint(com) - shape // 2   (era.py:172-176)
+__global__ __launch_bounds__(WG) void k_com(const float2* P, int M, int K, int* cm) {
+  __shared__ double red[12];
+  double r[3] = {0.0, 0.0, 0.0};
+  for (int p = threadIdx.x; p < M * K; p += WG) {
+    const double a = (double)norm2(P[p]);
+    r[0] += a;
+    r[1] += a * (double)(p / K);
+    r[2] += a * (double)(p % K);
+  }
+  block_sum<3>(r, red);
+  if (threadIdx.x == 0) {
+    if (r[0] > 0.0) {
+      cm[0] = (int)rint(r[1] / r[0]) - M / 2;
+      cm[1] = (int)rint(r[2] / r[0]) - K / 2;
+    } else {
+      cm[0] = 0;
+      cm[1] = 0;
+    }
+  }
+}
+
+// dst[b, i, j] = src[b, (i + cm0) mod rows, (j + cm1) mod cols]  (roll by -cm)
+__global__ __launch_bounds__(WG) void k_roll(const float2* src, float2* dst, int batch, int rows,
+                                             int cols, const int* cm) {
+  const size_t plane = (size_t)rows * cols, total = plane * batch;
+  const int s0 = ((cm[0] % rows) + rows) % rows, s1 = ((cm[1] % cols) + cols) % cols;
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const size_t b = idx / plane;
+    const int p = (int)(idx % plane), i = p / cols, j = p % cols;
+    int is = i + s0; if (is >= rows) is -= rows;
+    int js = j + s1; if (js >= cols) js -= cols;
+    dst[idx] = src[b * plane + (size_t)is * cols + js];
+  }
+}
+
+// bbar[p] = (sum over chunks of partb[c][p]) / N
+__global__ __launch_bounds__(WG) void k_bg_mean(const double* partb, int nchunks, int MK, int N,
+                                                float* bbar) {
+  const int p = blockIdx.x * WG + threadIdx.x;
+  if (p >= MK) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; ++c) s += partb[(size_t)c * MK + p];
+  bbar[p] = (float)(s / (double)N);
+}
+
+// amp[n, i, j] = sqrt(I[n, (i + M/2) % M, (j + K/2) % K]) (ifftshift, era.py:497-498)
+// and partials of sum mask * I with the unshifted mask (era.py:495).
+__global__ __launch_bounds__(WG) void k_prepare_amp(const float* I, int N, int M, int K,
+                                                    const uint8_t* mask_centred, float* amp,
+                                                    double* part) {
+  __shared__ double red[4];
+  const size_t MK = (size_t)M * K, total = MK * N;
+  double r[1] = {0.0};
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const size_t n = idx / MK;
+    const int p = (int)(idx % MK), i = p / K, j = p % K;
+    const int is = (i + M / 2) % M, js = (j + K / 2) % K;
+    const float v = I[n * MK + (size_t)is * K + js];
+    amp[idx] = sqrtf(v);
+    const float m = mask_centred ? (float)mask_centred[(size_t)is * K + js] : 1.f;
+    r[0] += (double)m * (double)v;
+  }
+  block_sum<1>(r, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+__global__ __launch_bounds__(WG) void k_abs(const float2* X, float* out, size_t count) {
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < count;
+       idx += (size_t)gridDim.x * WG)
+    out[idx] = sqrtf(norm2(X[idx]));
+}
+
+// ---------------------------------------------------------------------------
+// per-iteration metrics: fixed-order sums of the partial regions
+// ---------------------------------------------------------------------------
+enum Slot {
+  S_EMOD = 0,   // sum mask (Mg - amp)^2
+  S_ECON_ON,    // sum |Obak - O|^2
+  S_ECON_OD,    // sum |O|^2
+  S_ECON_PN,    // sum |Pbak - P|^2
+  S_ECON_PD,    // sum |P|^2
+  S_ESUP_E,     // sum |psi' - P O_n|^2 (explicit, real space) or DM step
+  S_ESUP_A,     // sum |psi_hat'|^2 (spectrum after the modulus step)
+  S_ESUP_X,     // sum Re(conj(O) num)
+  S_ESUP_H,     // sum |O|^2 H
+  NSLOT
+};
+
+struct Regions {
+  const double* p[NSLOT];
+  int n[NSLOT];
+};
+
+enum EsupMode { ESUP_NONE = 0, ESUP_EXPLICIT = 1, ESUP_IDENTITY = 2 };
+
+// metrics[0..2] = eMod, eCon, eSup for one iteration
+__global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, double MK,
+                                                  int esup_mode, double* metrics) {
+  __shared__ double red[16 * NSLOT];
+  double s[NSLOT];
+#pragma unroll
+  for (int k = 0; k < NSLOT; ++k) {
+    double acc = 0.0;
+    if (rg.p[k])
+      for (int i = threadIdx.x; i < rg.n[k]; i += blockDim.x) acc += rg.p[k][i];
+    s[k] = acc;
+  }
+  block_sum<NSLOT>(s, red);
+  if (threadIdx.x == 0) {
+    metrics[0] = sqrt(s[S_EMOD] / I_norm);
+    const double den = s[S_ECON_OD] + s[S_ECON_PD];
+    metrics[1] = sqrt((s[S_ECON_ON] + s[S_ECON_PN]) / den);
+    double es = 0.0;
+    if (esup_mode == ESUP_EXPLICIT) {
+      es = sqrt(MK * s[S_ESUP_E] / I_norm);
+    } else if (esup_mode == ESUP_IDENTITY) {
+      const double q = s[S_ESUP_A] - MK * (2.0 * s[S_ESUP_X] - s[S_ESUP_H]);
+      es = sqrt((q > 0.0 ? q : 0.0) / I_norm);
+    } else {
+      es = __builtin_nan("");
+    }
+    metrics[2] = es;
+  }
+}
+
+// plain sum of one region into out[0]
+__global__ __launch_bounds__(1024) void k_sum(const double* p, int n, double* out) {
+  __shared__ double red[16];
+  double s[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s[0] += p[i];
+  block_sum<1>(s, red);
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+}  // namespace pty

@@ -1,0 +1,1045 @@
+// ptyhip: host engine and C ABI of the gfx950 ERA / DM iteration.
+//
+// One context = one scan geometry (N frames of M x K, object U x V, integer
+// window offsets) plus the workspace its kernels need.  pty_run executes the
+// reference's loop (ptychography/era.py:152-291, ptychography/dm.py:45-294) as
+// a sequence of kernel launches on the context's stream, keeping every array
+// on the device; per-iteration metrics are written to device memory and
+// copied back once per call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "ptyhip.h"
+
+using namespace pty;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define PTY_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(PTY_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+#define PTY_LAUNCHED(name)                                                             \
+  do {                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess)                                                              \
+      return fail(PTY_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define PTY_TRY(expr)        \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != PTY_OK) return r_; \
+  } while (0)
+
+#define PTY_SIZES(X) X(16) X(32) X(64) X(96) X(128) X(192) X(256)
+
+int threads_per_line(int L) {
+  switch (L) {
+#define C(Lv) case Lv: return Plan<Lv>::T;
+    PTY_SIZES(C)
+#undef C
+    default: return 0;
+  }
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(PTY_ENOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                " bytes: " + hipGetErrorString(e));
+  }
+  return PTY_OK;
+}
+
+template <class T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+constexpr int ECON_BLOCKS = 512;
+constexpr int ESUP_BLOCKS = 1024;
+
+}  // namespace
+
+struct pty_ctx {
+  int N = 0, M = 0, K = 0, U = 0, V = 0;
+  hipStream_t stream = nullptr;
+  Geom g{};
+  int2* off = nullptr;
+  int tiles_x = 0, tiles_y = 0, ntiles = 0;
+  int* tile_ptr = nullptr;
+  int* tile_frames = nullptr;
+  float2* twM = nullptr;
+  float2* twK = nullptr;
+  // column-stage launch shape
+  int col_blocks = 0, col_chunk = 1, col_chunks = 0;
+  // probe-reduction launch shape
+  int probe_chunk = 1, probe_chunks = 0;
+
+  // bound (caller-owned)
+  const float* amp = nullptr;
+  const uint8_t* mask = nullptr;
+  double I_norm = 0.0;
+  float2* O = nullptr;
+  float2* P = nullptr;
+
+  // owned workspace
+  float2* X = nullptr;       // N*M*K frame scratch (ERA: psi')
+  float2* F = nullptr;       // N*M*K DM iterate f
+  float2* Xp = nullptr;      // M*K probe scratch
+  float2* Obak = nullptr;    // U*V
+  float2* Pbak = nullptr;    // M*K
+  float2* Os = nullptr;      // U*V DM metric object
+  float2* Oroll = nullptr;   // U*V scratch for the periodic object roll
+  float2* Ps = nullptr;      // M*K DM metric probe / roll scratch
+  double* H = nullptr;       // U*V P heatmap
+  double* G = nullptr;       // M*K O heatmap
+  bool H_valid = false, G_valid = false;
+  float* bg = nullptr;       // N*M*K background state
+  bool bg_set = false;
+  float* bbar = nullptr;     // M*K main-path frame mean
+  float* bbar_m = nullptr;   // M*K metric-path frame mean
+  bool era_bg_shared = false;  // ERA: the state collapsed to bbar after the first mean
+  float* pamp = nullptr;     // M*K |FFT P0|
+  bool pamp_set = false;
+  double* partb = nullptr;   // col_chunks * M*K background sums
+  double2* pnum = nullptr;   // probe_chunks * M*K
+  double* pg = nullptr;      // probe_chunks * M*K
+  int* cm = nullptr;         // probe centre shift
+  double* metrics = nullptr;
+  int metrics_cap = 0;
+  double* scal = nullptr;    // scalar results
+  // partial pool
+  double* part_emod = nullptr;   // col WGs
+  double* part_esa = nullptr;    // col WGs
+  double* part_econO = nullptr;  // 2 * max(ntiles, ECON_BLOCKS)
+  double* part_econP = nullptr;  // 2 * max(MK/256, ECON_BLOCKS)
+  double* part_esup = nullptr;   // max(ESUP_BLOCKS, row blocks)
+  double* part_esxh = nullptr;   // 2 * ntiles
+  int cap_col = 0, cap_econO = 0, cap_econP = 0, cap_esup = 0;
+
+  // optional per-stage timing with HIP events on the context stream
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;   // pairs (start, stop)
+  std::vector<int> prof_stage;       // stage of each used pair
+  size_t prof_used = 0;
+  int prof_override = -1;            // >= 0: attribute launches to this stage
+  double prof_ms[PTY_NSTAGE] = {};
+  int64_t prof_cnt[PTY_NSTAGE] = {};
+
+  // DM bookkeeping
+  bool dm_ready = false;
+  bool have_metric_state = false;
+  int last_algo = -1;
+
+  size_t MK() const { return (size_t)M * K; }
+  size_t NMK() const { return (size_t)N * M * K; }
+};
+
+namespace {
+
+// Brackets the launches of one stage with a pair of events when profiling.
+struct StageTimer {
+  pty_ctx* c;
+  long idx = -1;
+  StageTimer(pty_ctx* ctx, int stage) : c(ctx) {
+    if (!c->prof_on) return;
+    if (c->prof_override >= 0) {
+      if (c->prof_override != stage && stage != PTY_STAGE_MISC) return;  // outer timer covers it
+      stage = c->prof_override;
+    }
+    if (c->prof_used * 2 + 2 > c->prof_ev.size()) {
+      for (int k = 0; k < 2; ++k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        c->prof_ev.push_back(e);
+      }
+      c->prof_stage.push_back(stage);
+    }
+    idx = (long)c->prof_used++;
+    c->prof_stage[idx] = stage;
+    (void)hipEventRecord(c->prof_ev[2 * idx], c->stream);
+  }
+  ~StageTimer() {
+    if (idx >= 0) (void)hipEventRecord(c->prof_ev[2 * idx + 1], c->stream);
+  }
+};
+
+// after a stream sync: fold the recorded pairs into the per-stage totals
+void prof_collect(pty_ctx* c) {
+  for (size_t k = 0; k < c->prof_used; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->prof_ev[2 * k], c->prof_ev[2 * k + 1]) == hipSuccess) {
+      c->prof_ms[c->prof_stage[k]] += ms;
+      c->prof_cnt[c->prof_stage[k]] += 1;
+    }
+  }
+  c->prof_used = 0;
+}
+
+// --------------------------------------------------------------------------
+// launch helpers
+// --------------------------------------------------------------------------
+
+template <int SRC>
+int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows) {
+  StageTimer tm(c, PTY_STAGE_ROWS_FWD);
+  const int T = threads_per_line(c->K), lpw = WG / T;
+  dim3 grid((nrows + lpw - 1) / lpw);
+  switch (c->K) {
+#define C(Lv)                                                                              \
+  case Lv:                                                                                 \
+    hipLaunchKernelGGL((k_rows_fwd<Lv, SRC>), grid, dim3(WG), 0, c->stream, c->g, O, P, F, X, \
+                       (const float2*)c->twK, nrows);                                      \
+    break;
+    PTY_SIZES(C)
+#undef C
+    default: return fail(PTY_EUNSUP, "unsupported K");
+  }
+  PTY_LAUNCHED("k_rows_fwd");
+  return PTY_OK;
+}
+
+template <int DST>
+int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const float2* P,
+             float scale, double* partial, int nrows) {
+  StageTimer tm(c, PTY_STAGE_ROWS_INV);
+  const int T = threads_per_line(c->K), lpw = WG / T;
+  dim3 grid((nrows + lpw - 1) / lpw);
+  switch (c->K) {
+#define C(Lv)                                                                               \
+  case Lv:                                                                                  \
+    hipLaunchKernelGGL((k_rows_inv<Lv, DST>), grid, dim3(WG), 0, c->stream, c->g, Y, Out, O, P, \
+                       scale, partial, (const float2*)c->twK, nrows);                       \
+    break;
+    PTY_SIZES(C)
+#undef C
+    default: return fail(PTY_EUNSUP, "unsupported K");
+  }
+  PTY_LAUNCHED("k_rows_inv");
+  return PTY_OK;
+}
+
+int row_blocks(pty_ctx* c, int nrows) {
+  const int lpw = WG / threads_per_line(c->K);
+  return (nrows + lpw - 1) / lpw;
+}
+
+ColArgs col_args(pty_ctx* c) {
+  ColArgs a{};
+  a.mask = c->mask;
+  a.scale = 1.0f / (float)c->MK();
+  a.tw = c->twM;
+  a.nframes = c->N;
+  a.chunk = c->col_chunk;
+  return a;
+}
+
+// returns the number of partials written (gridDim.x * gridDim.y)
+template <int MODE>
+int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
+  StageTimer tm(c, PTY_STAGE_COLS);
+  const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
+  dim3 grid(c->col_blocks, chunks);
+  switch (c->M) {
+#define C(Lv)                                                                                  \
+  case Lv:                                                                                     \
+    hipLaunchKernelGGL((k_cols<Lv, MODE>), grid, dim3(WG), 0, c->stream, c->g, a);             \
+    break;
+    PTY_SIZES(C)
+#undef C
+    default: return fail(PTY_EUNSUP, "unsupported M");
+  }
+  PTY_LAUNCHED("k_cols");
+  if (nparts) *nparts = c->col_blocks * chunks;
+  return PTY_OK;
+}
+
+int gather(pty_ctx* c, const float2* S, const float2* P, const float2* Oold, float2* Oout,
+           double* H, int fresh, int storeH, float alpha, bool econ, bool esup) {
+  StageTimer tm(c, PTY_STAGE_GATHER);
+  GatherArgs a{};
+  a.S = S;
+  a.P = P;
+  a.Oold = econ ? Oold : nullptr;
+  a.Oout = Oout;
+  a.H = H;
+  a.fresh = fresh;
+  a.storeH = storeH;
+  a.alpha = alpha;
+  a.tile_ptr = c->tile_ptr;
+  a.tile_frames = c->tile_frames;
+  a.tiles_x = c->tiles_x;
+  a.part_econ = econ ? c->part_econO : nullptr;
+  a.part_esup = esup ? c->part_esxh : nullptr;
+  hipLaunchKernelGGL(k_gather_O, dim3(c->ntiles), dim3(WG), 0, c->stream, c->g, a);
+  PTY_LAUNCHED("k_gather_O");
+  return PTY_OK;
+}
+
+int probe_update(pty_ctx* c, const float2* S, const float2* O, const float2* Pold, float2* Pout,
+                 double* G, int fresh, int storeG, float alpha, bool econ) {
+  StageTimer tm(c, PTY_STAGE_PROBE);
+  const int MK = (int)c->MK();
+  ProbeArgs a{};
+  a.S = S;
+  a.O = O;
+  a.pnum = c->pnum;
+  a.pg = c->pg;
+  a.fresh = fresh;
+  a.chunk = c->probe_chunk;
+  dim3 g1((MK + WG - 1) / WG, c->probe_chunks);
+  hipLaunchKernelGGL(k_probe_partial, g1, dim3(WG), 0, c->stream, c->g, a);
+  PTY_LAUNCHED("k_probe_partial");
+  ProbeFinalArgs f{};
+  f.pnum = c->pnum;
+  f.pg = c->pg;
+  f.nchunks = c->probe_chunks;
+  f.fresh = fresh;
+  f.storeG = storeG;
+  f.G = G;
+  f.alpha = alpha;
+  f.Pold = econ ? Pold : nullptr;
+  f.Pout = Pout;
+  f.part_econ = econ ? c->part_econP : nullptr;
+  hipLaunchKernelGGL(k_probe_final, dim3((MK + WG - 1) / WG), dim3(WG), 0, c->stream, c->g, f);
+  PTY_LAUNCHED("k_probe_final");
+  return PTY_OK;
+}
+
+int copy_d2d(pty_ctx* c, void* dst, const void* src, size_t bytes) {
+  PTY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return PTY_OK;
+}
+
+
+// P <- Pmod_P(Pamp, P, mask, alpha)   (era.py:597-610)
+int pmod_probe(pty_ctx* c, float2* Pbuf, float alpha) {
+  StageTimer tm(c, PTY_STAGE_MISC);
+  const int saved = c->prof_override;
+  c->prof_override = PTY_STAGE_MISC;
+  struct Restore { pty_ctx* c; int v; ~Restore() { c->prof_override = v; } } restore{c, saved};
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, Pbuf, c->Xp, c->M));
+  ColArgs a = col_args(c);
+  a.X = c->Xp;
+  a.amp = c->pamp;
+  a.amp_stride = 0;
+  a.alpha = alpha;
+  a.nframes = 1;
+  a.chunk = 1;
+  PTY_TRY(cols<COL_PMODP>(c, a));
+  PTY_TRY(rows_inv<DST_STORE>(c, c->Xp, Pbuf, nullptr, nullptr, 1.0f, nullptr, c->M));
+  return PTY_OK;
+}
+
+// roll P and O (and, for DM, every frame of f) by -cm, cm from |P|^2
+int centre_probe(pty_ctx* c, float2* Pbuf, float2* Obuf, bool roll_f) {
+  StageTimer tm(c, PTY_STAGE_MISC);
+  hipLaunchKernelGGL(k_com, dim3(1), dim3(WG), 0, c->stream, (const float2*)Pbuf, c->M, c->K, c->cm);
+  PTY_LAUNCHED("k_com");
+  const int nb = 1024;
+  hipLaunchKernelGGL(k_roll, dim3(nb), dim3(WG), 0, c->stream, (const float2*)Pbuf, c->Xp, 1, c->M,
+                     c->K, (const int*)c->cm);
+  PTY_LAUNCHED("k_roll");
+  PTY_TRY(copy_d2d(c, Pbuf, c->Xp, c->MK() * sizeof(float2)));
+  hipLaunchKernelGGL(k_roll, dim3(nb), dim3(WG), 0, c->stream, (const float2*)Obuf, c->Oroll, 1,
+                     c->U, c->V, (const int*)c->cm);
+  PTY_LAUNCHED("k_roll");
+  PTY_TRY(copy_d2d(c, Obuf, c->Oroll, (size_t)c->U * c->V * sizeof(float2)));
+  if (roll_f) {
+    hipLaunchKernelGGL(k_roll, dim3(4096), dim3(WG), 0, c->stream, (const float2*)c->F, c->X, c->N,
+                       c->M, c->K, (const int*)c->cm);
+    PTY_LAUNCHED("k_roll");
+    std::swap(c->F, c->X);
+  }
+  return PTY_OK;
+}
+
+int econ_pass(pty_ctx* c, const float2* cur, const float2* bak, size_t count, double* part) {
+  StageTimer tm(c, PTY_STAGE_REDUCE);
+  hipLaunchKernelGGL(k_econ, dim3(ECON_BLOCKS), dim3(WG), 0, c->stream, cur, bak, count, part);
+  PTY_LAUNCHED("k_econ");
+  return PTY_OK;
+}
+
+int metrics_launch(pty_ctx* c, const Regions& rg, int esup_mode, double* out) {
+  StageTimer tm(c, PTY_STAGE_REDUCE);
+  hipLaunchKernelGGL(k_metrics, dim3(1), dim3(1024), 0, c->stream, rg, c->I_norm, (double)c->MK(),
+                     esup_mode, out);
+  PTY_LAUNCHED("k_metrics");
+  return PTY_OK;
+}
+
+int sum_region(pty_ctx* c, const double* p, int n, double* host_out) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, c->stream, p, n, c->scal);
+  PTY_LAUNCHED("k_sum");
+  PTY_HIP(hipMemcpyAsync(host_out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int bg_mean(pty_ctx* c, int nchunks, float* out) {
+  StageTimer tm(c, PTY_STAGE_REDUCE);
+  const int MK = (int)c->MK();
+  hipLaunchKernelGGL(k_bg_mean, dim3((MK + WG - 1) / WG), dim3(WG), 0, c->stream,
+                     (const double*)c->partb, nchunks, MK, c->N, out);
+  PTY_LAUNCHED("k_bg_mean");
+  return PTY_OK;
+}
+
+int ensure_F(pty_ctx* c) {
+  if (!c->F) PTY_TRY(dev_alloc(&c->F, c->NMK()));
+  return PTY_OK;
+}
+
+int check_bound(pty_ctx* c, bool need_bg) {
+  if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
+  if (!c->O || !c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
+  if (need_bg && !c->bg_set) return fail(PTY_ESTATE, "methods 4-6 need pty_set_background");
+  return PTY_OK;
+}
+
+// --------------------------------------------------------------------------
+// one ERA iteration (era.py:152-215 and :223-291)
+// --------------------------------------------------------------------------
+int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
+  const bool bgm = o.method >= 4;
+  const int upd = (o.method - 1) % 3;  // 0 = O, 1 = P, 2 = OP
+  const bool op_step = upd == 2 && (it % o.op_every) == 0;
+  const bool pmodp = it < o.pmod_probe_until;
+  const int nrows = c->N * c->M;
+  const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
+  const bool fusedO = upd == 0;
+  const bool fusedP = upd == 1 && !pmodp;
+  const bool esup_id = upd == 0 && !pmodp;
+
+  if (!fusedO && !fusedP) {
+    StageTimer tm(c, PTY_STAGE_MISC);
+    if (upd != 1) PTY_TRY(copy_d2d(c, c->Obak, c->O, UV * sizeof(float2)));
+    if (upd != 0) PTY_TRY(copy_d2d(c, c->Pbak, c->P, MK * sizeof(float2)));
+  }
+
+  // modulus projection: psi' = IFFT(Pmod(FFT(O_n P)))
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
+  ColArgs a = col_args(c);
+  a.X = c->X;
+  a.amp = c->amp;
+  a.amp_stride = (long)MK;
+  a.alpha = o.alpha;
+  a.part = c->part_emod;
+  a.part2 = esup_id ? c->part_esa : nullptr;
+  int ncol = 0;
+  if (bgm) {
+    a.bg = c->era_bg_shared ? c->bbar : c->bg;
+    a.bg_stride = c->era_bg_shared ? 0 : (long)MK;
+    a.partb = c->partb;
+    PTY_TRY(cols<COL_ERAB>(c, a, &ncol));
+  } else {
+    PTY_TRY(cols<COL_ERA1>(c, a, &ncol));
+  }
+  PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->X, nullptr, nullptr, 1.0f, nullptr, nrows));
+
+  // consistency projection
+  if (upd == 0) {
+    PTY_TRY(gather(c, c->X, c->P, c->O, c->O, c->H, !c->H_valid, 1, o.alpha, true, esup_id));
+    c->H_valid = true;
+  } else if (upd == 1) {
+    PTY_TRY(probe_update(c, c->X, c->O, c->P, c->P, c->G, !c->G_valid, 1, o.alpha, fusedP));
+    c->G_valid = true;
+  } else if (op_step) {
+    for (int r = 0; r < o.op_n; ++r) {
+      PTY_TRY(gather(c, c->X, c->P, nullptr, c->O, c->H, 1, 1, o.alpha, false, false));
+      PTY_TRY(probe_update(c, c->X, c->O, nullptr, c->P, c->G, 1, 1, o.alpha, false));
+      c->H_valid = c->G_valid = true;
+    }
+    if (o.probe_centering) {
+      PTY_TRY(centre_probe(c, c->P, c->O, false));
+      c->H_valid = c->G_valid = false;
+    }
+  } else {
+    PTY_TRY(gather(c, c->X, c->P, nullptr, c->O, c->H, !c->H_valid, 1, o.alpha, false, false));
+    c->H_valid = true;
+  }
+
+  if (pmodp) PTY_TRY(pmod_probe(c, c->P, o.alpha));
+
+  if (bgm) {
+    PTY_TRY(bg_mean(c, (c->N + c->col_chunk - 1) / c->col_chunk, c->bbar));
+    c->era_bg_shared = true;
+  }
+
+  // metrics
+  Regions rg{};
+  rg.p[S_EMOD] = c->part_emod;
+  rg.n[S_EMOD] = ncol;
+  if (fusedO) {
+    rg.p[S_ECON_ON] = c->part_econO;
+    rg.p[S_ECON_OD] = c->part_econO + c->ntiles;
+    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = c->ntiles;
+  } else if (fusedP) {
+    const int nb = (int)((MK + WG - 1) / WG);
+    rg.p[S_ECON_PN] = c->part_econP;
+    rg.p[S_ECON_PD] = c->part_econP + nb;
+    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = nb;
+  } else {
+    if (upd != 1) {
+      PTY_TRY(econ_pass(c, c->O, c->Obak, UV, c->part_econO));
+      rg.p[S_ECON_ON] = c->part_econO;
+      rg.p[S_ECON_OD] = c->part_econO + ECON_BLOCKS;
+      rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
+    }
+    if (upd != 0) {
+      PTY_TRY(econ_pass(c, c->P, c->Pbak, MK, c->part_econP));
+      rg.p[S_ECON_PN] = c->part_econP;
+      rg.p[S_ECON_PD] = c->part_econP + ECON_BLOCKS;
+      rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = ECON_BLOCKS;
+    }
+  }
+  int esup_mode;
+  if (esup_id) {
+    esup_mode = ESUP_IDENTITY;
+    rg.p[S_ESUP_A] = c->part_esa;
+    rg.n[S_ESUP_A] = ncol;
+    rg.p[S_ESUP_X] = c->part_esxh;
+    rg.p[S_ESUP_H] = c->part_esxh + c->ntiles;
+    rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = c->ntiles;
+  } else {
+    esup_mode = ESUP_EXPLICIT;
+    StageTimer tm(c, PTY_STAGE_REDUCE);
+    hipLaunchKernelGGL(k_esup, dim3(ESUP_BLOCKS), dim3(WG), 0, c->stream, c->g, (const float2*)c->X,
+                       (const float2*)c->O, (const float2*)c->P, c->part_esup);
+    PTY_LAUNCHED("k_esup");
+    rg.p[S_ESUP_E] = c->part_esup;
+    rg.n[S_ESUP_E] = ESUP_BLOCKS;
+  }
+  return metrics_launch(c, rg, esup_mode, metrics_out);
+}
+
+// --------------------------------------------------------------------------
+// one DM iteration (dm.py:45-160 and :169-294)
+// --------------------------------------------------------------------------
+int dm_project(pty_ctx* c, const pty_opts& o, int64_t it, int upd, bool op_step, float2* Ob,
+               float2* Pb, bool metric) {
+  if (upd == 0) {
+    PTY_TRY(gather(c, c->F, Pb, nullptr, Ob, c->H, !c->H_valid, 1, o.alpha, false, false));
+    c->H_valid = true;
+  } else if (upd == 1) {
+    PTY_TRY(probe_update(c, c->F, Ob, nullptr, Pb, c->G, !c->G_valid, 1, o.alpha, false));
+    c->G_valid = true;
+  } else if (op_step) {
+    for (int r = 0; r < o.op_n; ++r) {
+      PTY_TRY(gather(c, c->F, Pb, nullptr, Ob, c->H, 1, metric ? 0 : 1, o.alpha, false, false));
+      PTY_TRY(probe_update(c, c->F, Ob, nullptr, Pb, c->G, 1, metric ? 0 : 1, o.alpha, false));
+      if (!metric) c->H_valid = c->G_valid = true;
+    }
+    if (o.probe_centering) {
+      PTY_TRY(centre_probe(c, Pb, Ob, !metric));
+      if (!metric) c->H_valid = c->G_valid = false;
+    }
+  } else {
+    // dm.py:85 / :130: project with the main-iterate probe and the cached heatmap
+    PTY_TRY(gather(c, c->F, c->P, nullptr, Ob, c->H, !c->H_valid, 1, o.alpha, false, false));
+    c->H_valid = true;
+  }
+  (void)it;
+  return PTY_OK;
+}
+
+int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
+  const bool bgm = o.method >= 4;
+  const int upd = (o.method - 1) % 3;
+  const bool op_step = upd == 2 && (it % o.op_every) == 0;
+  const bool pmodp = it < o.pmod_probe_until;
+  const int nrows = c->N * c->M;
+  const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
+  const int nchunks = (c->N + c->col_chunk - 1) / c->col_chunk;
+
+  if (bgm) {
+    // leading Pmod_7 of (f, b) (dm.py:171)
+    PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->F, c->X, nrows));
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.amp = c->amp;
+    a.amp_stride = (long)MK;
+    a.alpha = o.alpha;
+    a.bg = c->bg;
+    a.bg_stride = (long)MK;
+    a.partb = c->partb;
+    PTY_TRY(cols<COL_DMB_LEAD>(c, a));
+    PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->F, nullptr, nullptr, 1.0f, nullptr, nrows));
+    PTY_TRY(bg_mean(c, nchunks, c->bbar));
+  }
+
+  PTY_TRY(dm_project(c, o, it, upd, op_step, c->O, c->P, false));
+  if (pmodp) PTY_TRY(pmod_probe(c, c->P, o.alpha));
+
+  // f <- f - Ps f + Pm(2 Ps f - f)    (dm.py:91-97, :216-224)
+  PTY_TRY(rows_fwd<SRC_DM>(c, c->O, c->P, c->F, c->X, nrows));
+  {
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.amp = c->amp;
+    a.amp_stride = (long)MK;
+    a.alpha = o.alpha;
+    if (bgm) {
+      a.bg = c->bg;
+      a.bg_stride = (long)MK;
+      a.bbar = c->bbar;
+      a.partb = c->partb;
+      PTY_TRY(cols<COL_DMB_UPD>(c, a));
+    } else {
+      PTY_TRY(cols<COL_DM1>(c, a));
+    }
+  }
+  PTY_TRY(rows_inv<DST_DMUPD>(c, c->X, c->F, c->O, c->P, 1.0f, c->part_esup, nrows));
+  const int nrb = row_blocks(c, nrows);
+
+  // metrics: Os, Ps = Psup(f)  (dm.py:104-137)
+  {
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->Os, c->O, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->Ps, c->P, MK * sizeof(float2)));
+  }
+  PTY_TRY(dm_project(c, o, it, upd, op_step, c->Os, c->Ps, true));
+  if (pmodp) PTY_TRY(pmod_probe(c, c->Ps, o.alpha));
+  if (bgm) PTY_TRY(bg_mean(c, nchunks, c->bbar_m));
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->Ps, nullptr, c->X, nrows));
+  int ncol = 0;
+  {
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.amp = c->amp;
+    a.amp_stride = (long)MK;
+    a.part = c->part_emod;
+    if (bgm) {
+      a.bbar = c->bbar_m;
+      PTY_TRY(cols<COL_ERR7>(c, a, &ncol));
+    } else {
+      PTY_TRY(cols<COL_ERR1>(c, a, &ncol));
+    }
+  }
+  Regions rg{};
+  rg.p[S_EMOD] = c->part_emod;
+  rg.n[S_EMOD] = ncol;
+  if (upd != 1) {
+    PTY_TRY(econ_pass(c, c->Os, c->Obak, UV, c->part_econO));
+    rg.p[S_ECON_ON] = c->part_econO;
+    rg.p[S_ECON_OD] = c->part_econO + ECON_BLOCKS;
+    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
+  }
+  if (upd != 0) {
+    PTY_TRY(econ_pass(c, c->Ps, c->Pbak, MK, c->part_econP));
+    rg.p[S_ECON_PN] = c->part_econP;
+    rg.p[S_ECON_PD] = c->part_econP + ECON_BLOCKS;
+    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = ECON_BLOCKS;
+  }
+  rg.p[S_ESUP_E] = c->part_esup;
+  rg.n[S_ESUP_E] = nrb;
+  PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, metrics_out));
+  {
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->Obak, c->Os, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->Pbak, c->Ps, MK * sizeof(float2)));
+  }
+  c->have_metric_state = true;
+  return PTY_OK;
+}
+
+void twiddles(int L, std::vector<float2>& out) {
+  out.resize(L);
+  for (int k = 0; k < L; ++k) {
+    const double ang = -2.0 * M_PI * (double)k / (double)L;
+    out[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+const char* pty_last_error(void) { return g_err.c_str(); }
+
+int pty_supported_frame(int32_t L) { return threads_per_line(L) > 0 ? 1 : 0; }
+
+int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_t V,
+               const int32_t* R_host) {
+  if (!out) return fail(PTY_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (N <= 0) return fail(PTY_EINVAL, "N must be positive");
+  if (!pty_supported_frame(M) || !pty_supported_frame(K))
+    return fail(PTY_EUNSUP, "frame sides must be one of 16, 32, 64, 96, 128, 192, 256 (got " +
+                                std::to_string(M) + " x " + std::to_string(K) + ")");
+  if (U < M || V < K) return fail(PTY_EINVAL, "object smaller than a frame");
+  if (!R_host) return fail(PTY_EINVAL, "R is NULL");
+  if ((size_t)N * M * K > ((size_t)1 << 34)) return fail(PTY_EINVAL, "frame stack too large");
+  std::vector<int2> off(N);
+  for (int n = 0; n < N; ++n) {
+    const int y = -R_host[2 * n], x = -R_host[2 * n + 1];
+    if (y < 0 || x < 0 || y + M > U || x + K > V)
+      return fail(PTY_EINVAL, "frame " + std::to_string(n) + " window (" + std::to_string(y) +
+                                  ", " + std::to_string(x) + ") lies outside the object");
+    off[n] = make_int2(y, x);  // .x = row offset, .y = column offset
+  }
+  pty_ctx* c = new pty_ctx();
+  c->N = N; c->M = M; c->K = K; c->U = U; c->V = V;
+  auto cleanup = [&](int code) { pty_destroy(c); return code; };
+
+  // object tiles -> ascending frame lists (CSR)
+  c->tiles_y = (U + TILE_H - 1) / TILE_H;
+  c->tiles_x = (V + TILE_W - 1) / TILE_W;
+  c->ntiles = c->tiles_x * c->tiles_y;
+  std::vector<int> cnt(c->ntiles + 1, 0);
+  for (int n = 0; n < N; ++n)
+    for (int ty = off[n].x / TILE_H; ty <= (off[n].x + M - 1) / TILE_H; ++ty)
+      for (int tx = off[n].y / TILE_W; tx <= (off[n].y + K - 1) / TILE_W; ++tx)
+        cnt[ty * c->tiles_x + tx + 1]++;
+  for (int t = 0; t < c->ntiles; ++t) cnt[t + 1] += cnt[t];
+  std::vector<int> frames(cnt[c->ntiles]);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int n = 0; n < N; ++n)
+    for (int ty = off[n].x / TILE_H; ty <= (off[n].x + M - 1) / TILE_H; ++ty)
+      for (int tx = off[n].y / TILE_W; tx <= (off[n].y + K - 1) / TILE_W; ++tx)
+        frames[fill[ty * c->tiles_x + tx]++] = n;
+
+  // launch shapes
+  c->col_blocks = (K + (WG / threads_per_line(M)) - 1) / (WG / threads_per_line(M));
+  c->col_chunk = std::max(1, std::min(16, (int)((long)N * c->col_blocks / 2048)));
+  c->col_chunks = (N + c->col_chunk - 1) / c->col_chunk;
+  const int MK = M * K;
+  const int pblocks = (MK + WG - 1) / WG;
+  c->probe_chunks = std::max(1, std::min(N, std::max(1, 4096 / pblocks)));
+  c->probe_chunk = (N + c->probe_chunks - 1) / c->probe_chunks;
+  c->probe_chunks = (N + c->probe_chunk - 1) / c->probe_chunk;
+
+  c->cap_col = c->col_blocks * c->col_chunks;
+  c->cap_econO = std::max(c->ntiles, ECON_BLOCKS);
+  c->cap_econP = std::max(pblocks, ECON_BLOCKS);
+  c->cap_esup = std::max(ESUP_BLOCKS, row_blocks(c, N * M));
+
+  int r;
+  if ((r = dev_alloc(&c->off, N)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->tile_ptr, c->ntiles + 1)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->tile_frames, frames.size())) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->twM, M)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->twK, K)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->X, c->NMK())) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->Xp, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->Obak, (size_t)U * V)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->Os, (size_t)U * V)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->Oroll, (size_t)U * V)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->Pbak, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->Ps, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->H, (size_t)U * V)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->G, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->bbar, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->bbar_m, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->pamp, MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->pnum, (size_t)c->probe_chunks * MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->pg, (size_t)c->probe_chunks * MK)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->cm, 2)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->scal, 8)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_emod, c->cap_col)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_esa, c->cap_col)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_econO, 2 * (size_t)c->cap_econO)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_econP, 2 * (size_t)c->cap_econP)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_esup, c->cap_esup)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_esxh, 2 * (size_t)c->ntiles)) != PTY_OK) return cleanup(r);
+
+  std::vector<float2> tw;
+  hipError_t e = hipSuccess;
+  twiddles(M, tw);
+  if (e == hipSuccess) e = hipMemcpy(c->twM, tw.data(), M * sizeof(float2), hipMemcpyHostToDevice);
+  twiddles(K, tw);
+  if (e == hipSuccess) e = hipMemcpy(c->twK, tw.data(), K * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->off, off.data(), N * sizeof(int2), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->tile_ptr, cnt.data(), (c->ntiles + 1) * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->tile_frames, frames.data(), frames.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return cleanup(fail(PTY_EHIP, std::string("upload: ") + hipGetErrorString(e)));
+  c->g.N = N; c->g.M = M; c->g.K = K; c->g.U = U; c->g.V = V;
+  c->g.off = c->off;
+  *out = c;
+  return PTY_OK;
+}
+
+int pty_destroy(pty_ctx* c) {
+  if (!c) return PTY_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  else (void)hipDeviceSynchronize();
+  dev_free(c->off); dev_free(c->tile_ptr); dev_free(c->tile_frames);
+  dev_free(c->twM); dev_free(c->twK);
+  dev_free(c->X); dev_free(c->F); dev_free(c->Xp);
+  dev_free(c->Obak); dev_free(c->Pbak); dev_free(c->Os); dev_free(c->Oroll); dev_free(c->Ps);
+  dev_free(c->H); dev_free(c->G); dev_free(c->bg); dev_free(c->bbar); dev_free(c->bbar_m);
+  dev_free(c->pamp); dev_free(c->partb); dev_free(c->pnum); dev_free(c->pg);
+  dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal);
+  dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
+  dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  delete c;
+  return PTY_OK;
+}
+
+int pty_set_stream(pty_ctx* c, void* stream) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  c->stream = (hipStream_t)stream;
+  return PTY_OK;
+}
+
+int pty_prepare_amp(pty_ctx* c, const float* I, const uint8_t* mask_centred, float* amp,
+                    double* I_norm_out) {
+  if (!c || !I || !amp) return fail(PTY_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_prepare_amp, dim3(ESUP_BLOCKS), dim3(WG), 0, c->stream, I, c->N, c->M, c->K,
+                     mask_centred, amp, c->part_esup);
+  PTY_LAUNCHED("k_prepare_amp");
+  double s = 0.0;
+  PTY_TRY(sum_region(c, c->part_esup, ESUP_BLOCKS, &s));
+  if (I_norm_out) *I_norm_out = s;
+  return PTY_OK;
+}
+
+int pty_set_data(pty_ctx* c, const float* amp, const uint8_t* mask, double I_norm) {
+  if (!c || !amp) return fail(PTY_EINVAL, "NULL argument");
+  c->amp = amp;
+  c->mask = mask;
+  c->I_norm = I_norm;
+  return PTY_OK;
+}
+
+int pty_set_state(pty_ctx* c, pty_cfloat* O, pty_cfloat* P) {
+  if (!c || !O || !P) return fail(PTY_EINVAL, "NULL argument");
+  c->O = reinterpret_cast<float2*>(O);
+  c->P = reinterpret_cast<float2*>(P);
+  c->H_valid = c->G_valid = false;
+  return PTY_OK;
+}
+
+int pty_set_background(pty_ctx* c, const float* b, int32_t per_frame) {
+  if (!c || !b) return fail(PTY_EINVAL, "NULL argument");
+  if (!c->bg) PTY_TRY(dev_alloc(&c->bg, c->NMK()));
+  if (!c->partb) PTY_TRY(dev_alloc(&c->partb, (size_t)c->col_chunks * c->MK()));
+  if (per_frame) {
+    PTY_TRY(copy_d2d(c, c->bg, b, c->NMK() * sizeof(float)));
+  } else {
+    for (int n = 0; n < c->N; ++n)
+      PTY_TRY(copy_d2d(c, c->bg + (size_t)n * c->MK(), b, c->MK() * sizeof(float)));
+  }
+  c->bg_set = true;
+  c->era_bg_shared = false;
+  return PTY_OK;
+}
+
+int pty_set_pamp(pty_ctx* c, const float* pamp) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (pamp) {
+    PTY_TRY(copy_d2d(c, c->pamp, pamp, c->MK() * sizeof(float)));
+  } else {
+    if (!c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
+    PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->P, c->Xp, c->M));
+    ColArgs a = col_args(c);
+    a.X = c->Xp;
+    a.nframes = 1;
+    a.chunk = 1;
+    PTY_TRY(cols<COL_FWD>(c, a));
+    hipLaunchKernelGGL(k_abs, dim3(64), dim3(WG), 0, c->stream, (const float2*)c->Xp, c->pamp, c->MK());
+    PTY_LAUNCHED("k_abs");
+  }
+  c->pamp_set = true;
+  return PTY_OK;
+}
+
+int pty_run(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t begin,
+            double* metrics_host) {
+  if (!c || !o) return fail(PTY_EINVAL, "NULL argument");
+  if (o->method < 1 || o->method > 6) return fail(PTY_EINVAL, "method must be 1..6");
+  if (o->algorithm != 0 && o->algorithm != 1) return fail(PTY_EINVAL, "algorithm must be 0 or 1");
+  if (o->op_n < 0 || o->op_every < 1) return fail(PTY_EINVAL, "bad OP_iters");
+  if (iters < 0) return fail(PTY_EINVAL, "iters < 0");
+  PTY_TRY(check_bound(c, o->method >= 4));
+  if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
+  if (iters > c->metrics_cap) {
+    dev_free(c->metrics);
+    PTY_TRY(dev_alloc(&c->metrics, 3 * (size_t)iters));
+    c->metrics_cap = iters;
+  }
+  if (begin) {
+    c->H_valid = c->G_valid = false;
+    c->have_metric_state = false;
+    c->era_bg_shared = false;
+    const size_t UV = (size_t)c->U * c->V;
+    PTY_TRY(copy_d2d(c, c->Obak, c->O, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->Pbak, c->P, c->MK() * sizeof(float2)));
+    if (o->algorithm == 1) {
+      PTY_TRY(ensure_F(c));
+      hipLaunchKernelGGL(k_make_exits, dim3(4096), dim3(WG), 0, c->stream, c->g, (const float2*)c->O,
+                         (const float2*)c->P, c->F);
+      PTY_LAUNCHED("k_make_exits");
+      c->dm_ready = true;
+    }
+  }
+  if (o->algorithm == 1 && !c->dm_ready) return fail(PTY_ESTATE, "DM run needs begin=1 first");
+  c->last_algo = o->algorithm;
+  for (int32_t k = 0; k < iters; ++k) {
+    const int64_t it = iter0 + k;
+    if (o->algorithm == 0) PTY_TRY(era_iter(c, *o, it, c->metrics + 3 * (size_t)k));
+    else PTY_TRY(dm_iter(c, *o, it, c->metrics + 3 * (size_t)k));
+  }
+  if (metrics_host && iters > 0)
+    PTY_HIP(hipMemcpyAsync(metrics_host, c->metrics, 3 * (size_t)iters * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  if (c->prof_on) prof_collect(c);
+  return PTY_OK;
+}
+
+int pty_profile(pty_ctx* c, int32_t enable) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->prof_used = 0;
+  c->prof_on = enable != 0;
+  for (int k = 0; k < PTY_NSTAGE; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_cnt[k] = 0;
+  }
+  return PTY_OK;
+}
+
+int pty_profile_read(pty_ctx* c, double* ms, int64_t* cnt) {
+  if (!c || !ms || !cnt) return fail(PTY_EINVAL, "NULL argument");
+  for (int k = 0; k < PTY_NSTAGE; ++k) {
+    ms[k] = c->prof_ms[k];
+    cnt[k] = c->prof_cnt[k];
+  }
+  return PTY_OK;
+}
+
+int pty_intensity(pty_ctx* c, float* I) {
+  if (!c || !I) return fail(PTY_EINVAL, "NULL argument");
+  if (!c->O || !c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
+  const bool dm = c->last_algo == 1 && c->have_metric_state;
+  const float2* Ob = dm ? c->Os : c->O;
+  const float2* Pb = dm ? c->Ps : c->P;
+  PTY_TRY(rows_fwd<SRC_OP>(c, Ob, Pb, nullptr, c->X, c->N * c->M));
+  ColArgs a = col_args(c);
+  a.X = c->X;
+  a.outI = I;
+  PTY_TRY(cols<COL_INTENS>(c, a));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_heatmap(pty_ctx* c, double* H, int32_t* valid) {
+  if (!c || !H || !valid) return fail(PTY_EINVAL, "NULL argument");
+  *valid = c->H_valid ? 1 : 0;
+  if (c->H_valid) PTY_TRY(copy_d2d(c, H, c->H, (size_t)c->U * c->V * sizeof(double)));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_background(pty_ctx* c, float* b) {
+  if (!c || !b) return fail(PTY_EINVAL, "NULL argument");
+  if (!c->bg_set) return fail(PTY_ESTATE, "no background state");
+  const float* src = c->bg;  // first frame of the state before any mean
+  if (c->last_algo == 1 && c->have_metric_state) src = c->bbar_m;
+  else if (c->last_algo == 0 && c->era_bg_shared) src = c->bbar;
+  PTY_TRY(copy_d2d(c, b, src, c->MK() * sizeof(float)));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_fft2(pty_ctx* c, pty_cfloat* data, int32_t nframes, int32_t inverse) {
+  if (!c || !data) return fail(PTY_EINVAL, "NULL argument");
+  if (nframes < 0) return fail(PTY_EINVAL, "nframes < 0");
+  if (nframes == 0) return PTY_OK;
+  float2* d = reinterpret_cast<float2*>(data);
+  ColArgs a = col_args(c);
+  a.X = d;
+  a.nframes = nframes;
+  a.chunk = std::max(1, std::min(16, (int)((long)nframes * c->col_blocks / 2048)));
+  if (!inverse) {
+    PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, d, d, nframes * c->M));
+    PTY_TRY(cols<COL_FWD>(c, a));
+  } else {
+    PTY_TRY(cols<COL_INV>(c, a));
+    PTY_TRY(rows_inv<DST_STORE>(c, d, d, nullptr, nullptr, 1.0f, nullptr, nframes * c->M));
+  }
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfloat* ex) {
+  if (!c || !O || !P || !ex) return fail(PTY_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_make_exits, dim3(4096), dim3(WG), 0, c->stream, c->g,
+                     reinterpret_cast<const float2*>(O), reinterpret_cast<const float2*>(P),
+                     reinterpret_cast<float2*>(ex));
+  PTY_LAUNCHED("k_make_exits");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_pmod1(pty_ctx* c, pty_cfloat* exits, float alpha, double* emod) {
+  if (!c || !exits) return fail(PTY_EINVAL, "NULL argument");
+  if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
+  float2* d = reinterpret_cast<float2*>(exits);
+  const int nrows = c->N * c->M;
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, d, d, nrows));
+  ColArgs a = col_args(c);
+  a.X = d;
+  a.amp = c->amp;
+  a.amp_stride = (long)c->MK();
+  a.alpha = alpha;
+  a.part = c->part_emod;
+  int ncol = 0;
+  PTY_TRY(cols<COL_ERA1>(c, a, &ncol));
+  PTY_TRY(rows_inv<DST_STORE>(c, d, d, nullptr, nullptr, 1.0f, nullptr, nrows));
+  double s = 0.0;
+  PTY_TRY(sum_region(c, c->part_emod, ncol, &s));
+  if (emod) *emod = s;
+  return PTY_OK;
+}
+
+int pty_psup_O(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* P, pty_cfloat* O_out, double* H,
+               int32_t fresh, float alpha) {
+  if (!c || !ex || !P || !O_out || !H) return fail(PTY_EINVAL, "NULL argument");
+  PTY_TRY(gather(c, reinterpret_cast<const float2*>(ex), reinterpret_cast<const float2*>(P), nullptr,
+                 reinterpret_cast<float2*>(O_out), H, fresh ? 1 : 0, 1, alpha, false, false));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_psup_P(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* O, pty_cfloat* P_out, double* G,
+               int32_t fresh, float alpha) {
+  if (!c || !ex || !O || !P_out || !G) return fail(PTY_EINVAL, "NULL argument");
+  PTY_TRY(probe_update(c, reinterpret_cast<const float2*>(ex), reinterpret_cast<const float2*>(O),
+                       nullptr, reinterpret_cast<float2*>(P_out), G, fresh ? 1 : 0, 1, alpha, false));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+"""Device-side engine: one ptyhip context plus the torch tensors it works on.
+
+PyTorch is used only to own device memory and to name the stream; all
+arithmetic of the iteration runs in the HIP kernels of ``libptyhip.so``.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import PtyOpts, call
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def require_gpu(device=None):
+    """Return a ROCm torch device or raise: the ERA / DM path has no CPU fallback."""
+    _abi.load()
+    if not torch.cuda.is_available():
+        raise RuntimeError("ptychography_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "the ERA/DM iteration has no CPU fallback")
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if dev.type != "cuda":
+        raise ValueError(f"device must be a GPU device, got {dev}")
+    return dev
+
+
+class Engine:
+    """A scan geometry on one GPU: frames, object window offsets and workspace.
+
+    ``R`` follows the reference convention AFTER its preamble shift
+    (ptychography/era.py:504-505): ``R <= 0`` and frame n sees
+    ``O[-R[n,0]:M-R[n,0], -R[n,1]:K-R[n,1]]``.
+    """
+
+    def __init__(self, N, M, K, U, V, R, device=None):
+        self.device = require_gpu(device)
+        self.N, self.M, self.K, self.U, self.V = int(N), int(M), int(K), int(U), int(V)
+        Rc = np.ascontiguousarray(np.asarray(R, dtype=np.int32).reshape(self.N, 2))
+        self._R = Rc
+        self._ctx = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            call("pty_create", ctypes.byref(self._ctx), self.N, self.M, self.K, self.U, self.V,
+                 Rc.ctypes.data_as(ctypes.c_void_p))
+            stream = torch.cuda.current_stream(self.device)
+            call("pty_set_stream", self._ctx, ctypes.c_void_p(stream.cuda_stream))
+        self._keep = {}
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            _abi.load().pty_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _sync_torch(self):
+        # the context runs on torch's current stream; make prior torch work visible
+        torch.cuda.current_stream(self.device).synchronize()
+
+    # -- binding ------------------------------------------------------------
+    def prepare_amp(self, I, mask_centred=None):
+        """amp = ifftshift(sqrt(I)), I_norm = sum(mask * I) (ptychography/era.py:495-498)."""
+        I = I.to(self.device, torch.float32).contiguous()
+        amp = torch.empty_like(I)
+        m = None
+        if mask_centred is not None:
+            m = mask_centred.to(self.device, torch.uint8).contiguous()
+        self._sync_torch()
+        out = ctypes.c_double()
+        call("pty_prepare_amp", self._ctx, _ptr(I), _ptr(m), _ptr(amp), ctypes.byref(out))
+        return amp, float(out.value)
+
+    def set_data(self, amp, mask_shifted, I_norm):
+        self._keep["amp"] = amp
+        self._keep["mask"] = mask_shifted
+        call("pty_set_data", self._ctx, _ptr(amp), _ptr(mask_shifted), float(I_norm))
+
+    def set_state(self, O, P):
+        assert O.dtype == torch.complex64 and P.dtype == torch.complex64
+        assert O.is_contiguous() and P.is_contiguous()
+        assert tuple(O.shape) == (self.U, self.V) and tuple(P.shape) == (self.M, self.K)
+        self._keep["O"], self._keep["P"] = O, P
+        self._sync_torch()
+        call("pty_set_state", self._ctx, _ptr(O), _ptr(P))
+
+    def set_background(self, b, per_frame):
+        b = b.to(self.device, torch.float32).contiguous()
+        self._sync_torch()
+        call("pty_set_background", self._ctx, _ptr(b), int(bool(per_frame)))
+
+    def set_pamp(self, pamp=None):
+        if pamp is not None:
+            pamp = pamp.to(self.device, torch.float32).contiguous()
+            self._sync_torch()
+        call("pty_set_pamp", self._ctx, _ptr(pamp))
+
+    # -- running ------------------------------------------------------------
+    def run(self, algorithm, method, iters, iter0=0, begin=False, OP_iters=(1, 1),
+            Pmod_probe_until=0, probe_centering=False, alpha=1e-10, metrics=True):
+        """Run ``iters`` iterations; returns an (iters, 3) array of eMod, eCon, eSup."""
+        o = PtyOpts(algorithm=int(algorithm), method=int(method), op_n=int(OP_iters[0]),
+                    op_every=int(OP_iters[1]), pmod_probe_until=int(Pmod_probe_until),
+                    probe_centering=int(bool(probe_centering)), alpha=float(alpha))
+        out = np.zeros((max(iters, 0), 3), dtype=np.float64) if metrics else None
+        ptr = out.ctypes.data_as(ctypes.c_void_p) if (metrics and iters > 0) else None
+        call("pty_run", self._ctx, ctypes.byref(o), int(iter0), int(iters), int(bool(begin)), ptr)
+        return out
+
+    def profile(self, enable=True):
+        call("pty_profile", self._ctx, int(bool(enable)))
+
+    def profile_read(self):
+        """{stage: (total_ms, launches)} since profiling was enabled."""
+        ms = np.zeros(len(_abi.STAGES), np.float64)
+        cnt = np.zeros(len(_abi.STAGES), np.int64)
+        call("pty_profile_read", self._ctx, ms.ctypes.data_as(ctypes.c_void_p),
+             cnt.ctypes.data_as(ctypes.c_void_p))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_abi.STAGES)}
+
+    def intensity(self):
+        out = torch.empty((self.N, self.M, self.K), dtype=torch.float32, device=self.device)
+        call("pty_intensity", self._ctx, _ptr(out))
+        return out
+
+    def heatmap(self):
+        H = torch.empty((self.U, self.V), dtype=torch.float64, device=self.device)
+        valid = ctypes.c_int32()
+        call("pty_heatmap", self._ctx, _ptr(H), ctypes.byref(valid))
+        return H if valid.value else None
+
+    def background(self):
+        b = torch.empty((self.M, self.K), dtype=torch.float32, device=self.device)
+        call("pty_background", self._ctx, _ptr(b))
+        return b
+
+    # -- building blocks ----------------------------------------------------
+    def fft2(self, data, inverse=False):
+        """In-place batched 2-D FFT of a (n, M, K) complex64 tensor."""
+        assert data.dtype == torch.complex64 and data.is_contiguous()
+        assert tuple(data.shape[1:]) == (self.M, self.K)
+        self._sync_torch()
+        call("pty_fft2", self._ctx, _ptr(data), int(data.shape[0]), int(bool(inverse)))
+        return data
+
+    def make_exits(self, O, P):
+        out = torch.empty((self.N, self.M, self.K), dtype=torch.complex64, device=self.device)
+        self._sync_torch()
+        call("pty_make_exits", self._ctx, _ptr(O), _ptr(P), _ptr(out))
+        return out
+
+    def pmod1(self, exits, alpha=1e-10):
+        self._sync_torch()
+        e = ctypes.c_double()
+        call("pty_pmod1", self._ctx, _ptr(exits), float(alpha), ctypes.byref(e))
+        return exits, e.value
+
+    def psup_O(self, exits, P, H=None, alpha=1e-10):
+        O = torch.empty((self.U, self.V), dtype=torch.complex64, device=self.device)
+        fresh = H is None
+        if H is None:
+            H = torch.empty((self.U, self.V), dtype=torch.float64, device=self.device)
+        self._sync_torch()
+        call("pty_psup_O", self._ctx, _ptr(exits), _ptr(P), _ptr(O), _ptr(H), int(fresh), float(alpha))
+        return O, H
+
+    def psup_P(self, exits, O, G=None, alpha=1e-10):
+        P = torch.empty((self.M, self.K), dtype=torch.complex64, device=self.device)
+        fresh = G is None
+        if G is None:
+            G = torch.empty((self.M, self.K), dtype=torch.float64, device=self.device)
+        self._sync_torch()
+        call("pty_psup_P", self._ctx, _ptr(exits), _ptr(O), _ptr(P), _ptr(G), int(fresh), float(alpha))
+        return P, G

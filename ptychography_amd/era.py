@@ -1,0 +1,211 @@
+"""``ERA`` -- error-reduction ptychography on one MI355X.
+
+Drop-in for ``Ptychography.ERA`` (ptychography/era.py:13-333): same positional
+and keyword arguments, same returns ``(O, info)`` / ``(P, info)`` /
+``(O, P, info)`` (or without ``info`` when ``full_output=False``), same side
+effect on the caller's ``R`` (shifted in place so each column's maximum is 0,
+ptychography/era.py:504-505).  ``info`` holds ``'I'``, ``'eMod'``, ``'eCon'``,
+``'heatmap'`` and ``'background'`` as in the reference, plus ``'eSup'``.
+
+All iteration arithmetic runs in complex64 in the gfx950 kernels of
+``libptyhip.so``; the error sums and the object normalisation use FP64.  The
+returned arrays have the precision the reference would use (complex128 for
+float64 intensities, complex64 for float32).
+"""
+
+import math
+
+import numpy as np
+import torch
+
+from .engine import Engine, require_gpu
+
+__all__ = ["ERA", "solve"]
+
+ALGO_ERA = 0
+ALGO_DM = 1
+
+
+def resolve_method(O, P, background, method):
+    """Method and update target as the reference's preamble decides (era.py:440-456)."""
+    if method is None:
+        if O is None and P is None:
+            method = 3
+        elif O is None:
+            method = 1
+        elif P is None:
+            method = 2
+        else:
+            raise ValueError("method=None with both O and P given is undefined in the reference "
+                             "(its 'update' is never set); pass method explicitly")
+        if background is not None:
+            method += 3
+    method = int(method)
+    if method not in (1, 2, 3, 4, 5, 6):
+        raise ValueError("method must be one of 1..6")
+    update = {1: 'O', 4: 'O', 2: 'P', 5: 'P', 3: 'OP', 6: 'OP'}[method]
+    return method, update
+
+
+def pmod_probe_until(Pmod_probe):
+    """Iterations i with ``Pmod_probe is not None and i < Pmod_probe`` are [0, until)."""
+    if Pmod_probe is None or Pmod_probe is False:
+        return 0
+    p = float(Pmod_probe)
+    if math.isinf(p):
+        return 1 << 62 if p > 0 else 0
+    return max(0, int(math.ceil(p)))
+
+
+def _as_numpy(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def _random(rng, shape):
+    return np.random.random(shape) if rng is None else rng.random(shape)
+
+
+def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
+          Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
+          full_output=True, device=None, callback=None, rng=None):
+    """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns."""
+    dev = require_gpu(device)
+    if algorithm not in (ALGO_ERA, ALGO_DM):
+        raise ValueError("algorithm must be 0 (ERA) or 1 (DM)")
+    if iters < 0:
+        raise ValueError("iters must be >= 0")
+    method, update = resolve_method(O, P, background, method)
+    if isinstance(OP_iters, (int, np.integer)):
+        OP_iters = (int(OP_iters), 1)
+    OP_iters = (int(OP_iters[0]), int(OP_iters[1]))
+
+    I_is_tensor = isinstance(I, torch.Tensor)
+    N, M, K = (int(s) for s in I.shape)
+    if dtype is None:
+        single = (I.dtype == torch.float32) if I_is_tensor else (np.asarray(I).dtype == np.float32)
+        dtype = 'single' if single else 'double'
+    if dtype == 'single':
+        r_dtype, c_dtype = np.float32, np.complex64
+    elif dtype == 'double':
+        r_dtype, c_dtype = np.float64, np.complex128
+    else:
+        raise ValueError("dtype must be None, 'single' or 'double'")
+
+    if not isinstance(R, np.ndarray):
+        raise TypeError("R must be a numpy integer array (it is shifted in place, era.py:504-505)")
+    if O is None:
+        shape = (M + int(R[:, 0].max() - R[:, 0].min()), K + int(R[:, 1].max() - R[:, 1].min()))
+        O = np.ones(shape, dtype=c_dtype)
+    if P is None:
+        P = _random(rng, (M, K)) + 1J * _random(rng, (M, K))
+    O = _as_numpy(O).astype(c_dtype)
+    P = _as_numpy(P).astype(c_dtype)
+    if P.shape != (M, K):
+        raise ValueError(f"P has shape {P.shape}, frames are {(M, K)}")
+
+    # in-place shift of the caller's R (era.py:504-505)
+    R[:, 0] -= R[:, 0].max()
+    R[:, 1] -= R[:, 1].max()
+
+    U, V = O.shape
+    eng = Engine(N, M, K, U, V, R, device=dev)
+    try:
+        mask_c = None
+        mask_s = None
+        if not (np.isscalar(mask) and mask == 1):
+            m = _as_numpy(mask)
+            if m.shape != (M, K):
+                raise ValueError(f"mask must be 1 or an array of shape {(M, K)}")
+            if not np.all((m == 0) | (m == 1)):
+                raise ValueError("mask values must be 0/1 (True = good pixel)")
+            mc = m.astype(np.uint8)
+            mask_c = torch.from_numpy(np.ascontiguousarray(mc)).to(dev)
+            mask_s = torch.from_numpy(np.ascontiguousarray(np.fft.ifftshift(mc))).to(dev)
+        I_t = I.to(dev) if I_is_tensor else torch.from_numpy(np.ascontiguousarray(I, dtype=np.float32))
+        amp, I_norm = eng.prepare_amp(I_t, mask_c)
+        del I_t
+        eng.set_data(amp, mask_s, I_norm)
+
+        O_t = torch.from_numpy(np.ascontiguousarray(O.astype(np.complex64))).to(dev)
+        P_t = torch.from_numpy(np.ascontiguousarray(P.astype(np.complex64))).to(dev)
+        eng.set_state(O_t, P_t)
+
+        bg_init = None
+        if method >= 4:
+            if background is None:
+                b = (_random(rng, (N, M, K)).astype(r_dtype) + 0.1).astype(np.float32)
+                eng.set_background(torch.from_numpy(b), per_frame=True)
+            else:
+                B = _as_numpy(background)
+                b = np.sqrt(np.fft.ifftshift(B)).astype(np.float32)  # every axis, as era.py:543
+                if b.ndim == 2:
+                    eng.set_background(torch.from_numpy(np.ascontiguousarray(b)), per_frame=False)
+                else:
+                    eng.set_background(torch.from_numpy(np.ascontiguousarray(
+                        np.broadcast_to(b, (N, M, K)))), per_frame=True)
+            bg_init = b[0] if b.ndim == 3 else b
+
+        until = pmod_probe_until(Pmod_probe)
+        if until > 0:
+            eng.set_pamp(None)   # |FFT(P0)| (era.py:546-547)
+
+        kw = dict(OP_iters=OP_iters, Pmod_probe_until=until, probe_centering=probe_centering,
+                  alpha=alpha)
+        if callback is None:
+            metrics = eng.run(algorithm, method, iters, iter0=0, begin=True, **kw)
+        else:
+            rows = []
+            for i in range(iters):
+                rows.append(eng.run(algorithm, method, 1, iter0=i, begin=(i == 0), **kw))
+                callback(i, O_t.cpu().numpy().astype(c_dtype), P_t.cpu().numpy().astype(c_dtype))
+            metrics = np.concatenate(rows, axis=0) if rows else np.zeros((0, 3))
+            if iters == 0:
+                eng.run(algorithm, method, 0, iter0=0, begin=True, **kw)
+
+        O_out = O_t.cpu().numpy().astype(c_dtype)
+        P_out = P_t.cpu().numpy().astype(c_dtype)
+        if not full_output:
+            return {'O': O_out, 'P': P_out}[update] if update != 'OP' else (O_out, P_out)
+
+        info = {}
+        info['I'] = eng.intensity().cpu().numpy().astype(r_dtype)
+        info['eMod'] = [float(x) for x in metrics[:, 0]]
+        info['eCon'] = [float(x) for x in metrics[:, 1]]
+        info['eSup'] = [float(x) for x in metrics[:, 2]]
+        H = eng.heatmap()
+        info['heatmap'] = None if H is None else H.cpu().numpy().astype(r_dtype)
+        if method >= 4:
+            if iters > 0:
+                b = eng.background().cpu().numpy().astype(r_dtype)
+            else:
+                b = np.asarray(bg_init, dtype=r_dtype)
+            info['background'] = np.fft.fftshift(b) ** 2
+        elif background is not None and algorithm == ALGO_ERA:
+            # era.py:311-314 reports a passed background even when it is unused
+            B = _as_numpy(background)
+            info['background'] = np.fft.fftshift(B[0] if B.ndim == 3 else B) ** 2
+        if update == 'O':
+            return O_out, info
+        if update == 'P':
+            return P_out, info
+        return O_out, P_out, info
+    finally:
+        eng.close()
+
+
+def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmod_probe=False,
+        probe_centering=False, hardware='cpu', alpha=1.0e-10, dtype=None, full_output=True,
+        device=None, callback=None, rng=None):
+    """Error-reduction algorithm (ptychography/era.py:13).  ``hardware`` is accepted and
+    ignored, as in the reference; the work always runs on the GPU ``device``.
+
+    Build-only keywords: ``device`` (torch device, default the current GPU),
+    ``callback(i, O, P)`` after every iteration, ``rng`` (object with
+    ``.random(shape)``) in place of ``np.random`` for the default P / background.
+    """
+    return solve(ALGO_ERA, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
+                 background=background, method=method, Pmod_probe=Pmod_probe,
+                 probe_centering=probe_centering, alpha=alpha, dtype=dtype,
+                 full_output=full_output, device=device, callback=callback, rng=rng)
