@@ -37,12 +37,14 @@ OBJ_BYTES_PX = 24
 DM4_BYTES_PX = 76
 
 # algorithmic bytes per frame pixel of each stage's launch (ERA method 1)
-STAGE_BYTES_PX = {
+STAGE_BYTES_PX_STAGED = {
     "rows_fwd": 8,    # write the row spectrum (object window and probe come from cache)
     "cols": 20,       # read spectrum 8 + amplitude 4, write 8
     "rows_inv": 16,   # read 8, write psi' 8
     "gather": 8,      # read psi' 8 (+ object pixel terms below)
 }
+# fused per-frame kernel (timed under the "cols" stage): amplitude 4 in, psi' 8 out
+STAGE_BYTES_PX_FUSED = {"cols": 12, "gather": 8}
 
 
 def parse():
@@ -55,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=48,
                     help="frames in the bounded CPU-baseline sample")
     ap.add_argument("--grid", type=int, default=64, help="raster side (64 -> 4096 positions)")
+    ap.add_argument("--path", choices=["default", "fused", "staged"], default="default")
+    ap.add_argument("--frame-threads", type=int, default=0)
+    ap.add_argument("--frame-slots", type=int, default=0)
     return ap.parse_args()
 
 
@@ -147,6 +152,8 @@ def main():
 
     # ---- ERA method 1 (headline) ----
     eng = Engine(N, M, M, U, V, scan["R"], device=dev)
+    if args.path != "default" or args.frame_threads or args.frame_slots:
+        eng.set_path(args.path != "staged", args.frame_threads, args.frame_slots)
     O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
     eng.run(0, 1, args.warmup, iter0=0, begin=True)
     eng.profile(True)
@@ -161,6 +168,8 @@ def main():
     del amp, O_t, P_t
 
     its = args.steps / dt
+    fused = prof["rows_fwd"][1] == 0
+    STAGE_BYTES_PX = STAGE_BYTES_PX_FUSED if fused else STAGE_BYTES_PX_STAGED
     stage_avg = {k: (ms / n if n else 0.0) for k, (ms, n) in prof.items()}
     dom = max((k for k in prof if k in STAGE_BYTES_PX), key=lambda k: prof[k][0])
     dom_bytes = STAGE_BYTES_PX[dom] * px + (OBJ_BYTES_PX * U * V if dom == "gather" else 0)
@@ -171,6 +180,8 @@ def main():
     dm = None
     if args.dm_steps > 0:
         eng = Engine(N, M, M, U, V, scan["R"], device=dev)
+        if args.path != "default" or args.frame_threads or args.frame_slots:
+            eng.set_path(args.path != "staged", args.frame_threads, args.frame_slots)
         O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
         b0 = np.sqrt(np.fft.ifftshift(scan["B"])).astype(np.float32)
         eng.set_background(torch.from_numpy(b0), per_frame=False)
@@ -207,7 +218,8 @@ def main():
         "config": {"workload": f"era-method1-{N}x{M}x{M}-object{U}x{V}", "frames": N,
                    "frame": [M, M], "object": [U, V], "method": 1, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": ("k_frame (fused rows+columns+modulus)" if fused and dom == "cols" else dom),
                      "kernel_ms": stage_avg[dom],
                      "algorithmic_bytes_per_launch": dom_bytes},
         "iteration_roofline_frac": iter_bytes * its / (HBM_PEAK_GBS * 1e9),

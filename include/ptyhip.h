@@ -104,6 +104,14 @@ int pty_heatmap(pty_ctx* ctx, double* H_dev, int32_t* valid);
 /* Current frame-averaged background (M x K float32, ifftshifted). */
 int pty_background(pty_ctx* ctx, float* b_dev);
 
+/* Choose between the fused per-frame kernel (one work-group transforms a whole
+ * frame through an Infinity-Cache-resident scratch slot; square 128 / 256
+ * frames, methods without background) and the staged row / column kernels.
+ * threads: 0 = keep, 512 or 1024 per work-group; slots: 0 = keep, else the
+ * number of work-groups (and scratch slots).  Default: fused where
+ * supported, one slot per compute unit. */
+int pty_set_path(pty_ctx* ctx, int32_t fused, int32_t threads, int32_t slots);
+
 /* ---- per-stage timing (HIP events on the context stream) ---- */
 enum {
   PTY_STAGE_ROWS_FWD = 0,  /* frame formation + forward row FFT */

@@ -43,6 +43,15 @@ __device__ __forceinline__ float2 mul_negi(float2 a) {
   else return make_float2(a.y, -a.x);
 }
 
+// Work-group barrier that orders LDS traffic only: global loads issued before
+// it stay in flight (a plain __syncthreads also waits for every outstanding
+// global load, which would serialise software-pipelined prefetches).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (I < N) {
@@ -175,8 +184,8 @@ struct LineLds {
 
 // One two-pass transform of a line held in registers v[E] (layoutA<Ra,Rb> in,
 // layoutB<Ra,Rb> out).  `mid` is this line's LDS exchange area, `tw` the
-// table exp(-2 pi i k / L).  Contains one __syncthreads (between the passes);
-// the caller must put a barrier between two calls that reuse `mid`.
+// table exp(-2 pi i k / L).  Contains one LDS barrier (between the passes);
+// the caller must put an LDS barrier between two calls that reuse `mid`.
 template <int L, int Ra, int Rb, int T, bool INV>
 __device__ __forceinline__ void line_fft(float2* v, float2* mid, const float2* __restrict__ tw, int t) {
   static_assert(Ra * Rb == L, "bad factorisation");
@@ -192,7 +201,7 @@ __device__ __forceinline__ void line_fft(float2* v, float2* mid, const float2* _
 #pragma unroll
     for (int r = 0; r < Ra; ++r) mid[j * PAD + r] = v[q * Ra + r];
   });
-  __syncthreads();
+  lds_barrier();
   static_for<0, NQ2>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const int j = t + q * T;

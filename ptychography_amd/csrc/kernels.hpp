@@ -56,6 +56,24 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /*[4*NV]*
   }
 }
 
+// Work-groups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// XCD with its own L2.  Re-number them so that consecutive work items land on
+// the same XCD: neighbouring column blocks share amplitude cache lines,
+// neighbouring frames share object rows, neighbouring tiles share frame rows.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_remap(int b, int grid) {
+  if (grid % NXCD) return b;
+  return (b % NXCD) * (grid / NXCD) + b / NXCD;
+}
+
+// Copy the L-entry twiddle table into LDS (one barrier); line_fft then reads
+// its pass-2 twiddles from LDS instead of global memory.
+template <int L>
+__device__ __forceinline__ void stage_twiddles(float2* s_tw, const float2* __restrict__ tw) {
+  for (int k = threadIdx.x; k < L; k += blockDim.x) s_tw[k] = tw[k];
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // row stage, forward
 // ---------------------------------------------------------------------------
@@ -66,47 +84,65 @@ enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2 };
 //   SRC_DM: 2 O[..] P[i, c] - F[n, i, c]
 //   SRC_F : F[n, i, c]                 (X may alias F)
 template <int L, int SRC>
-__global__ __launch_bounds__(WG, 4) void k_rows_fwd(Geom g, const float2* O, const float2* P,
+__global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, const float2* P,
                                                  const float2* F, float2* X,
                                                  const float2* __restrict__ tw, int nrows) {
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
+  constexpr int E2 = (SRC == SRC_DM) ? E : 1;  // f values (DM)
   __shared__ float2 lds[LPW * W];
+  __shared__ float2 s_tw[L];
+  stage_twiddles<L>(s_tw, tw);
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
-  const int row = blockIdx.x * LPW + line;
-  const bool valid = row < nrows;
   const int M = g.M;
-  const int n = valid ? row / M : 0, i = valid ? row % M : 0;
-  float2 v[E];
-  if (valid) {
+  const int ngroups = (nrows + LPW - 1) / LPW;
+  // raw inputs of the next line group (f, or the object window and f), loaded
+  // one group ahead; the probe row is read from cache when it is needed
+  float2 n0[E], n2[E2];
+  auto load = [&](int grp) {
+    const int row = min(grp * LPW + line, nrows - 1);  // clamped: spare lines re-read a real row
     const size_t rbase = (size_t)row * L;
     if constexpr (SRC == SRC_F) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = F[rbase + idxA<R1, R2, T>(e, t)];
+      for (int e = 0; e < E; ++e) n0[e] = F[rbase + idxA<R1, R2, T>(e, t)];
     } else {
+      const int n = row / M, i = row % M;
       const int2 o = g.off[n];
       const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
-      const float2* Prow = P + (size_t)i * L;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int c = idxA<R1, R2, T>(e, t);
-        float2 psi = cmul(Orow[c], Prow[c]);
-        if constexpr (SRC == SRC_DM) {
-          float2 f = F[rbase + c];
-          psi = make_float2(2.f * psi.x - f.x, 2.f * psi.y - f.y);
-        }
+        n0[e] = Orow[c];
+        if constexpr (SRC == SRC_DM) n2[e] = F[rbase + c];
+      }
+    }
+  };
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  if (wg < ngroups) load(wg);
+  for (int grp = wg; grp < ngroups; grp += gridDim.x) {
+    float2 v[E];
+    if constexpr (SRC == SRC_F) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = n0[e];
+    } else {
+      const int i = min(grp * LPW + line, nrows - 1) % M;
+      const float2* Prow = P + (size_t)i * L;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        float2 psi = cmul(n0[e], Prow[idxA<R1, R2, T>(e, t)]);
+        if constexpr (SRC == SRC_DM) psi = make_float2(2.f * psi.x - n2[e].x, 2.f * psi.y - n2[e].y);
         v[e] = psi;
       }
     }
-  } else {
+    if (grp + (int)gridDim.x < ngroups) load(grp + gridDim.x);
+    line_fft<L, R1, R2, T, false>(v, lds + line * W, s_tw, t);
+    const int row = grp * LPW + line;
+    if (row < nrows) {
+      const size_t rbase = (size_t)row * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
-  }
-  line_fft<L, R1, R2, T, false>(v, lds + line * W, tw, t);
-  if (valid) {
-    const size_t rbase = (size_t)row * L;
-#pragma unroll
-    for (int e = 0; e < E; ++e) X[rbase + idxB<R1, R2, T>(e, t)] = v[e];
+      for (int e = 0; e < E; ++e) X[rbase + idxB<R1, R2, T>(e, t)] = v[e];
+    }
+    lds_barrier();
   }
 }
 
@@ -120,43 +156,57 @@ enum RowDst { DST_STORE = 0, DST_DMUPD = 1 };
 //   DST_DMUPD: Out[n, i, c] += y - O[..] P[i, c]        (DM: f - Ps f + Pm(..))
 //              partial[block] = sum |y - O P|^2          (DM step size)
 template <int L, int DST>
-__global__ __launch_bounds__(WG, 4) void k_rows_inv(Geom g, const float2* Y, float2* Out,
+__global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, float2* Out,
                                                  const float2* O, const float2* P, float scale,
                                                  double* partial, const float2* __restrict__ tw,
                                                  int nrows) {
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[LPW * W];
+  __shared__ float2 s_tw[L];
   __shared__ double red[4];
+  stage_twiddles<L>(s_tw, tw);
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
-  const int row = blockIdx.x * LPW + line;
-  const bool valid = row < nrows;
-  const size_t rbase = (size_t)row * L;
-  float2 v[E];
+  const int ngroups = (nrows + LPW - 1) / LPW;
+  float2 nx[E];
+  auto load = [&](int grp) {
+    const int row = min(grp * LPW + line, nrows - 1);
+    const size_t rbase = (size_t)row * L;
 #pragma unroll
-  for (int e = 0; e < E; ++e)
-    v[e] = valid ? Y[rbase + idxA<R2, R1, T>(e, t)] : make_float2(0.f, 0.f);
-  line_fft<L, R2, R1, T, true>(v, lds + line * W, tw, t);
+    for (int e = 0; e < E; ++e) nx[e] = Y[rbase + idxA<R2, R1, T>(e, t)];
+  };
   double acc[1] = {0.0};
-  if (valid) {
-    if constexpr (DST == DST_STORE) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  if (wg < ngroups) load(wg);
+  for (int grp = wg; grp < ngroups; grp += gridDim.x) {
+    float2 v[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) Out[rbase + idxB<R2, R1, T>(e, t)] = v[e] * scale;
-    } else {
-      const int M = g.M, n = row / M, i = row % M;
-      const int2 o = g.off[n];
-      const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
-      const float2* Prow = P + (size_t)i * L;
+    for (int e = 0; e < E; ++e) v[e] = nx[e];
+    if (grp + (int)gridDim.x < ngroups) load(grp + gridDim.x);
+    line_fft<L, R2, R1, T, true>(v, lds + line * W, s_tw, t);
+    const int row = grp * LPW + line;
+    if (row < nrows) {
+      const size_t rbase = (size_t)row * L;
+      if constexpr (DST == DST_STORE) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int c = idxB<R2, R1, T>(e, t);
-        const float2 ex = cmul(Orow[c], Prow[c]);
-        const float2 y = v[e] * scale;
-        const float2 d = y - ex;
-        acc[0] += (double)norm2(d);
-        Out[rbase + c] = Out[rbase + c] + d;
+        for (int e = 0; e < E; ++e) Out[rbase + idxB<R2, R1, T>(e, t)] = v[e] * scale;
+      } else {
+        const int M = g.M, n = row / M, i = row % M;
+        const int2 o = g.off[n];
+        const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
+        const float2* Prow = P + (size_t)i * L;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int c = idxB<R2, R1, T>(e, t);
+          const float2 ex = cmul(Orow[c], Prow[c]);
+          const float2 y = v[e] * scale;
+          const float2 d = y - ex;
+          acc[0] += (double)norm2(d);
+          Out[rbase + c] = Out[rbase + c] + d;
+        }
       }
     }
+    lds_barrier();
   }
   if constexpr (DST == DST_DMUPD) {
     block_sum<1>(acc, red);
@@ -208,6 +258,10 @@ template <int MODE> struct ColTraits {
   static constexpr bool bsum = MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD;
   static constexpr bool err = MODE == COL_ERA1 || MODE == COL_ERR1 || MODE == COL_ERR7 ||
                               MODE == COL_ERAB;
+  // waves per SIMD the register budget is tuned for (modes that hoist the
+  // amplitude / background loads above the FFT need more registers)
+  static constexpr int wpe = (MODE == COL_ERR1 || MODE == COL_ERR7 || MODE == COL_ERAB ||
+                              MODE == COL_DMB_UPD || MODE == COL_DMB_LEAD) ? 2 : 3;
 };
 
 // Uniform part of the row index held in register slot e (layoutA / layoutB);
@@ -218,144 +272,163 @@ template <int Ra, int Rb, int T>
 __device__ __forceinline__ constexpr int urowB(int e) { return (e / Rb) * T + (e % Rb) * Ra; }
 
 template <int L, int MODE>
-__global__ __launch_bounds__(WG, 3) void k_cols(Geom g, ColArgs a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, WG),
+                          amdgpu_waves_per_eu(ColTraits<MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
   using TR = ColTraits<MODE>;
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = WG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
+  __shared__ float2 s_tw[L];
   __shared__ double red[8];
+  stage_twiddles<L>(s_tw, a.tw);
   const int tid = threadIdx.x, l = tid % CPW, t = tid / CPW;
   const int K = g.K;
-  const int c = blockIdx.x * CPW + l;
-  const bool cv = c < K;
-  const int n0 = blockIdx.y * a.chunk;
-  const int n1 = min(a.nframes, n0 + a.chunk);
-  // element (row t + u, column c) of a frame is at [u*K + toff]: u*K is uniform
-  const uint32_t toff = (uint32_t)(t * K + c);
+  const int ncb = (K + CPW - 1) / CPW;                 // column blocks per frame
+  const int nchunks = (a.nframes + a.chunk - 1) / a.chunk;
+  const int nitems = ncb * nchunks;                    // item = (column block, frame chunk)
   float2* mid = lds + l * W;
   double acc[2] = {0.0, 0.0};
-  double bacc[TR::bsum ? E : 1];
+  // element (row t + u, column c) of a frame is at [u*K + toff]: u*K is uniform
+  float2 nx[E];
+  auto load = [&](int item, int n) {
+    const int c = min((item % ncb) * CPW + l, K - 1);  // clamped: spare columns re-read a real one
+    const uint32_t toff = (uint32_t)(t * K + c);
+    const float2* Xn = a.X + (size_t)n * L * K;
 #pragma unroll
-  for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0.0;
-
-  for (int n = n0; n < n1; ++n) {
-    // opaque, wave-uniform frame index: keeps the frame base in SGPRs instead
-    // of per-element 64-bit induction variables
-    const int nu = __builtin_amdgcn_readfirstlane(n);
-    float2* Xn = a.X + (size_t)nu * L * K;
-    float2 v[E];
-    if (cv) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int u = TR::fwd ? urowA<R1, R2, T>(e) : urowB<R1, R2, T>(e);
-        v[e] = (Xn + (size_t)u * K)[toff];
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
+    for (int e = 0; e < E; ++e) {
+      const int u = TR::fwd ? urowA<R1, R2, T>(e) : urowB<R1, R2, T>(e);
+      nx[e] = (Xn + (size_t)u * K)[toff];
     }
-    if constexpr (TR::fwd) line_fft<L, R1, R2, T, false>(v, mid, a.tw, t);
-
-    if constexpr (MODE != COL_FWD && MODE != COL_INV) {
-      if (cv) {
-        const float* ampn = a.amp ? a.amp + (size_t)nu * a.amp_stride : nullptr;
-        float* bgn = a.bg ? a.bg + (size_t)nu * a.bg_stride : nullptr;
+  };
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  if (wg < nitems) load(wg, (wg / ncb) * a.chunk);
+  for (int item = wg; item < nitems; item += gridDim.x) {
+    const int cb = item % ncb, chunk = item / ncb;
+    const int c = cb * CPW + l;
+    const bool cv = c < K;
+    const uint32_t toff = (uint32_t)(t * K + c);
+    const int n0 = chunk * a.chunk;
+    const int n1 = min(a.nframes, n0 + a.chunk);
+    double bacc[TR::bsum ? E : 1];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int u = urowB<R1, R2, T>(e);
-          const size_t uo = (size_t)u * K;
-          const float2 z = v[e];
-          const float z2 = norm2(z);
-          if constexpr (MODE == COL_INTENS) {
-            const int r = t + u;
-            const int rs = (r + L / 2) % L, cs = (c + K / 2) % K;
-            a.outI[(size_t)nu * L * K + (size_t)rs * K + cs] = z2;
-          } else {
-            const float m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;
-            const float am = (ampn + uo)[toff];
-            if constexpr (MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP) {
-              const float mg = sqrtf(z2) + a.alpha;
-              if constexpr (MODE == COL_ERA1) {
-                const float d = mg - am;
+    for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0.0;
+
+    for (int n = n0; n < n1; ++n) {
+      // opaque, wave-uniform frame index: keeps the frame base in SGPRs
+      const int nu = __builtin_amdgcn_readfirstlane(n);
+      float2* Xn = a.X + (size_t)nu * L * K;
+      float2 v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = nx[e];
+      // prefetch: next frame of this item, else the first frame of the next item
+      if (nu + 1 < n1) {
+        load(item, nu + 1);
+      } else if (item + (int)gridDim.x < nitems) {
+        const int ni = item + gridDim.x;
+        load(ni, (ni / ncb) * a.chunk);
+      }
+      if constexpr (TR::fwd) line_fft<L, R1, R2, T, false>(v, mid, s_tw, t);
+
+      if constexpr (MODE != COL_FWD && MODE != COL_INV) {
+        if (cv) {
+          const float* ampn = a.amp ? a.amp + (size_t)nu * a.amp_stride : nullptr;
+          float* bgn = a.bg ? a.bg + (size_t)nu * a.bg_stride : nullptr;
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int u = urowB<R1, R2, T>(e);
+            const size_t uo = (size_t)u * K;
+            const float2 z = v[e];
+            const float z2 = norm2(z);
+            if constexpr (MODE == COL_INTENS) {
+              const int r = t + u;
+              const int rs = (r + L / 2) % L, cs = (c + K / 2) % K;
+              a.outI[(size_t)nu * L * K + (size_t)rs * K + cs] = z2;
+            } else {
+              const float m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;
+              const float am = (ampn + uo)[toff];
+              if constexpr (MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP) {
+                const float mg = sqrtf(z2) + a.alpha;
+                if constexpr (MODE == COL_ERA1) {
+                  const float d = mg - am;
+                  acc[0] += (double)(m * d * d);
+                }
+                const float fac = (m != 0.f) ? am / mg : 1.f;
+                const float2 zn = z * fac;
+                if constexpr (MODE == COL_ERA1) acc[1] += (double)norm2(zn);
+                v[e] = zn * a.scale;
+              } else if constexpr (MODE == COL_ERR1) {
+                const float d = sqrtf(z2) - am;
                 acc[0] += (double)(m * d * d);
-              }
-              const float fac = (m != 0.f) ? am / mg : 1.f;
-              const float2 zn = z * fac;
-              if constexpr (MODE == COL_ERA1) acc[1] += (double)norm2(zn);
-              v[e] = zn * a.scale;
-            } else if constexpr (MODE == COL_ERR1) {
-              const float d = sqrtf(z2) - am;
-              acc[0] += (double)(m * d * d);
-            } else if constexpr (MODE == COL_ERR7) {
-              const float b = (a.bbar + uo)[toff];
-              const float d = sqrtf(z2 + b * b) - am;
-              acc[0] += (double)(m * d * d);
-            } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
-              float* bp = (bgn + uo) + toff;
-              const float b = *bp;
-              const float mg = sqrtf(z2 + b * b + a.alpha);
-              if constexpr (MODE == COL_ERAB) {
-                const float d = mg - am;
+              } else if constexpr (MODE == COL_ERR7) {
+                const float b = (a.bbar + uo)[toff];
+                const float d = sqrtf(z2 + b * b) - am;
                 acc[0] += (double)(m * d * d);
+              } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
+                float* bp = (bgn + uo) + toff;
+                const float b = *bp;
+                const float mg = sqrtf(z2 + b * b + a.alpha);
+                if constexpr (MODE == COL_ERAB) {
+                  const float d = mg - am;
+                  acc[0] += (double)(m * d * d);
+                }
+                const float fac = m * am / mg;
+                const float bn = b * fac;
+                bacc[e] += (double)bn;
+                if constexpr (MODE == COL_DMB_LEAD) *bp = bn;
+                const float2 zn = z * fac;
+                if constexpr (MODE == COL_ERAB) acc[1] += (double)norm2(zn);
+                v[e] = zn * a.scale;
+              } else if constexpr (MODE == COL_DMB_UPD) {
+                float* bp = (bgn + uo) + toff;
+                const float b = *bp;
+                const float bb = (a.bbar + uo)[toff];
+                const float b0 = 2.f * bb - b;
+                const float mg = sqrtf(z2 + b0 * b0 + a.alpha);
+                const float fac = m * am / mg;
+                const float bn = (b - bb) + b0 * fac;
+                *bp = bn;
+                bacc[e] += (double)bn;
+                v[e] = z * (fac * a.scale);
               }
-              const float fac = m * am / mg;
-              const float bn = b * fac;
-              bacc[e] += (double)bn;
-              if constexpr (MODE == COL_DMB_LEAD) *bp = bn;
-              const float2 zn = z * fac;
-              if constexpr (MODE == COL_ERAB) acc[1] += (double)norm2(zn);
-              v[e] = zn * a.scale;
-            } else if constexpr (MODE == COL_DMB_UPD) {
-              float* bp = (bgn + uo) + toff;
-              const float b = *bp;
-              const float bb = (a.bbar + uo)[toff];
-              const float b0 = 2.f * bb - b;
-              const float mg = sqrtf(z2 + b0 * b0 + a.alpha);
-              const float fac = m * am / mg;
-              const float bn = (b - bb) + b0 * fac;
-              *bp = bn;
-              bacc[e] += (double)bn;
-              v[e] = z * (fac * a.scale);
             }
           }
         }
       }
+
+      if constexpr (TR::store_fwd) {
+        if (cv) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) (Xn + (size_t)urowB<R1, R2, T>(e) * K)[toff] = v[e];
+        }
+      }
+      if constexpr (TR::inv) {
+        if constexpr (TR::fwd) lds_barrier();
+        if constexpr (MODE == COL_INV) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
+        }
+        line_fft<L, R2, R1, T, true>(v, mid, s_tw, t);
+        if (cv) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) (Xn + (size_t)urowA<R1, R2, T>(e) * K)[toff] = v[e];
+        }
+      }
+      lds_barrier();
     }
 
-    if constexpr (TR::store_fwd) {
+    if constexpr (TR::bsum) {
       if (cv) {
+        double* pb = a.partb + (size_t)chunk * L * K;
 #pragma unroll
-        for (int e = 0; e < E; ++e) (Xn + (size_t)urowB<R1, R2, T>(e) * K)[toff] = v[e];
+        for (int e = 0; e < E; ++e) (pb + (size_t)urowB<R1, R2, T>(e) * K)[toff] = bacc[e];
       }
-    }
-    if constexpr (TR::inv) {
-      if constexpr (TR::fwd) __syncthreads();
-      if constexpr (MODE == COL_INV) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
-      }
-      line_fft<L, R2, R1, T, true>(v, mid, a.tw, t);
-      if (cv) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) (Xn + (size_t)urowA<R1, R2, T>(e) * K)[toff] = v[e];
-      }
-    }
-    __syncthreads();
-  }
-
-  if constexpr (TR::bsum) {
-    if (cv) {
-      double* pb = a.partb + (size_t)blockIdx.y * L * K;
-#pragma unroll
-      for (int e = 0; e < E; ++e) (pb + (size_t)urowB<R1, R2, T>(e) * K)[toff] = bacc[e];
     }
   }
   if constexpr (TR::err) {
     block_sum<2>(acc, red);
     if (tid == 0) {
-      const int b = blockIdx.y * gridDim.x + blockIdx.x;
-      if (a.part) a.part[b] = acc[0];
-      if (a.part2) a.part2[b] = acc[1];
+      if (a.part) a.part[blockIdx.x] = acc[0];
+      if (a.part2) a.part2[blockIdx.x] = acc[1];
     }
   }
 }
@@ -377,6 +450,7 @@ struct GatherArgs {
   const int* __restrict__ tile_ptr;     // CSR over object tiles
   const int* __restrict__ tile_frames;  // ascending frame ids per tile
   int tiles_x;
+  int ntiles;
   double* part_econ;       // [2][nblocks]  sum |Oold - O|^2, sum |O|^2
   double* part_esup;       // [2][nblocks]  sum Re(conj(O) num), sum |O|^2 H
 };
@@ -384,10 +458,14 @@ struct GatherArgs {
 // Each object pixel sums conj(P) * S_n over the frames whose window covers it,
 // in ascending frame order, then divides by (sum |P|^2 + alpha) in FP64.
 __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
+  constexpr int FCAP = 512;  // frames staged in LDS per batch
   __shared__ double red[16];
+  __shared__ int s_n[FCAP];
+  __shared__ int2 s_o[FCAP];
   const int tid = threadIdx.x, tx = tid & (TILE_W - 1), ty = tid / TILE_W;
   constexpr int PY = TILE_H / (WG / TILE_W);  // pixels per thread (rows)
-  const int tile = blockIdx.x;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (tile >= a.ntiles) return;  // grid padded to a multiple of the XCD count
   const int u0 = (tile / a.tiles_x) * TILE_H, v = (tile % a.tiles_x) * TILE_W + tx;
   const int M = g.M, K = g.K;
   float2 acc[PY];
@@ -395,21 +473,31 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
 #pragma unroll
   for (int k = 0; k < PY; ++k) { acc[k] = make_float2(0.f, 0.f); hacc[k] = 0.0; }
   const int b0 = a.tile_ptr[tile], b1 = a.tile_ptr[tile + 1];
-  for (int b = b0; b < b1; ++b) {
-    const int n = a.tile_frames[b];
-    const int2 o = g.off[n];
-    const int j = v - o.y;
-    if ((unsigned)j >= (unsigned)K) continue;
-    const float2* Sn = a.S + (size_t)n * M * K + j;
-    const float2* Pj = a.P + j;
+  for (int bs = b0; bs < b1; bs += FCAP) {
+    const int cnt = min(FCAP, b1 - bs);
+    __syncthreads();
+    for (int k = tid; k < cnt; k += WG) {
+      const int n = a.tile_frames[bs + k];
+      s_n[k] = n;
+      s_o[k] = g.off[n];
+    }
+    __syncthreads();
+    for (int b = 0; b < cnt; ++b) {
+      const int n = s_n[b];
+      const int2 o = s_o[b];
+      const int j = v - o.y;
+      if ((unsigned)j >= (unsigned)K) continue;
+      const float2* Sn = a.S + (size_t)n * M * K + j;
+      const float2* Pj = a.P + j;
 #pragma unroll
-    for (int k = 0; k < PY; ++k) {
-      const int i = u0 + ty + k * (WG / TILE_W) - o.x;
-      if ((unsigned)i < (unsigned)M) {
-        const float2 p = Pj[(size_t)i * K];
-        const float2 s = Sn[(size_t)i * K];
-        acc[k] = acc[k] + cmulc(s, p);
-        if (a.fresh) hacc[k] += (double)norm2(p);
+      for (int k = 0; k < PY; ++k) {
+        const int i = u0 + ty + k * (WG / TILE_W) - o.x;
+        if ((unsigned)i < (unsigned)M) {
+          const float2 p = Pj[(size_t)i * K];
+          const float2 s = Sn[(size_t)i * K];
+          acc[k] = acc[k] + cmulc(s, p);
+          if (a.fresh) hacc[k] += (double)norm2(p);
+        }
       }
     }
   }
@@ -440,9 +528,9 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
   if (a.part_econ || a.part_esup) {
     block_sum<4>(r, red);
     if (tid == 0) {
-      const int nb = gridDim.x;
-      if (a.part_econ) { a.part_econ[blockIdx.x] = r[0]; a.part_econ[nb + blockIdx.x] = r[1]; }
-      if (a.part_esup) { a.part_esup[blockIdx.x] = r[2]; a.part_esup[nb + blockIdx.x] = r[3]; }
+      const int nb = a.ntiles;
+      if (a.part_econ) { a.part_econ[tile] = r[0]; a.part_econ[nb + tile] = r[1]; }
+      if (a.part_esup) { a.part_esup[tile] = r[2]; a.part_esup[nb + tile] = r[3]; }
     }
   }
 }

@@ -10,10 +10,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "frame.hpp"
 #include "kernels.hpp"
 #include "ptyhip.h"
 
@@ -95,6 +97,7 @@ struct pty_ctx {
   float2* twK = nullptr;
   // column-stage launch shape
   int col_blocks = 0, col_chunk = 1, col_chunks = 0;
+  int cus = 256;             // compute units of the device (persistent grids)
   // probe-reduction launch shape
   int probe_chunk = 1, probe_chunks = 0;
 
@@ -148,6 +151,14 @@ struct pty_ctx {
   int prof_override = -1;            // >= 0: attribute launches to this stage
   double prof_ms[PTY_NSTAGE] = {};
   int64_t prof_cnt[PTY_NSTAGE] = {};
+
+  // fused per-frame path (square frames of a supported side)
+  bool frame_ok = false;     // geometry supports it
+  bool use_frame = false;    // currently selected
+  int frame_nt = 1024;       // threads per work-group
+  int frame_slots = 0;       // work-groups (= scratch slots)
+  float2* slots = nullptr;   // frame_slots * M*K scratch
+  size_t slots_cap = 0;      // slots allocated
 
   // DM bookkeeping
   bool dm_ready = false;
@@ -203,11 +214,18 @@ void prof_collect(pty_ctx* c) {
 // launch helpers
 // --------------------------------------------------------------------------
 
+// grid of a persistent launch: all items if few, else `cap` rounded down to a
+// multiple of the XCD count (so the XCD-aware renumbering applies)
+int persistent_grid(long items, long cap) {
+  if (items <= cap) return (int)items;
+  return (int)std::max<long>(NXCD, cap / NXCD * NXCD);
+}
+
 template <int SRC>
 int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows) {
   StageTimer tm(c, PTY_STAGE_ROWS_FWD);
   const int T = threads_per_line(c->K), lpw = WG / T;
-  dim3 grid((nrows + lpw - 1) / lpw);
+  dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, 3 * c->cus));
   switch (c->K) {
 #define C(Lv)                                                                              \
   case Lv:                                                                                 \
@@ -224,10 +242,11 @@ int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, floa
 
 template <int DST>
 int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const float2* P,
-             float scale, double* partial, int nrows) {
+             float scale, double* partial, int nrows, int* nparts = nullptr) {
   StageTimer tm(c, PTY_STAGE_ROWS_INV);
   const int T = threads_per_line(c->K), lpw = WG / T;
-  dim3 grid((nrows + lpw - 1) / lpw);
+  dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, 3 * c->cus));
+  if (nparts) *nparts = grid.x;
   switch (c->K) {
 #define C(Lv)                                                                               \
   case Lv:                                                                                  \
@@ -257,23 +276,73 @@ ColArgs col_args(pty_ctx* c) {
   return a;
 }
 
-// returns the number of partials written (gridDim.x * gridDim.y)
+// persistent launch over (column block, frame chunk) items; returns the
+// number of per-work-group partials written (= grid size)
 template <int MODE>
 int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   StageTimer tm(c, PTY_STAGE_COLS);
+  // frame chunks only matter where per-chunk background sums are accumulated
+  if (!ColTraits<MODE>::bsum) a.chunk = 1;
   const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
-  dim3 grid(c->col_blocks, chunks);
+  const long items = (long)c->col_blocks * chunks;
+  const int grid = persistent_grid(items, (long)ColTraits<MODE>::wpe * c->cus);
+  if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
   switch (c->M) {
 #define C(Lv)                                                                                  \
   case Lv:                                                                                     \
-    hipLaunchKernelGGL((k_cols<Lv, MODE>), grid, dim3(WG), 0, c->stream, c->g, a);             \
+    hipLaunchKernelGGL((k_cols<Lv, MODE>), dim3(grid), dim3(WG), 0, c->stream, c->g, a);       \
     break;
     PTY_SIZES(C)
 #undef C
     default: return fail(PTY_EUNSUP, "unsupported M");
   }
   PTY_LAUNCHED("k_cols");
-  if (nparts) *nparts = c->col_blocks * chunks;
+  if (nparts) *nparts = grid;
+  return PTY_OK;
+}
+
+bool frame_supported(int M, int K) { return M == K && (M == 128 || M == 256); }
+
+int ensure_slots(pty_ctx* c) {
+  const size_t need = (size_t)c->frame_slots * c->MK();
+  if (c->slots_cap >= need) return PTY_OK;
+  dev_free(c->slots);
+  c->slots_cap = 0;
+  PTY_TRY(dev_alloc(&c->slots, need));
+  c->slots_cap = need;
+  return PTY_OK;
+}
+
+FrameArgs frame_args(pty_ctx* c) {
+  FrameArgs a{};
+  a.slots = c->slots;
+  a.amp = c->amp;
+  a.mask = c->mask;
+  a.scale = 1.0f / (float)c->MK();
+  a.tw = c->twK;
+  return a;
+}
+
+// one launch of the fused per-frame kernel over all frames; returns the
+// number of per-work-group partials it wrote
+template <int MODE>
+int frame(pty_ctx* c, FrameArgs a, int* nparts = nullptr) {
+  StageTimer tm(c, PTY_STAGE_COLS);
+  PTY_TRY(ensure_slots(c));
+  a.slots = c->slots;
+  const int grid = std::min(c->frame_slots, c->N);
+  const int key = c->M * 10000 + c->frame_nt;
+  switch (key) {
+#define C(Lv, NTv)                                                                           \
+  case Lv * 10000 + NTv:                                                                     \
+    hipLaunchKernelGGL((k_frame<Lv, NTv, MODE>), dim3(grid), dim3(NTv), 0, c->stream, c->g, a); \
+    break;
+    C(128, 512) C(128, 1024) C(256, 512) C(256, 1024)
+#undef C
+    default: return fail(PTY_EUNSUP, "fused frame path: unsupported size / threads");
+  }
+  PTY_LAUNCHED("k_frame");
+  if (nparts) *nparts = grid;
   return PTY_OK;
 }
 
@@ -292,9 +361,11 @@ int gather(pty_ctx* c, const float2* S, const float2* P, const float2* Oold, flo
   a.tile_ptr = c->tile_ptr;
   a.tile_frames = c->tile_frames;
   a.tiles_x = c->tiles_x;
+  a.ntiles = c->ntiles;
   a.part_econ = econ ? c->part_econO : nullptr;
   a.part_esup = esup ? c->part_esxh : nullptr;
-  hipLaunchKernelGGL(k_gather_O, dim3(c->ntiles), dim3(WG), 0, c->stream, c->g, a);
+  const int grid = (c->ntiles + NXCD - 1) / NXCD * NXCD;
+  hipLaunchKernelGGL(k_gather_O, dim3(grid), dim3(WG), 0, c->stream, c->g, a);
   PTY_LAUNCHED("k_gather_O");
   return PTY_OK;
 }
@@ -442,6 +513,17 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   }
 
   // modulus projection: psi' = IFFT(Pmod(FFT(O_n P)))
+  int ncol = 0;
+  if (c->use_frame && !bgm) {
+    FrameArgs f = frame_args(c);
+    f.O = c->O;
+    f.P = c->P;
+    f.Xout = c->X;
+    f.alpha = o.alpha;
+    f.part = c->part_emod;
+    f.part2 = esup_id ? c->part_esa : nullptr;
+    PTY_TRY(frame<FR_ERA1>(c, f, &ncol));
+  } else {
   PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
   ColArgs a = col_args(c);
   a.X = c->X;
@@ -450,7 +532,6 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   a.alpha = o.alpha;
   a.part = c->part_emod;
   a.part2 = esup_id ? c->part_esa : nullptr;
-  int ncol = 0;
   if (bgm) {
     a.bg = c->era_bg_shared ? c->bbar : c->bg;
     a.bg_stride = c->era_bg_shared ? 0 : (long)MK;
@@ -460,6 +541,7 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     PTY_TRY(cols<COL_ERA1>(c, a, &ncol));
   }
   PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->X, nullptr, nullptr, 1.0f, nullptr, nrows));
+  }
 
   // consistency projection
   if (upd == 0) {
@@ -596,6 +678,16 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   if (pmodp) PTY_TRY(pmod_probe(c, c->P, o.alpha));
 
   // f <- f - Ps f + Pm(2 Ps f - f)    (dm.py:91-97, :216-224)
+  int nstep = 0;
+  if (c->use_frame && !bgm) {
+    FrameArgs f = frame_args(c);
+    f.O = c->O;
+    f.P = c->P;
+    f.F = c->F;
+    f.alpha = o.alpha;
+    f.part2 = c->part_esup;
+    PTY_TRY(frame<FR_DM1>(c, f, &nstep));
+  } else {
   PTY_TRY(rows_fwd<SRC_DM>(c, c->O, c->P, c->F, c->X, nrows));
   {
     ColArgs a = col_args(c);
@@ -613,8 +705,8 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
       PTY_TRY(cols<COL_DM1>(c, a));
     }
   }
-  PTY_TRY(rows_inv<DST_DMUPD>(c, c->X, c->F, c->O, c->P, 1.0f, c->part_esup, nrows));
-  const int nrb = row_blocks(c, nrows);
+  PTY_TRY(rows_inv<DST_DMUPD>(c, c->X, c->F, c->O, c->P, 1.0f, c->part_esup, nrows, &nstep));
+  }
 
   // metrics: Os, Ps = Psup(f)  (dm.py:104-137)
   {
@@ -625,9 +717,15 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   PTY_TRY(dm_project(c, o, it, upd, op_step, c->Os, c->Ps, true));
   if (pmodp) PTY_TRY(pmod_probe(c, c->Ps, o.alpha));
   if (bgm) PTY_TRY(bg_mean(c, nchunks, c->bbar_m));
-  PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->Ps, nullptr, c->X, nrows));
   int ncol = 0;
-  {
+  if (c->use_frame && !bgm) {
+    FrameArgs f = frame_args(c);
+    f.O = c->Os;
+    f.P = c->Ps;
+    f.part = c->part_emod;
+    PTY_TRY(frame<FR_ERR1>(c, f, &ncol));
+  } else {
+    PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->Ps, nullptr, c->X, nrows));
     ColArgs a = col_args(c);
     a.X = c->X;
     a.amp = c->amp;
@@ -656,7 +754,7 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = ECON_BLOCKS;
   }
   rg.p[S_ESUP_E] = c->part_esup;
-  rg.n[S_ESUP_E] = nrb;
+  rg.n[S_ESUP_E] = nstep;
   PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, metrics_out));
   {
     StageTimer tm(c, PTY_STAGE_MISC);
@@ -736,10 +834,26 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   c->probe_chunk = (N + c->probe_chunks - 1) / c->probe_chunks;
   c->probe_chunks = (N + c->probe_chunk - 1) / c->probe_chunk;
 
-  c->cap_col = c->col_blocks * c->col_chunks;
+  c->frame_ok = frame_supported(M, K);
+  {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceProp_t prop;
+      if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    }
+    const char* e_nt = getenv("PTY_FRAME_NT");
+    const char* e_sl = getenv("PTY_FRAME_SLOTS");
+    const char* e_on = getenv("PTY_FRAME");
+    c->frame_nt = (e_nt && atoi(e_nt) == 512) ? 512 : 1024;
+    c->frame_slots = e_sl ? std::max(1, atoi(e_sl)) : cus;
+    c->cus = cus;
+    c->use_frame = c->frame_ok && !(e_on && atoi(e_on) == 0);
+  }
+  c->cap_col = std::max(4 * c->cus, c->frame_slots);
   c->cap_econO = std::max(c->ntiles, ECON_BLOCKS);
   c->cap_econP = std::max(pblocks, ECON_BLOCKS);
-  c->cap_esup = std::max(ESUP_BLOCKS, row_blocks(c, N * M));
+  c->cap_esup = std::max(std::max(ESUP_BLOCKS, row_blocks(c, N * M)), c->frame_slots);
 
   int r;
   if ((r = dev_alloc(&c->off, N)) != PTY_OK) return cleanup(r);
@@ -798,7 +912,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->Obak); dev_free(c->Pbak); dev_free(c->Os); dev_free(c->Oroll); dev_free(c->Ps);
   dev_free(c->H); dev_free(c->G); dev_free(c->bg); dev_free(c->bbar); dev_free(c->bbar_m);
   dev_free(c->pamp); dev_free(c->partb); dev_free(c->pnum); dev_free(c->pg);
-  dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal);
+  dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -918,6 +1032,22 @@ int pty_run(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t
   return PTY_OK;
 }
 
+int pty_set_path(pty_ctx* c, int32_t fused, int32_t threads, int32_t slots) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (fused && !c->frame_ok)
+    return fail(PTY_EUNSUP, "the fused per-frame path needs square 128 or 256 frames");
+  if (threads != 0 && threads != 512 && threads != 1024)
+    return fail(PTY_EINVAL, "threads must be 0 (keep), 512 or 1024");
+  if (slots < 0) return fail(PTY_EINVAL, "slots < 0");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->use_frame = fused != 0;
+  if (threads) c->frame_nt = threads;
+  if (slots) c->frame_slots = slots;
+  if (c->frame_slots > c->cap_col || c->frame_slots > c->cap_esup)
+    return fail(PTY_EINVAL, "slots exceed the partial-sum capacity of this context");
+  return PTY_OK;
+}
+
 int pty_profile(pty_ctx* c, int32_t enable) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   PTY_HIP(hipStreamSynchronize(c->stream));
@@ -945,11 +1075,19 @@ int pty_intensity(pty_ctx* c, float* I) {
   const bool dm = c->last_algo == 1 && c->have_metric_state;
   const float2* Ob = dm ? c->Os : c->O;
   const float2* Pb = dm ? c->Ps : c->P;
-  PTY_TRY(rows_fwd<SRC_OP>(c, Ob, Pb, nullptr, c->X, c->N * c->M));
-  ColArgs a = col_args(c);
-  a.X = c->X;
-  a.outI = I;
-  PTY_TRY(cols<COL_INTENS>(c, a));
+  if (c->use_frame) {
+    FrameArgs f = frame_args(c);
+    f.O = Ob;
+    f.P = Pb;
+    f.outI = I;
+    PTY_TRY(frame<FR_INTENS>(c, f));
+  } else {
+    PTY_TRY(rows_fwd<SRC_OP>(c, Ob, Pb, nullptr, c->X, c->N * c->M));
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.outI = I;
+    PTY_TRY(cols<COL_INTENS>(c, a));
+  }
   PTY_HIP(hipStreamSynchronize(c->stream));
   return PTY_OK;
 }

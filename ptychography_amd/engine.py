@@ -29,6 +29,11 @@ def require_gpu(device=None):
     return dev
 
 
+# Path override for every new Engine: None = library default (fused per-frame
+# kernel where the frame shape supports it), False = staged kernels, True = fused.
+FUSED = None
+
+
 class Engine:
     """A scan geometry on one GPU: frames, object window offsets and workspace.
 
@@ -49,6 +54,8 @@ class Engine:
             stream = torch.cuda.current_stream(self.device)
             call("pty_set_stream", self._ctx, ctypes.c_void_p(stream.cuda_stream))
         self._keep = {}
+        if FUSED is not None:
+            self.set_path(FUSED)
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -114,6 +121,10 @@ class Engine:
         ptr = out.ctypes.data_as(ctypes.c_void_p) if (metrics and iters > 0) else None
         call("pty_run", self._ctx, ctypes.byref(o), int(iter0), int(iters), int(bool(begin)), ptr)
         return out
+
+    def set_path(self, fused, threads=0, slots=0):
+        """Select the fused per-frame kernel (True) or the staged kernels (False)."""
+        call("pty_set_path", self._ctx, int(bool(fused)), int(threads), int(slots))
 
     def profile(self, enable=True):
         call("pty_profile", self._ctx, int(bool(enable)))
