@@ -108,9 +108,16 @@ int pty_background(pty_ctx* ctx, float* b_dev);
  * frame through an Infinity-Cache-resident scratch slot; square 128 / 256
  * frames, methods without background) and the staged row / column kernels.
  * threads: 0 = keep, 512 or 1024 per work-group; slots: 0 = keep, else the
- * number of work-groups (and scratch slots).  Default: fused where
- * supported, one slot per compute unit. */
+ * number of work-groups (and scratch slots).  Default: fused for 128 x 128
+ * frames (staged for 256 x 256, where it measured faster), one slot per
+ * compute unit. */
 int pty_set_path(pty_ctx* ctx, int32_t fused, int32_t threads, int32_t slots);
+
+/* enable != 0: pty_run captures each distinct kind of iteration (its branch
+ * decisions and buffer addresses) once as a hipGraph on a private stream and
+ * replays it; results are identical to eager launches.  Stage timing is not
+ * recorded while graphs are on. */
+int pty_set_graphs(pty_ctx* ctx, int32_t enable);
 
 /* ---- per-stage timing (HIP events on the context stream) ---- */
 enum {

@@ -49,6 +49,7 @@ SIGNATURES = {
     "pty_heatmap": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_i32)]),
     "pty_background": (ctypes.c_int, [_vp, _vp]),
     "pty_set_path": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
+    "pty_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "pty_profile": (ctypes.c_int, [_vp, _i32]),
     "pty_profile_read": (ctypes.c_int, [_vp, _vp, _vp]),
     "pty_fft2": (ctypes.c_int, [_vp, _vp, _i32, _i32]),

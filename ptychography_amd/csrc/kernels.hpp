@@ -765,9 +765,10 @@ struct Regions {
 
 enum EsupMode { ESUP_NONE = 0, ESUP_EXPLICIT = 1, ESUP_IDENTITY = 2 };
 
-// metrics[0..2] = eMod, eCon, eSup for one iteration
+// metrics[0..2] (or, with a slot counter, metrics[3*slot .. 3*slot+2] followed
+// by slot += 1, so one captured graph serves every iteration)
 __global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, double MK,
-                                                  int esup_mode, double* metrics) {
+                                                  int esup_mode, double* metrics, int* slot) {
   __shared__ double red[16 * NSLOT];
   double s[NSLOT];
 #pragma unroll
@@ -779,6 +780,10 @@ __global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, dou
   }
   block_sum<NSLOT>(s, red);
   if (threadIdx.x == 0) {
+    if (slot) {
+      metrics += 3 * (size_t)(*slot);
+      *slot += 1;
+    }
     metrics[0] = sqrt(s[S_EMOD] / I_norm);
     const double den = s[S_ECON_OD] + s[S_ECON_PD];
     metrics[1] = sqrt((s[S_ECON_ON] + s[S_ECON_PN]) / den);

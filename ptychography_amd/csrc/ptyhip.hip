@@ -160,6 +160,20 @@ struct pty_ctx {
   float2* slots = nullptr;   // frame_slots * M*K scratch
   size_t slots_cap = 0;      // slots allocated
 
+  // hipGraph replay of whole iterations
+  struct GraphEntry {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+    bool H_valid, G_valid, era_bg_shared, have_metric_state;
+    float2* F;
+    float2* X;
+  };
+  bool graphs = false;
+  hipStream_t gstream = nullptr;     // default-priority stream used for capture / replay
+  int* slot = nullptr;               // device metrics slot counter
+  bool use_slot = false;             // k_metrics writes through the slot counter
+  std::vector<GraphEntry> gcache;
+
   // DM bookkeeping
   bool dm_ready = false;
   bool have_metric_state = false;
@@ -458,7 +472,7 @@ int econ_pass(pty_ctx* c, const float2* cur, const float2* bak, size_t count, do
 int metrics_launch(pty_ctx* c, const Regions& rg, int esup_mode, double* out) {
   StageTimer tm(c, PTY_STAGE_REDUCE);
   hipLaunchKernelGGL(k_metrics, dim3(1), dim3(1024), 0, c->stream, rg, c->I_norm, (double)c->MK(),
-                     esup_mode, out);
+                     esup_mode, out, c->use_slot ? c->slot : nullptr);
   PTY_LAUNCHED("k_metrics");
   return PTY_OK;
 }
@@ -765,6 +779,80 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   return PTY_OK;
 }
 
+// Host decisions and buffer addresses that shape one iteration's launches.
+std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
+  const int upd = (o.method - 1) % 3;
+  auto bits = [](double d) { uint64_t u; std::memcpy(&u, &d, 8); return u; };
+  return {(uint64_t)o.algorithm, (uint64_t)o.method, (uint64_t)o.op_n,
+          (uint64_t)(upd == 2 && (it % o.op_every) == 0), (uint64_t)(it < o.pmod_probe_until),
+          (uint64_t)o.probe_centering, bits((double)o.alpha), bits(c->I_norm),
+          (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
+          (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
+          (uint64_t)(uintptr_t)c->O, (uint64_t)(uintptr_t)c->P, (uint64_t)(uintptr_t)c->F,
+          (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
+          (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
+          (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp};
+}
+
+void drop_graphs(pty_ctx* c) {
+  for (auto& e : c->gcache) (void)hipGraphExecDestroy(e.exec);
+  c->gcache.clear();
+}
+
+// Iterations as replays of captured graphs, one graph per distinct iteration
+// kind (cached with the host-state transitions the iteration implies).
+int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
+  if (!c->gstream) PTY_HIP(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+  if (!c->slot) PTY_TRY(dev_alloc(&c->slot, 1));
+  hipStream_t user = c->stream;
+  PTY_HIP(hipStreamSynchronize(user));
+  const int zero = 0;
+  PTY_HIP(hipMemcpy(c->slot, &zero, sizeof(int), hipMemcpyHostToDevice));
+  c->stream = c->gstream;
+  c->use_slot = true;
+  const bool prof = c->prof_on;
+  c->prof_on = false;  // stage events are not recorded inside graphs
+  struct Restore {
+    pty_ctx* c; hipStream_t s; bool prof;
+    ~Restore() { c->stream = s; c->use_slot = false; c->prof_on = prof; }
+  } restore{c, user, prof};
+  for (int32_t k = 0; k < iters; ++k) {
+    const int64_t it = iter0 + k;
+    std::vector<uint64_t> key = iter_key(c, o, it);
+    pty_ctx::GraphEntry* hit = nullptr;
+    for (auto& e : c->gcache)
+      if (e.key == key) { hit = &e; break; }
+    if (!hit) {
+      PTY_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int r = o.algorithm == 0 ? era_iter(c, o, it, c->metrics) : dm_iter(c, o, it, c->metrics);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (r != PTY_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      if (ec != hipSuccess) return fail(PTY_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return fail(PTY_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      c->gcache.push_back({key, exec, c->H_valid, c->G_valid, c->era_bg_shared,
+                           c->have_metric_state, c->F, c->X});
+      hit = &c->gcache.back();
+    } else {
+      c->H_valid = hit->H_valid;
+      c->G_valid = hit->G_valid;
+      c->era_bg_shared = hit->era_bg_shared;
+      c->have_metric_state = hit->have_metric_state;
+      c->F = hit->F;
+      c->X = hit->X;
+    }
+    PTY_HIP(hipGraphLaunch(hit->exec, c->stream));
+  }
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
 void twiddles(int L, std::vector<float2>& out) {
   out.resize(L);
   for (int k = 0; k < L; ++k) {
@@ -848,7 +936,9 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->frame_nt = (e_nt && atoi(e_nt) == 512) ? 512 : 1024;
     c->frame_slots = e_sl ? std::max(1, atoi(e_sl)) : cus;
     c->cus = cus;
-    c->use_frame = c->frame_ok && !(e_on && atoi(e_on) == 0);
+    // measured: at 256^2 the staged kernels are faster than the fused one
+    c->use_frame = c->frame_ok && M == 128 && !(e_on && atoi(e_on) == 0);
+    if (e_on && atoi(e_on) == 1) c->use_frame = c->frame_ok;
   }
   c->cap_col = std::max(4 * c->cus, c->frame_slots);
   c->cap_econO = std::max(c->ntiles, ECON_BLOCKS);
@@ -916,6 +1006,9 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  drop_graphs(c);
+  if (c->gstream) (void)hipStreamDestroy(c->gstream);
+  dev_free(c->slot);
   delete c;
   return PTY_OK;
 }
@@ -1019,10 +1112,15 @@ int pty_run(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t
   }
   if (o->algorithm == 1 && !c->dm_ready) return fail(PTY_ESTATE, "DM run needs begin=1 first");
   c->last_algo = o->algorithm;
-  for (int32_t k = 0; k < iters; ++k) {
-    const int64_t it = iter0 + k;
-    if (o->algorithm == 0) PTY_TRY(era_iter(c, *o, it, c->metrics + 3 * (size_t)k));
-    else PTY_TRY(dm_iter(c, *o, it, c->metrics + 3 * (size_t)k));
+  if (c->use_frame) PTY_TRY(ensure_slots(c));  // never allocate inside a capture
+  if (c->graphs) {
+    PTY_TRY(run_graphs(c, *o, iter0, iters));
+  } else {
+    for (int32_t k = 0; k < iters; ++k) {
+      const int64_t it = iter0 + k;
+      if (o->algorithm == 0) PTY_TRY(era_iter(c, *o, it, c->metrics + 3 * (size_t)k));
+      else PTY_TRY(dm_iter(c, *o, it, c->metrics + 3 * (size_t)k));
+    }
   }
   if (metrics_host && iters > 0)
     PTY_HIP(hipMemcpyAsync(metrics_host, c->metrics, 3 * (size_t)iters * sizeof(double),
@@ -1045,6 +1143,14 @@ int pty_set_path(pty_ctx* c, int32_t fused, int32_t threads, int32_t slots) {
   if (slots) c->frame_slots = slots;
   if (c->frame_slots > c->cap_col || c->frame_slots > c->cap_esup)
     return fail(PTY_EINVAL, "slots exceed the partial-sum capacity of this context");
+  return PTY_OK;
+}
+
+int pty_set_graphs(pty_ctx* c, int32_t enable) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->graphs = enable != 0;
+  if (!c->graphs) drop_graphs(c);
   return PTY_OK;
 }
 

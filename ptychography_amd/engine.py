@@ -126,6 +126,10 @@ class Engine:
         """Select the fused per-frame kernel (True) or the staged kernels (False)."""
         call("pty_set_path", self._ctx, int(bool(fused)), int(threads), int(slots))
 
+    def set_graphs(self, enable=True):
+        """Replay each kind of iteration as a captured hipGraph."""
+        call("pty_set_graphs", self._ctx, int(bool(enable)))
+
     def profile(self, enable=True):
         call("pty_profile", self._ctx, int(bool(enable)))
 

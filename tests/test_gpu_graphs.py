@@ -1,0 +1,68 @@
+"""hipGraph replay of iterations gives bit-identical results to eager launches,
+across several pty_run calls with eager torch work and synchronisations in
+between (each call replays cached graphs)."""
+
+import numpy as np
+import pytest
+import torch
+
+from ptychography_amd.engine import Engine
+from tests.scans import broadband_start, flat_start, small_scan
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(s, dev, algorithm, method, graphs, P0, chunks=(3, 4, 5), bg=None, **kw):
+    N, M, K = s.I.shape
+    eng = Engine(N, M, K, *s.O.shape, s.R.astype(np.int32), device=dev)
+    eng.set_graphs(graphs)
+    amp, I_norm = eng.prepare_amp(torch.from_numpy(s.I.astype(np.float32)), None)
+    eng.set_data(amp, None, I_norm)
+    O_t = torch.from_numpy(broadband_start(s.O.shape).astype(np.complex64)).to(dev)
+    P_t = torch.from_numpy(P0.astype(np.complex64)).to(dev)
+    eng.set_state(O_t, P_t)
+    if bg is not None:
+        eng.set_background(torch.from_numpy(bg.astype(np.float32)), per_frame=False)
+    rows, it = [], 0
+    for ci, n in enumerate(chunks):
+        rows.append(eng.run(algorithm, method, n, iter0=it, begin=(ci == 0), **kw))
+        it += n
+        junk = torch.randn(1000, device=dev).sum()   # eager work between replays
+        torch.cuda.synchronize()
+        del junk
+    out = (np.concatenate(rows), O_t.cpu().numpy(), P_t.cpu().numpy())
+    eng.close()
+    return out
+
+
+@pytest.mark.parametrize("frame", [64, 128])
+def test_era_graphs_bit_identical(frame, gpu_device):
+    print(f"graphs ERA frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=(4, 4), step=frame // 5, jitter=1)
+    P0 = flat_start(s)
+    kw = dict(OP_iters=(1, 2), Pmod_probe_until=5)
+    a = _run(s, gpu_device, 0, 3, False, P0, **kw)
+    b = _run(s, gpu_device, 0, 3, True, P0, **kw)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_dm_background_graphs_bit_identical(gpu_device):
+    print("graphs DM method 4", flush=True)
+    s = small_scan(frame=32, grid=(4, 4), step=6, jitter=1, background_frac=0.02)
+    bg = np.sqrt(np.random.default_rng(2).random((32, 32)) * 0.5)
+    a = _run(s, gpu_device, 1, 4, False, s.P, bg=bg)
+    b = _run(s, gpu_device, 1, 4, True, s.P, bg=bg)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_dm_centering_graphs_bit_identical(gpu_device):
+    print("graphs DM method 3 with centring", flush=True)
+    s = small_scan(frame=32, grid=(4, 4), step=6, jitter=1)
+    P0 = np.roll(flat_start(s), (2, 3), axis=(0, 1))
+    kw = dict(OP_iters=(1, 1), probe_centering=True)
+    a = _run(s, gpu_device, 1, 3, False, P0, **kw)
+    b = _run(s, gpu_device, 1, 3, True, P0, **kw)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
