@@ -162,6 +162,43 @@ int pty_psup_O(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* P_de
 int pty_psup_P(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* O_dev,
                pty_cfloat* P_out_dev, double* G_dev, int32_t fresh, float alpha);
 
+/* ---- learned initialiser (no counterpart in the reference; the north-star
+ * conv encoder-decoder whose output seeds O) -------------------------------
+ * bf16 tensors are uint16 bit patterns; activations NHWC; weights
+ * [9*Cin][Cout] (tap-major, tap = 3*dy + dx).  stream: hipStream_t or NULL. */
+const char* pty_nn_last_error(void);
+/* C (M x N, fp32) = A (M x K) B (K x N), bf16 row-major: MFMA lane-layout probe. */
+int pty_nn_gemm_probe(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+                      void* stream);
+/* y = act(conv3x3(x, w) + bias), zero padding; y (bf16) and/or yf (fp32). */
+int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y,
+                       float* yf, int B, int H, int W, int Cin, int Cout, int relu, void* stream);
+/* dx = conv3x3^T(dy, w) */
+int pty_nn_conv3x3_dgrad(const uint16_t* dy, const uint16_t* w, uint16_t* dx, int B, int H, int W,
+                         int Cin, int Cout, void* stream);
+/* dw (fp32 [9*Cin][Cout]) and db (fp32 [Cout], optional); ws: splits*9*Cin*Cout floats. */
+int pty_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, float* ws,
+                         int splits, int B, int H, int W, int Cin, int Cout, void* stream);
+int pty_nn_relu_bwd(uint16_t* dy, const uint16_t* y, size_t n, void* stream);
+/* 2x2 average pool (backward = 1/4 spread), nearest 2x upsample (backward = sum). */
+int pty_nn_pool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, int backward,
+                 void* stream);
+int pty_nn_up2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, int backward,
+               void* stream);
+/* loss partials (1024 doubles) of sum w (p-t)^2, w = wgt[i % period] (or 1);
+ * dy = gscale * 2 w (p-t) / n (bf16, optional). */
+int pty_nn_mse(const float* p, const float* t, size_t n, uint16_t* dy, float gscale, double* part,
+               const float* wgt, size_t period, void* stream);
+int pty_nn_adam(float* w, uint16_t* wb, const float* g, float* m, float* v, size_t n, float lr,
+                float b1, float b2, float eps, int step, float gscale, void* stream);
+int pty_nn_to_bf16(const float* x, uint16_t* y, size_t n, void* stream);
+/* O = weighted, phase-aligned overlap average of N complex M x K patches at
+ * R's windows (patch pixel weights pw, or 1); object pixels whose summed
+ * weight is <= floor get `fill`.  acc / wsum: U*V workspace. */
+int pty_nn_stitch(const pty_cfloat* patches, const float* pw, const int32_t* R_host, int N, int M,
+                  int K, int U, int V, float floor, float fill, pty_cfloat* acc, float* wsum,
+                  pty_cfloat* O, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,23 @@ SIGNATURES = {
     "pty_pmod1": (ctypes.c_int, [_vp, _vp, _f32, ctypes.POINTER(_f64)]),
     "pty_psup_O": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
+    "pty_nn_last_error": (ctypes.c_char_p, []),
+    "pty_nn_gemm_probe": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "pty_nn_conv3x3_fwd": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32,
+                                          _i32, _vp]),
+    "pty_nn_conv3x3_dgrad": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "pty_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32,
+                                            _i32, _vp]),
+    "pty_nn_relu_bwd": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "pty_nn_pool2": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "pty_nn_up2": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "pty_nn_mse": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp, _f32, _vp, _vp, ctypes.c_size_t,
+                                  _vp]),
+    "pty_nn_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _f32, _f32, _f32, _f32,
+                                   _i32, _f32, _vp]),
+    "pty_nn_to_bf16": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "pty_nn_stitch": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _f32, _f32, _vp,
+                                     _vp, _vp, _vp]),
 }
 
 SUPPORTED_FRAMES = (16, 32, 64, 96, 128, 192, 256)
@@ -94,7 +111,8 @@ def load(path=None):
 
 def check(fn_name, code):
     if code != PTY_OK:
-        msg = load().pty_last_error()
+        lib = load()
+        msg = lib.pty_nn_last_error() if fn_name.startswith("pty_nn_") else lib.pty_last_error()
         raise PtyError(fn_name, code, msg.decode() if msg else "")
 
 

@@ -11,7 +11,7 @@ import time
 
 HERE = pathlib.Path(__file__).resolve().parent
 ROOT = HERE.parent
-SRC = HERE / "csrc" / "ptyhip.hip"
+SRCS = [HERE / "csrc" / "ptyhip.hip", HERE / "csrc" / "nn.hip"]
 LIB = HERE / "libptyhip.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -30,9 +30,10 @@ FLAGS = [
 
 
 def build(verbose=False):
-    """Compile ``csrc/ptyhip.hip`` into ``libptyhip.so`` (always recompiles)."""
+    """Compile ``csrc/ptyhip.hip`` and ``csrc/nn.hip`` into ``libptyhip.so`` (always recompiles)."""
     tmp = LIB.with_suffix(".so.tmp%d" % os.getpid())
-    cmd = [HIPCC, *FLAGS, f"-I{ROOT / 'include'}", f"-I{HERE / 'csrc'}", str(SRC), "-o", str(tmp)]
+    cmd = [HIPCC, *FLAGS, f"-I{ROOT / 'include'}", f"-I{HERE / 'csrc'}", *map(str, SRCS), "-o",
+           str(tmp)]
     t0 = time.time()
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

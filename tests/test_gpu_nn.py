@@ -1,0 +1,199 @@
+"""BF16 MFMA kernels of the learned initialiser against PyTorch CPU float64.
+
+Inputs and weights are bf16-representable, so the float64 references see the
+exact operands; outputs compared in fp32 hold to 1e-5 relative (FP32
+accumulation), bf16 outputs to 8e-3 (one bf16 rounding).
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from ptychography_amd import _abi
+from ptychography_amd.learned import InitNet, initial_object, training_frames, train
+
+pytestmark = pytest.mark.gpu
+vp = ctypes.c_void_p
+
+
+def P(t):
+    return vp(t.data_ptr()) if t is not None else None
+
+
+def bf(x, dev):
+    return torch.from_numpy(np.asarray(x, np.float32)).to(torch.bfloat16).to(dev)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (96, 80, 72), (33, 17, 200)])
+def test_mfma_gemm_probe(M, N, K, gpu_device):
+    rng = np.random.default_rng(M + N + K)
+    A = bf(rng.standard_normal((M, K)), gpu_device)
+    B = bf(rng.standard_normal((K, N)), gpu_device)
+    C = torch.empty((M, N), dtype=torch.float32, device=gpu_device)
+    _abi.call("pty_nn_gemm_probe", P(A), P(B), P(C), M, N, K, None)
+    torch.cuda.synchronize()
+    ref = A.double().cpu().numpy() @ B.double().cpu().numpy()
+    assert rel(C.cpu().numpy(), ref) < 1e-6
+
+
+def _conv_ref(x, w, b, relu):
+    """x (B,H,W,Cin), w (9*Cin, Cout) -> (B,H,W,Cout), float64 CPU."""
+    Bn, H, W, Cin = x.shape
+    Cout = w.shape[1]
+    wt = torch.from_numpy(w).reshape(3, 3, Cin, Cout).permute(3, 2, 0, 1).contiguous()
+    xt = torch.from_numpy(x).permute(0, 3, 1, 2)
+    y = torch.nn.functional.conv2d(xt, wt, torch.from_numpy(b) if b is not None else None, padding=1)
+    if relu:
+        y = torch.relu(y)
+    return y.permute(0, 2, 3, 1).numpy()
+
+
+@pytest.mark.parametrize("B,H,W,Cin,Cout", [(2, 16, 16, 1, 32), (2, 8, 8, 32, 64), (1, 16, 8, 64, 16)])
+def test_conv_forward_dgrad_wgrad(B, H, W, Cin, Cout, gpu_device):
+    rng = np.random.default_rng(Cin * 7 + Cout)
+    x = bf(rng.standard_normal((B, H, W, Cin)), gpu_device)
+    w = bf(rng.standard_normal((9 * Cin, Cout)) * 0.2, gpu_device)
+    b = torch.from_numpy(rng.standard_normal(Cout).astype(np.float32)).to(gpu_device)
+    y = torch.empty((B, H, W, Cout), dtype=torch.bfloat16, device=gpu_device)
+    yf = torch.empty((B, H, W, Cout), dtype=torch.float32, device=gpu_device)
+    _abi.call("pty_nn_conv3x3_fwd", P(x), P(w), P(b), P(y), P(yf), B, H, W, Cin, Cout, 0, None)
+    torch.cuda.synchronize()
+    xd, wd, bd = (t.double().cpu().numpy() for t in (x, w, b))
+    ref = _conv_ref(xd, wd, bd, False)
+    assert rel(yf.cpu().numpy(), ref) < 1e-5
+    assert rel(y.float().cpu().numpy(), ref) < 8e-3
+    # gradients of L = sum(g * conv(x, w))
+    g = bf(rng.standard_normal((B, H, W, Cout)), gpu_device)
+    dx = torch.empty_like(x)
+    _abi.call("pty_nn_conv3x3_dgrad", P(g), P(w), P(dx), B, H, W, Cin, Cout, None)
+    splits = 2 if B * H * W >= 64 else 1
+    dw = torch.empty((9 * Cin, Cout), dtype=torch.float32, device=gpu_device)
+    db = torch.empty(Cout, dtype=torch.float32, device=gpu_device)
+    ws = torch.empty(splits * 9 * Cin * Cout, dtype=torch.float32, device=gpu_device)
+    _abi.call("pty_nn_conv3x3_wgrad", P(x), P(g), P(dw), P(db), P(ws), splits, B, H, W, Cin, Cout, None)
+    torch.cuda.synchronize()
+    xt = torch.from_numpy(xd).requires_grad_(True)
+    wt = torch.from_numpy(wd).requires_grad_(True)
+    bt = torch.from_numpy(bd).requires_grad_(True)
+    wc = wt.reshape(3, 3, Cin, Cout).permute(3, 2, 0, 1)
+    yt = torch.nn.functional.conv2d(xt.permute(0, 3, 1, 2), wc, bt, padding=1).permute(0, 2, 3, 1)
+    (yt * torch.from_numpy(g.double().cpu().numpy())).sum().backward()
+    assert rel(dx.float().cpu().numpy(), xt.grad.numpy()) < 8e-3
+    assert rel(dw.cpu().numpy(), wt.grad.numpy()) < 1e-5
+    assert rel(db.cpu().numpy(), bt.grad.numpy()) < 1e-5
+
+
+def test_pool_up_and_relu_bwd(gpu_device):
+    rng = np.random.default_rng(3)
+    x = bf(rng.standard_normal((2, 8, 6, 5)), gpu_device)
+    y = torch.empty((2, 4, 3, 5), dtype=torch.bfloat16, device=gpu_device)
+    _abi.call("pty_nn_pool2", P(x), P(y), 2, 8, 6, 5, 0, None)
+    xd = x.double().cpu().numpy()
+    ref = xd.reshape(2, 4, 2, 3, 2, 5).mean(axis=(2, 4))
+    torch.cuda.synchronize()
+    assert rel(y.float().cpu().numpy(), ref) < 8e-3
+    up = torch.empty((2, 16, 12, 5), dtype=torch.bfloat16, device=gpu_device)
+    _abi.call("pty_nn_up2", P(x), P(up), 2, 8, 6, 5, 0, None)
+    torch.cuda.synchronize()
+    assert np.array_equal(up.float().cpu().numpy(), np.repeat(np.repeat(xd, 2, 1), 2, 2))
+    # backward of up = 2x2 sum; of pool = 1/4 spread
+    dxu = torch.empty_like(x)
+    _abi.call("pty_nn_up2", P(up), P(dxu), 2, 8, 6, 5, 1, None)
+    torch.cuda.synchronize()
+    assert rel(dxu.float().cpu().numpy(), 4 * xd) < 8e-3
+    dy = bf(rng.standard_normal(x.shape), gpu_device)
+    yy = bf(rng.standard_normal(x.shape), gpu_device)
+    expect = dy.float().cpu().numpy() * (yy.float().cpu().numpy() > 0)
+    _abi.call("pty_nn_relu_bwd", P(dy), P(yy), dy.numel(), None)
+    torch.cuda.synchronize()
+    assert np.array_equal(dy.float().cpu().numpy(), expect)
+
+
+SMALL = [("conv", 1, 8), ("conv", 8, 8), ("pool",), ("conv", 8, 16), ("up",), ("conv", 16, 8),
+         ("conv", 8, 2)]
+
+
+def _net_ref(net, x, t, w):
+    """float64 CPU emulation of the GPU network (bf16 rounding of every stored
+    activation and of the output gradient), with autograd for the reference grads."""
+    def r(a):  # round through bf16 like the GPU stores
+        return a.to(torch.bfloat16).to(torch.float64)
+
+    class RoundBf16(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, a):
+            return r(a)
+
+        @staticmethod
+        def backward(ctx, g):
+            return r(g)
+
+    params = []
+    h = torch.from_numpy(x.float().cpu().numpy()).double().permute(0, 3, 1, 2)
+    pi = 0
+    for l in net.layers:
+        if l[0] == "conv":
+            p = net.params[pi]
+            pi += 1
+            wt = torch.from_numpy(p["wb"].float().cpu().numpy()).double().requires_grad_(True)
+            bt = torch.from_numpy(p["b"].cpu().numpy()).double().requires_grad_(True)
+            params.append((wt, bt))
+            wc = wt.reshape(3, 3, p["cin"], p["cout"]).permute(3, 2, 0, 1)
+            h = torch.nn.functional.conv2d(h, wc, bt, padding=1)
+            if p["relu"]:
+                h = RoundBf16.apply(torch.relu(h))
+        elif l[0] == "pool":
+            h = RoundBf16.apply(torch.nn.functional.avg_pool2d(h, 2))
+        else:
+            h = RoundBf16.apply(torch.nn.functional.interpolate(h, scale_factor=2, mode="nearest"))
+    out = h.permute(0, 2, 3, 1)
+    tt = torch.from_numpy(t.cpu().numpy()).double()
+    wt_ = torch.from_numpy(np.repeat(w.ravel(), 2).reshape(out.shape[1], out.shape[2], 2)).double()
+    loss = (wt_ * (out - tt) ** 2).mean()
+    loss.backward()
+    return out.detach().numpy(), float(loss), [(a.grad.numpy(), b.grad.numpy()) for a, b in params]
+
+
+def test_network_gradients_match_float64_reference(gpu_device):
+    rng = np.random.default_rng(11)
+    net = InitNet(frame=16, seed=4, device=gpu_device, layers=SMALL)
+    x = bf(np.abs(rng.standard_normal((4, 16, 16, 1))), gpu_device)
+    t = torch.from_numpy(rng.standard_normal((4, 16, 16, 2)).astype(np.float32)).to(gpu_device)
+    w = (rng.random((16, 16)) + 0.5).astype(np.float32)
+    wt = torch.from_numpy(np.repeat(w.ravel(), 2)).to(gpu_device)
+    pred = net.forward(x, keep=True)
+    loss, dy = net.loss(pred, t, wt, grad=True, gscale=1.0)
+    net.backward(dy)
+    torch.cuda.synchronize()
+    out_ref, loss_ref, grads_ref = _net_ref(net, x, t, w)
+    assert rel(pred.cpu().numpy(), out_ref) < 1e-2
+    assert abs(loss - loss_ref) < 2e-2 * loss_ref
+    for p, (gw, gb) in zip(net.params, grads_ref):
+        assert rel(p["gw"].cpu().numpy(), gw) < 5e-2
+        assert rel(p["gb"].cpu().numpy(), gb) < 5e-2
+
+
+def test_training_reduces_loss_and_gate(gpu_device):
+    torch.manual_seed(0)
+    x, t, Pr, w = training_frames(2048, seed=5, device=gpu_device, obj_size=192, per_object=512)
+    net = InitNet(seed=1, device=gpu_device)
+    hist = train(net, x, t, w, steps=150, batch=32, lr=2e-3)
+    first, last = np.mean(hist[:10]), np.mean(hist[-10:])
+    print(f"training loss {first:.4e} -> {last:.4e}")
+    assert np.all(np.isfinite(hist)) and last < 0.7 * first
+    # the gate keeps whichever start has the lower first-iteration eMod
+    from ptychography_amd import sim
+    s = sim.make_scan("c5", grid=(6, 6), noise=False)
+    U, V = s.O.shape
+    O0, info = initial_object(net, s.I, s.R, s.P / np.sqrt(np.abs(s.P).max() ** 2) * np.abs(s.P).max(),
+                              U, V, photons=sim.RECIPES["c5"].photons, device=gpu_device)
+    assert info["used_learned"] == (info["eMod_learned_start"] < info["eMod_flat_start"])
+    assert O0.shape == (U, V)
