@@ -170,17 +170,29 @@ const char* pty_nn_last_error(void);
 /* C (M x N, fp32) = A (M x K) B (K x N), bf16 row-major: MFMA lane-layout probe. */
 int pty_nn_gemm_probe(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
                       void* stream);
-/* y = act(conv3x3(x, w) + bias), zero padding; y (bf16) and/or yf (fp32). */
-int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y,
-                       float* yf, int B, int H, int W, int Cin, int Cout, int relu, void* stream);
-/* dx = conv3x3^T(dy, w) */
-int pty_nn_conv3x3_dgrad(const uint16_t* dy, const uint16_t* w, uint16_t* dx, int B, int H, int W,
-                         int Cin, int Cout, void* stream);
-/* dw (fp32 [9*Cin][Cout]) and db (fp32 [Cout], optional); ws: splits*9*Cin*Cout floats. */
+/* y = act(conv3x3(x, w) + bias), zero padding; y (bf16) and/or yf (fp32).
+ * wT: the per-tap transpose [9][Cout][Cin] of w (pty_nn_weight_t), used by the
+ * direct-convolution kernel (W in {8,16,32,64,128}, Cin and Cout multiples of
+ * 32); NULL selects the generic implicit GEMM. */
+int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const uint16_t* wT,
+                       const float* bias, uint16_t* y, float* yf, int B, int H, int W, int Cin,
+                       int Cout, int relu, void* stream);
+/* dx = conv3x3^T(dy, w); mask (optional, shape of dx): dx = 0 where mask <= 0
+ * (the ReLU backward of the layer that produced this conv's input). */
+int pty_nn_conv3x3_dgrad(const uint16_t* dy, const uint16_t* w, uint16_t* dx,
+                         const uint16_t* mask, int B, int H, int W, int Cin, int Cout,
+                         void* stream);
+/* wT[t][co][ci] = w[t][ci][co] */
+int pty_nn_weight_t(const uint16_t* w, uint16_t* wT, int Cin, int Cout, void* stream);
+/* dw (fp32 [9*Cin][Cout]) and db (fp32 [Cout], optional) summed over `splits`
+ * fixed partial buffers; ws: splits * (9*Cin*Cout + Cout) floats. */
 int pty_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, float* ws,
                          int splits, int B, int H, int W, int Cin, int Cout, void* stream);
+/* split count that fills the device for pty_nn_conv3x3_wgrad (>= 1). */
+int pty_nn_wgrad_splits(int B, int H, int W, int Cin, int Cout);
 int pty_nn_relu_bwd(uint16_t* dy, const uint16_t* y, size_t n, void* stream);
-/* 2x2 average pool (backward = 1/4 spread), nearest 2x upsample (backward = sum). */
+/* 2x2 average pool (backward = 1/4 spread), nearest 2x upsample (backward = sum).
+ * These and relu_bwd need 16-byte aligned tensors; pool2 / up2 need C % 8 == 0. */
 int pty_nn_pool2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, int backward,
                  void* stream);
 int pty_nn_up2(const uint16_t* x, uint16_t* y, int B, int H, int W, int C, int backward,
@@ -192,13 +204,27 @@ int pty_nn_mse(const float* p, const float* t, size_t n, uint16_t* dy, float gsc
 int pty_nn_adam(float* w, uint16_t* wb, const float* g, float* m, float* v, size_t n, float lr,
                 float b1, float b2, float eps, int step, float gscale, void* stream);
 int pty_nn_to_bf16(const float* x, uint16_t* y, size_t n, void* stream);
-/* O = weighted, phase-aligned overlap average of N complex M x K patches at
- * R's windows (patch pixel weights pw, or 1); object pixels whose summed
- * weight is <= floor get `fill`.  acc / wsum: U*V workspace. */
-int pty_nn_stitch(const pty_cfloat* patches, const float* pw, const int32_t* R_host, int N, int M,
-                  int K, int U, int V, float floor, float fill, pty_cfloat* acc, float* wsum,
-                  pty_cfloat* O, void* stream);
-
+/* Training batch: xb[b] = T_b(x[idx[b]]) (bf16, M x M) and tb[b] = T_b(t[idx[b]])
+ * (complex as 2 floats), T_b one of the 8 maps from i -> (-i) mod M on rows
+ * and columns and a transpose (exact for a probe symmetric under them),
+ * code hash(seed, b) & 7 when aug != 0, identity otherwise. */
+int pty_nn_batch_gather(const uint16_t* x, const float* t, const int64_t* idx, int B, int M,
+                        int aug, uint32_t seed, uint16_t* xb, float* tb, void* stream);
+/* Phase synchronisation of predicted patches (each known up to a constant phase).
+ * pair_overlap: c[q] (double complex) = sum over the overlap of pair q = (i, j)
+ *   of conj(w p_i) (w p_j); off: N (row, col) window offsets >= 0, pairs: npairs
+ *   (i, j).  phase_sync: projected power iterations z_i <- s_i / |s_i|,
+ *   s_i = sum_j C_ij z_j from z = 1, CSR rowptr (N+1) / entries (j, qs) with
+ *   C_ij = c[qs] (qs >= 0) or conj(c[-qs-1]), nent = rowptr[N] entries,
+ *   N <= 8192; work: 3 * nent floats (expanded values and columns).
+ * phase_exits: out[n] = P * patch_n * z_n, whose overlap projection
+ *   (pty_psup_O) is the |P|^2-weighted average of the aligned patches. */
+int pty_nn_pair_overlap(const pty_cfloat* patches, const float* w, const int32_t* off,
+                        const int32_t* pairs, int npairs, int M, int K, double* c, void* stream);
+int pty_nn_phase_sync(const int32_t* rowptr, const int32_t* entries, const double* c, int N,
+                      int iters, pty_cfloat* z, pty_cfloat* work, int32_t nent, void* stream);
+int pty_nn_phase_exits(const pty_cfloat* patches, const pty_cfloat* z, const pty_cfloat* P, int N,
+                       int M, int K, pty_cfloat* out, void* stream);
 #ifdef __cplusplus
 }
 #endif

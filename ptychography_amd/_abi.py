@@ -59,11 +59,13 @@ SIGNATURES = {
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_nn_last_error": (ctypes.c_char_p, []),
     "pty_nn_gemm_probe": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp]),
-    "pty_nn_conv3x3_fwd": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32,
-                                          _i32, _vp]),
-    "pty_nn_conv3x3_dgrad": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "pty_nn_conv3x3_fwd": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32,
+                                          _i32, _i32, _vp]),
+    "pty_nn_conv3x3_dgrad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "pty_nn_weight_t": (ctypes.c_int, [_vp, _vp, _i32, _i32, _vp]),
     "pty_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32,
                                             _i32, _vp]),
+    "pty_nn_wgrad_splits": (ctypes.c_int, [_i32, _i32, _i32, _i32, _i32]),
     "pty_nn_relu_bwd": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "pty_nn_pool2": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
     "pty_nn_up2": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
@@ -72,8 +74,11 @@ SIGNATURES = {
     "pty_nn_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _f32, _f32, _f32, _f32,
                                    _i32, _f32, _vp]),
     "pty_nn_to_bf16": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
-    "pty_nn_stitch": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _f32, _f32, _vp,
-                                     _vp, _vp, _vp]),
+    "pty_nn_batch_gather": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, ctypes.c_uint32, _vp,
+                                           _vp, _vp]),
+    "pty_nn_pair_overlap": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "pty_nn_phase_sync": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp]),
+    "pty_nn_phase_exits": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
 }
 
 SUPPORTED_FRAMES = (16, 32, 64, 96, 128, 192, 256)

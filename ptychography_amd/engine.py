@@ -57,6 +57,11 @@ class Engine:
         if FUSED is not None:
             self.set_path(FUSED)
 
+    @property
+    def R(self):
+        """The (N, 2) int32 window offsets (R <= 0) the engine was built with."""
+        return self._R
+
     # -- lifetime -----------------------------------------------------------
     def close(self):
         if self._ctx:

@@ -69,14 +69,25 @@ def _random(rng, shape):
 
 def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
           Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
-          full_output=True, device=None, callback=None, rng=None):
-    """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns."""
+          full_output=True, device=None, callback=None, rng=None, init=None, init_net=None, graphs=False):
+    """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns.
+
+    ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
+    the starting object by the network's stitched estimate when its
+    first-iteration eMod beats that of the ``O`` passed in; ``info['init']``
+    records both eMods, the choice and the time spent."""
     dev = require_gpu(device)
     if algorithm not in (ALGO_ERA, ALGO_DM):
         raise ValueError("algorithm must be 0 (ERA) or 1 (DM)")
     if iters < 0:
         raise ValueError("iters must be >= 0")
     method, update = resolve_method(O, P, background, method)
+    if init not in (None, 'flat', 'learned'):
+        raise ValueError("init must be None, 'flat' or 'learned'")
+    if init == 'learned' and init_net is None:
+        raise ValueError("init='learned' needs init_net (a trained learned.InitNet)")
+    if init == 'learned' and update == 'P':
+        raise ValueError("init='learned' seeds O, but this method updates only P")
     if isinstance(OP_iters, (int, np.integer)):
         OP_iters = (int(OP_iters), 1)
     OP_iters = (int(OP_iters[0]), int(OP_iters[1]))
@@ -125,11 +136,16 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
             mask_s = torch.from_numpy(np.ascontiguousarray(np.fft.ifftshift(mc))).to(dev)
         I_t = I.to(dev) if I_is_tensor else torch.from_numpy(np.ascontiguousarray(I, dtype=np.float32))
         amp, I_norm = eng.prepare_amp(I_t, mask_c)
-        del I_t
         eng.set_data(amp, mask_s, I_norm)
 
         O_t = torch.from_numpy(np.ascontiguousarray(O.astype(np.complex64))).to(dev)
         P_t = torch.from_numpy(np.ascontiguousarray(P.astype(np.complex64))).to(dev)
+        init_info = None
+        if init == 'learned':
+            from .learned import learned_start
+            O_t, init_info = learned_start(init_net, eng, I_t.to(dev, torch.float32).contiguous(),
+                                           P_t, O_t, alpha=alpha)
+        del I_t
         eng.set_state(O_t, P_t)
 
         bg_init = None
@@ -151,6 +167,8 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
         if until > 0:
             eng.set_pamp(None)   # |FFT(P0)| (era.py:546-547)
 
+        if graphs:
+            eng.set_graphs(True)
         kw = dict(OP_iters=OP_iters, Pmod_probe_until=until, probe_centering=probe_centering,
                   alpha=alpha)
         if callback is None:
@@ -170,6 +188,8 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
             return {'O': O_out, 'P': P_out}[update] if update != 'OP' else (O_out, P_out)
 
         info = {}
+        if init_info is not None:
+            info['init'] = init_info
         info['I'] = eng.intensity().cpu().numpy().astype(r_dtype)
         info['eMod'] = [float(x) for x in metrics[:, 0]]
         info['eCon'] = [float(x) for x in metrics[:, 1]]
@@ -197,15 +217,19 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
 
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmod_probe=False,
         probe_centering=False, hardware='cpu', alpha=1.0e-10, dtype=None, full_output=True,
-        device=None, callback=None, rng=None):
+        device=None, callback=None, rng=None, init=None, init_net=None, graphs=False):
     """Error-reduction algorithm (ptychography/era.py:13).  ``hardware`` is accepted and
     ignored, as in the reference; the work always runs on the GPU ``device``.
 
     Build-only keywords: ``device`` (torch device, default the current GPU),
     ``callback(i, O, P)`` after every iteration, ``rng`` (object with
-    ``.random(shape)``) in place of ``np.random`` for the default P / background.
+    ``.random(shape)``) in place of ``np.random`` for the default P / background,
+    ``init='learned'`` with ``init_net`` for the gated learned starting object,
+    ``graphs=True`` to replay each kind of iteration as a captured hipGraph
+    (bit-identical to launching the kernels one by one).
     """
     return solve(ALGO_ERA, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
                  background=background, method=method, Pmod_probe=Pmod_probe,
                  probe_centering=probe_centering, alpha=alpha, dtype=dtype,
-                 full_output=full_output, device=device, callback=callback, rng=rng)
+                 full_output=full_output, device=device, callback=callback, rng=rng,
+                 init=init, init_net=init_net, graphs=graphs)

@@ -62,6 +62,7 @@ class InitNet:
             d = dict(cin=cin, cout=cout, relu=(k < n_conv))
             d["w32"] = torch.from_numpy(w).to(self.dev)
             d["wb"] = d["w32"].to(torch.bfloat16)
+            d["wT"] = torch.empty_like(d["wb"])
             d["b"] = torch.zeros(cout, dtype=torch.float32, device=self.dev)
             for name, shape in (("gw", (9 * cin, cout)), ("gb", (cout,))):
                 d[name] = torch.zeros(shape, dtype=torch.float32, device=self.dev)
@@ -73,6 +74,7 @@ class InitNet:
         self.step_count = 0
         self._part = torch.zeros(1024, dtype=torch.float64, device=self.dev)
         self._ws = None
+        self._refresh_transposed()
 
     # -- helpers --------------------------------------------------------------
     def _stream(self):
@@ -83,6 +85,12 @@ class InitNet:
             self._ws = torch.empty(n, dtype=torch.float32, device=self.dev)
         return self._ws
 
+    def _refresh_transposed(self):
+        """Per-tap transposed bf16 weights used by the forward direct convolution."""
+        s = self._stream()
+        for p in self.params:
+            _abi.call("pty_nn_weight_t", _p(p["wb"]), _p(p["wT"]), p["cin"], p["cout"], s)
+
     def state_dict(self):
         return {f"w{i}": p["w32"].cpu() for i, p in enumerate(self.params)} | \
                {f"b{i}": p["b"].cpu() for i, p in enumerate(self.params)}
@@ -91,7 +99,8 @@ class InitNet:
         for i, p in enumerate(self.params):
             p["w32"].copy_(sd[f"w{i}"].to(self.dev))
             p["b"].copy_(sd[f"b{i}"].to(self.dev))
-            p["wb"] = p["w32"].to(torch.bfloat16)
+            p["wb"].copy_(p["w32"].to(torch.bfloat16))
+        self._refresh_transposed()
 
     # -- forward / backward ---------------------------------------------------
     def forward(self, x, keep=False):
@@ -111,7 +120,8 @@ class InitNet:
                                                   device=self.dev)
                 yf = torch.empty((B, H, W, p["cout"]), dtype=torch.float32, device=self.dev) \
                     if last else None
-                _abi.call("pty_nn_conv3x3_fwd", _p(cur), _p(p["wb"]), _p(p["b"]), _p(y), _p(yf),
+                _abi.call("pty_nn_conv3x3_fwd", _p(cur), _p(p["wb"]), _p(p["wT"]), _p(p["b"]),
+                          _p(y), _p(yf),
                           B, H, W, p["cin"], p["cout"], int(p["relu"]), s)
                 if last:
                     out_f = yf
@@ -134,12 +144,16 @@ class InitNet:
         return out_f
 
     def backward(self, dy):
-        """dy: (B, H, W, 2) bf16 gradient of the loss w.r.t. the output; fills gw / gb."""
+        """dy: (B, H, W, 2) bf16 gradient of the loss w.r.t. the output; fills gw / gb.
+
+        The ReLU backward of a conv layer is fused into the dgrad of the conv
+        that consumes its output when the two are adjacent."""
         s = self._stream()
         acts = self._acts
         B = dy.shape[0]
         pi = len(self.params)
         g = dy
+        masked = False   # g already carries the ReLU mask of the layer being processed
         for li in range(len(self.layers) - 1, -1, -1):
             l = self.layers[li]
             x = acts[li]
@@ -147,37 +161,43 @@ class InitNet:
             if l[0] == "conv":
                 pi -= 1
                 p = self.params[pi]
-                if p["relu"]:
+                if p["relu"] and not masked:
                     _abi.call("pty_nn_relu_bwd", _p(g), _p(acts[li + 1]), g.numel(), s)
-                P = B * H * W
-                splits = int(max(1, min(32, P // 8192)))
-                ws = self._ws_for(splits * 9 * p["cin"] * p["cout"])
+                splits = _abi.load().pty_nn_wgrad_splits(B, H, W, p["cin"], p["cout"])
+                ws = self._ws_for(splits * (9 * p["cin"] + 1) * p["cout"])
                 _abi.call("pty_nn_conv3x3_wgrad", _p(x), _p(g), _p(p["gw"]), _p(p["gb"]), _p(ws),
                           splits, B, H, W, p["cin"], p["cout"], s)
+                masked = False
                 if li > 0:
+                    prev_relu = self.layers[li - 1][0] == "conv" and self.params[pi - 1]["relu"]
                     dx = torch.empty_like(x)
-                    _abi.call("pty_nn_conv3x3_dgrad", _p(g), _p(p["wb"]), _p(dx), B, H, W,
-                              p["cin"], p["cout"], s)
+                    _abi.call("pty_nn_conv3x3_dgrad", _p(g), _p(p["wb"]), _p(dx),
+                              _p(x) if prev_relu else None, B, H, W, p["cin"], p["cout"], s)
                     g = dx
+                    masked = prev_relu
             elif l[0] == "pool":
                 dx = torch.empty_like(x)
                 _abi.call("pty_nn_pool2", _p(g), _p(dx), B, H, W, x.shape[3], 1, s)
                 g = dx
+                masked = False
             elif l[0] == "up":
                 dx = torch.empty_like(x)
                 _abi.call("pty_nn_up2", _p(g), _p(dx), B, H, W, x.shape[3], 1, s)
                 g = dx
+                masked = False
         self._acts = None
 
-    def loss(self, pred, target, weight=None, grad=True, gscale=1.0):
-        """Weighted MSE (mean over all outputs); returns (loss, dy bf16 or None)."""
+    def loss(self, pred, target, weight=None, grad=True, gscale=1.0, sync=True):
+        """Weighted MSE (mean over all outputs); returns (loss, dy bf16 or None).
+        ``sync=False`` returns the loss as a 0-d device tensor (no host wait)."""
         s = self._stream()
         n = pred.numel()
         dy = torch.empty(pred.shape, dtype=torch.bfloat16, device=self.dev) if grad else None
         period = weight.numel() if weight is not None else 1
         _abi.call("pty_nn_mse", _p(pred), _p(target), n, _p(dy), float(gscale), _p(self._part),
                   _p(weight), period, s)
-        return float(self._part.sum().item()) / n, dy
+        total = self._part.sum() / n
+        return (float(total.item()) if sync else total), dy
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, gscale=1.0):
         s = self._stream()
@@ -187,6 +207,7 @@ class InitNet:
                       p["w32"].numel(), lr, b1, b2, eps, self.step_count, float(gscale), s)
             _abi.call("pty_nn_adam", _p(p["b"]), None, _p(p["gb"]), _p(p["mb"]), _p(p["vb"]),
                       p["b"].numel(), lr, b1, b2, eps, self.step_count, float(gscale), s)
+        self._refresh_transposed()
 
 
 # ---------------------------------------------------------------------------
@@ -247,34 +268,73 @@ def training_frames(n_frames, seed=5, frame=64, pupil_radius=16, defocus=0.02, p
     return torch.cat(xs), torch.cat(ts), P, w
 
 
-def train(net, x, t, w, steps=2000, batch=64, lr=2e-3, seed=0, log_every=0, time_budget=None):
-    """Adam on mini-batches drawn with a seeded permutation; returns the loss history."""
+def train(net, x, t, w, steps=2000, batch=64, lr=2e-3, seed=0, log_every=0, time_budget=None,
+          augment=True):
+    """Adam on mini-batches drawn with a seeded permutation; returns the loss history.
+
+    Learning rate: linear warm-up over 100 steps, then cosine from ``lr`` to
+    ``lr / 10`` over ``steps`` -- or over ``time_budget`` seconds when that is
+    given (training then stops at the budget or at ``steps``).  ``augment``
+    applies a random one of the 8 exact reflections / transposes to every
+    sample (the probe of ``training_frames`` is symmetric under them)."""
     g = torch.Generator(device="cpu").manual_seed(seed)
     wt = torch.from_numpy(np.repeat(w.ravel(), 2)).to(net.dev)   # weight per (pixel, channel)
-    hist = []
+    losses = []
+    torch.cuda.synchronize(net.dev)
     t0 = time.time()
     n = x.shape[0]
     perm = torch.randperm(n, generator=g)
     pos = 0
     for step in range(steps):
+        elapsed = time.time() - t0
+        if time_budget is not None and elapsed > time_budget:
+            break
+        frac = elapsed / time_budget if time_budget is not None else step / steps
         if pos + batch > n:
             perm = torch.randperm(n, generator=g)
             pos = 0
         idx = perm[pos:pos + batch].to(net.dev)
         pos += batch
-        xb = x.index_select(0, idx).contiguous()
-        tb = t.index_select(0, idx).contiguous()
-        lr_s = lr * min(1.0, (step + 1) / 100) * (0.5 * (1 + np.cos(np.pi * step / steps)) * 0.9 + 0.1)
+        xb = torch.empty((batch,) + tuple(x.shape[1:]), dtype=x.dtype, device=net.dev)
+        tb = torch.empty((batch,) + tuple(t.shape[1:]), dtype=t.dtype, device=net.dev)
+        _abi.call("pty_nn_batch_gather", _p(x), _p(t), _p(idx), batch, x.shape[1], int(augment),
+                  (seed * 1000003 + step) & 0xffffffff, _p(xb), _p(tb), net._stream())
+        lr_s = lr * min(1.0, (step + 1) / 100) * (0.5 * (1 + np.cos(np.pi * min(frac, 1.0))) * 0.9 + 0.1)
         pred = net.forward(xb, keep=True)
-        loss, dy = net.loss(pred, tb, wt, grad=True, gscale=1024.0)
+        loss, dy = net.loss(pred, tb, wt, grad=True, gscale=1024.0, sync=False)
         net.backward(dy)
         net.adam(lr=lr_s, gscale=1024.0)
-        hist.append(loss)
+        losses.append(loss)
         if log_every and step % log_every == 0:
-            print(f"step {step} loss {loss:.4e} ({time.time() - t0:.1f}s)", flush=True)
-        if time_budget is not None and time.time() - t0 > time_budget:
-            break
-    return hist
+            print(f"step {step} loss {float(loss.item()):.4e} lr {lr_s:.2e} ({time.time() - t0:.1f}s)",
+                  flush=True)
+    return [float(v) for v in torch.stack(losses).cpu()] if losses else []
+
+
+def save_weights(net, path):
+    """Weights as bf16 bit patterns (what the forward pass uses) and fp32 biases."""
+    out = {"frame": np.int32(net.frame), "layers": np.array([",".join(map(str, l)) for l in net.layers])}
+    for i, p in enumerate(net.params):
+        out[f"w{i}"] = p["wb"].view(torch.int16).cpu().numpy().view(np.uint16)
+        out[f"b{i}"] = p["b"].cpu().numpy()
+    np.savez_compressed(path, **out)
+
+
+def load_weights(path, device=None):
+    """An InitNet with the weights written by ``save_weights``."""
+    d = np.load(path, allow_pickle=False)
+    layers = []
+    for l in d["layers"]:
+        f = str(l).split(",")
+        layers.append((f[0],) + tuple(int(v) for v in f[1:]))
+    net = InitNet(frame=int(d["frame"]), device=device, layers=layers)
+    for i, p in enumerate(net.params):
+        wb = torch.from_numpy(d[f"w{i}"].view(np.int16)).view(torch.bfloat16).to(net.dev)
+        p["wb"].copy_(wb)
+        p["w32"].copy_(wb.float())
+        p["b"].copy_(torch.from_numpy(d[f"b{i}"]).to(net.dev))
+    net._refresh_transposed()
+    return net
 
 
 def predict_patches(net, I, photons, batch=256):
@@ -289,62 +349,127 @@ def predict_patches(net, I, photons, batch=256):
     return torch.cat(out).contiguous()
 
 
-def stitch(patches, R, U, V, weight=None, floor=1e-3, fill=1.0):
-    """Weighted, phase-aligned overlap average of patches at R's windows (R <= 0)."""
+def overlap_pairs(off, M, K, frac=0.75):
+    """Pairs (i < j) whose windows are closer than ``frac`` of the frame in both
+    axes, as an (npairs, 2) int32 array in (i, j) lexicographic order.
+    Sorted by row, the candidates of each frame form one contiguous range."""
+    off = np.asarray(off, np.int64)
+    N = len(off)
+    ty, tx = int(frac * M), int(frac * K)
+    order = np.argsort(off[:, 0], kind="stable")
+    ys = off[order, 0]
+    lo = np.arange(1, N + 1)                             # later in row order only
+    hi = np.searchsorted(ys, ys + ty, side="left")      # first y_j >= y_i + ty
+    counts = np.maximum(hi - lo, 0)
+    a = np.repeat(np.arange(N), counts)
+    b = np.arange(counts.sum()) - np.repeat(np.cumsum(counts) - counts, counts) + np.repeat(lo, counts)
+    keep = np.abs(off[order[b], 1] - off[order[a], 1]) < tx
+    i, j = order[a[keep]], order[b[keep]]
+    pairs = np.stack([np.minimum(i, j), np.maximum(i, j)], 1)
+    pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    return pairs.astype(np.int32)
+
+
+def pair_csr(pairs, N):
+    """CSR rows of the Hermitian pair matrix: row i lists (j, qs), qs = q for
+    pair q = (i, j) and -q-1 (conjugate) for pair q = (j, i); columns ascending."""
+    q = np.arange(len(pairs), dtype=np.int64)
+    rows = np.concatenate([pairs[:, 0], pairs[:, 1]]).astype(np.int64)
+    cols = np.concatenate([pairs[:, 1], pairs[:, 0]]).astype(np.int64)
+    qs = np.concatenate([q, -q - 1])
+    order = np.lexsort((cols, rows))
+    rowptr = np.searchsorted(rows[order], np.arange(N + 1)).astype(np.int32)
+    ent = np.stack([cols[order], qs[order]], 1).astype(np.int32)
+    return rowptr, ent
+
+
+def sync_phases(patches, off, weight=None, iters=300, frac=0.75):
+    """Global phases z (N complex64, |z| = 1) that align the patches: projected
+    power iterations on the pairwise overlap products (csrc/nn.hip)."""
     dev = patches.device
     N, M, K = patches.shape
-    acc = torch.empty((U, V), dtype=torch.complex64, device=dev)
-    wsum = torch.empty((U, V), dtype=torch.float32, device=dev)
-    O = torch.empty((U, V), dtype=torch.complex64, device=dev)
-    wt = torch.from_numpy(np.ascontiguousarray(weight, np.float32)).to(dev) if weight is not None else None
-    Rc = np.ascontiguousarray(R, np.int32)
-    torch.cuda.current_stream(dev).synchronize()
-    _abi.call("pty_nn_stitch", _p(patches), _p(wt), Rc.ctypes.data_as(_vp), N, M, K, U, V,
-              float(floor), float(fill), _p(acc), _p(wsum), _p(O),
-              _vp(torch.cuda.current_stream(dev).cuda_stream))
-    return O
+    s = _vp(torch.cuda.current_stream(dev).cuda_stream)
+    pairs = overlap_pairs(off, M, K, frac)
+    rowptr, ent = pair_csr(pairs, N)
+    c = torch.zeros((max(len(pairs), 1), 2), dtype=torch.float64, device=dev)
+    off_t = torch.from_numpy(np.ascontiguousarray(off, np.int32)).to(dev)
+    pairs_t = torch.from_numpy(np.ascontiguousarray(pairs)).to(dev)
+    w_t = torch.from_numpy(np.ascontiguousarray(weight, np.float32)).to(dev) if weight is not None else None
+    _abi.call("pty_nn_pair_overlap", _p(patches), _p(w_t), _p(off_t), _p(pairs_t), len(pairs), M, K,
+              _p(c), s)
+    rowptr_t = torch.from_numpy(rowptr).to(dev)
+    ent_t = torch.from_numpy(ent).to(dev) if len(ent) else torch.zeros((1, 2), dtype=torch.int32, device=dev)
+    z = torch.empty(N, dtype=torch.complex64, device=dev)
+    work = torch.empty(max(3 * len(ent), 1), dtype=torch.float32, device=dev)
+    _abi.call("pty_nn_phase_sync", _p(rowptr_t), _p(ent_t), _p(c), N, int(iters), _p(z), _p(work),
+              len(ent), s)
+    return z
 
 
-def initial_object(net, I, R, P, U, V, photons, mask=None, alpha=1e-10, device=None):
-    """Learned O0 if its eMod beats the flat start's, else the flat object.
+def learned_start(net, eng, I_t, P_t, O_plain_t, photons=None, alpha=1e-10, sync_iters=300):
+    """Learned O0 on an engine whose data are set, gated by the first-iteration eMod.
 
-    Returns (O0 complex64 numpy, info dict with both starting eMods and the
-    choice).  The eMod of a start is the reference's first-iteration eMod
-    (ptychography/era.py:561-563, 210) of exits = O0 * P.
+    Patches are predicted from the frames, phase-aligned by ``sync_phases`` and
+    stitched by the overlap projection psup_O of the exits P * patch * z (the
+    |P|^2-weighted average).  The starting eMod of a guess is the reference's
+    first-iteration eMod (ptychography/era.py:561-563, 210) of exits O0 * P;
+    the learned O0 is kept only if it beats ``O_plain_t``'s.  Returns (O0
+    tensor, info).  Leaves the engine's state to be set by the caller.
     """
     from .era import ALGO_ERA
-    dev = require_gpu(device)
-    I_t = I if isinstance(I, torch.Tensor) else torch.from_numpy(np.asarray(I, np.float32))
-    I_t = I_t.to(dev, torch.float32)
-    Rs = np.asarray(R).copy()
-    Rs -= Rs.max(axis=0)
+    dev = eng.device
+    N, M, K = eng.N, eng.M, eng.K
+    if photons is None:
+        photons = float(I_t.sum(dtype=torch.float64).item()) / N
+    torch.cuda.synchronize(dev)
     t0 = time.time()
     patches = predict_patches(net, I_t, photons)
-    O_l = stitch(patches, Rs, U, V, illumination_weight(P))
-    torch.cuda.synchronize()
-    t_init = time.time() - t0
+    if (M, K) != (net.frame, net.frame):
+        raise ValueError(f"the network takes {net.frame}x{net.frame} frames, the scan has {M}x{K}")
+    off = -np.asarray(eng.R, np.int64)
+    w = illumination_weight(P_t.cpu().numpy())
+    z = sync_phases(patches, off, w, iters=sync_iters)
+    exits = torch.empty((N, M, K), dtype=torch.complex64, device=dev)
+    _abi.call("pty_nn_phase_exits", _p(patches), _p(z), _p(P_t), N, M, K, _p(exits),
+              _vp(torch.cuda.current_stream(dev).cuda_stream))
+    O_l, _ = eng.psup_O(exits, P_t, alpha=alpha)
+    del exits, patches
 
     def first_emod(O_start):
-        N, M, K = I_t.shape
-        eng = Engine(N, M, K, U, V, Rs, device=dev)
-        mc = None
-        ms = None
+        eng.set_state(O_start.clone(), P_t.clone())
+        return float(eng.run(ALGO_ERA, 1, 1, begin=True, alpha=alpha)[0, 0])
+
+    e_learned = first_emod(O_l)
+    e_plain = first_emod(O_plain_t)
+    use = bool(e_learned < e_plain)
+    torch.cuda.synchronize(dev)
+    info = dict(eMod_learned_start=e_learned, eMod_plain_start=e_plain, used_learned=use,
+                init_seconds=time.time() - t0)
+    return (O_l if use else O_plain_t), info
+
+
+def initial_object(net, I, R, P, U, V, photons=None, mask=None, alpha=1e-10, device=None,
+                   O_plain=None, sync_iters=300):
+    """Stand-alone learned O0 (complex64 numpy) and the gate's info; the plain
+    start is ``O_plain`` (default: a flat object of ones)."""
+    dev = require_gpu(device)
+    I_t = I if isinstance(I, torch.Tensor) else torch.from_numpy(np.asarray(I, np.float32))
+    I_t = I_t.to(dev, torch.float32).contiguous()
+    Rs = np.asarray(R).copy()
+    Rs -= Rs.max(axis=0)
+    N, M, K = I_t.shape
+    eng = Engine(N, M, K, U, V, Rs, device=dev)
+    try:
+        mc = ms = None
         if mask is not None and not (np.isscalar(mask) and mask == 1):
             mc = torch.from_numpy(np.asarray(mask, np.uint8)).to(dev)
             ms = torch.from_numpy(np.fft.ifftshift(np.asarray(mask, np.uint8))).to(dev)
         amp, I_norm = eng.prepare_amp(I_t, mc)
         eng.set_data(amp, ms, I_norm)
-        O_t = O_start.clone()
         P_t = torch.from_numpy(np.asarray(P, np.complex64)).to(dev)
-        eng.set_state(O_t, P_t)
-        m = eng.run(ALGO_ERA, 1, 1, begin=True, alpha=alpha)
+        plain = torch.ones((U, V), dtype=torch.complex64, device=dev) if O_plain is None else \
+            torch.from_numpy(np.asarray(O_plain, np.complex64)).to(dev)
+        O0, info = learned_start(net, eng, I_t, P_t, plain, photons, alpha, sync_iters)
+        return O0.cpu().numpy(), info
+    finally:
         eng.close()
-        return float(m[0, 0])
-
-    flat = torch.ones((U, V), dtype=torch.complex64, device=dev)
-    e_learned = first_emod(O_l)
-    e_flat = first_emod(flat)
-    use = e_learned < e_flat
-    info = dict(eMod_learned_start=e_learned, eMod_flat_start=e_flat, used_learned=bool(use),
-                init_seconds=t_init)
-    return (O_l if use else flat).cpu().numpy(), info

@@ -12,7 +12,9 @@ import pytest
 import torch
 
 from ptychography_amd import _abi
-from ptychography_amd.learned import InitNet, initial_object, training_frames, train
+from ptychography_amd.engine import Engine
+from ptychography_amd.learned import (InitNet, initial_object, overlap_pairs, sync_phases,
+                                      training_frames, train)
 
 pytestmark = pytest.mark.gpu
 vp = ctypes.c_void_p
@@ -56,61 +58,99 @@ def _conv_ref(x, w, b, relu):
     return y.permute(0, 2, 3, 1).numpy()
 
 
-@pytest.mark.parametrize("B,H,W,Cin,Cout", [(2, 16, 16, 1, 32), (2, 8, 8, 32, 64), (1, 16, 8, 64, 16)])
+def _wT(w, Cin, Cout, dev):
+    wT = torch.empty_like(w)
+    _abi.call("pty_nn_weight_t", P(w), P(wT), Cin, Cout, None)
+    return wT
+
+
+# shapes chosen to reach every kernel: first layer (1 input channel), few output
+# channels, direct convolution with 32- and 64-channel column tiles at W = 8
+# (tiles spanning two images, odd batch -> partial tile), 64 and 128, and the
+# generic implicit GEMM (channel counts that are not multiples of 32); the
+# weight gradients of the thin layers use the VALU kernel
+CONV_SHAPES = [(2, 16, 16, 1, 32), (1, 32, 32, 64, 2), (2, 64, 64, 32, 2), (2, 8, 8, 32, 64), (3, 8, 8, 32, 32),
+               (1, 64, 64, 32, 64), (1, 4, 128, 64, 32), (1, 16, 8, 64, 16), (2, 12, 12, 48, 40)]
+
+
+@pytest.mark.parametrize("B,H,W,Cin,Cout", CONV_SHAPES)
 def test_conv_forward_dgrad_wgrad(B, H, W, Cin, Cout, gpu_device):
-    rng = np.random.default_rng(Cin * 7 + Cout)
+    rng = np.random.default_rng(Cin * 7 + Cout + W)
     x = bf(rng.standard_normal((B, H, W, Cin)), gpu_device)
     w = bf(rng.standard_normal((9 * Cin, Cout)) * 0.2, gpu_device)
+    wT = _wT(w, Cin, Cout, gpu_device)
     b = torch.from_numpy(rng.standard_normal(Cout).astype(np.float32)).to(gpu_device)
-    y = torch.empty((B, H, W, Cout), dtype=torch.bfloat16, device=gpu_device)
-    yf = torch.empty((B, H, W, Cout), dtype=torch.float32, device=gpu_device)
-    _abi.call("pty_nn_conv3x3_fwd", P(x), P(w), P(b), P(y), P(yf), B, H, W, Cin, Cout, 0, None)
-    torch.cuda.synchronize()
     xd, wd, bd = (t.double().cpu().numpy() for t in (x, w, b))
     ref = _conv_ref(xd, wd, bd, False)
-    assert rel(yf.cpu().numpy(), ref) < 1e-5
-    assert rel(y.float().cpu().numpy(), ref) < 8e-3
+    for wt_arg in (wT, None):   # direct kernel (where the shape allows) and generic GEMM
+        y = torch.empty((B, H, W, Cout), dtype=torch.bfloat16, device=gpu_device)
+        yf = torch.empty((B, H, W, Cout), dtype=torch.float32, device=gpu_device)
+        _abi.call("pty_nn_conv3x3_fwd", P(x), P(w), P(wt_arg), P(b), P(y), P(yf), B, H, W, Cin, Cout,
+                  0, None)
+        torch.cuda.synchronize()
+        assert rel(yf.cpu().numpy(), ref) < 1e-5
+        assert rel(y.float().cpu().numpy(), ref) < 8e-3
+    yr = torch.empty((B, H, W, Cout), dtype=torch.bfloat16, device=gpu_device)
+    _abi.call("pty_nn_conv3x3_fwd", P(x), P(w), P(wT), P(b), P(yr), None, B, H, W, Cin, Cout, 1, None)
+    torch.cuda.synchronize()
+    assert rel(yr.float().cpu().numpy(), np.maximum(ref, 0)) < 8e-3
     # gradients of L = sum(g * conv(x, w))
     g = bf(rng.standard_normal((B, H, W, Cout)), gpu_device)
-    dx = torch.empty_like(x)
-    _abi.call("pty_nn_conv3x3_dgrad", P(g), P(w), P(dx), B, H, W, Cin, Cout, None)
-    splits = 2 if B * H * W >= 64 else 1
-    dw = torch.empty((9 * Cin, Cout), dtype=torch.float32, device=gpu_device)
-    db = torch.empty(Cout, dtype=torch.float32, device=gpu_device)
-    ws = torch.empty(splits * 9 * Cin * Cout, dtype=torch.float32, device=gpu_device)
-    _abi.call("pty_nn_conv3x3_wgrad", P(x), P(g), P(dw), P(db), P(ws), splits, B, H, W, Cin, Cout, None)
-    torch.cuda.synchronize()
     xt = torch.from_numpy(xd).requires_grad_(True)
     wt = torch.from_numpy(wd).requires_grad_(True)
     bt = torch.from_numpy(bd).requires_grad_(True)
     wc = wt.reshape(3, 3, Cin, Cout).permute(3, 2, 0, 1)
     yt = torch.nn.functional.conv2d(xt.permute(0, 3, 1, 2), wc, bt, padding=1).permute(0, 2, 3, 1)
     (yt * torch.from_numpy(g.double().cpu().numpy())).sum().backward()
-    assert rel(dx.float().cpu().numpy(), xt.grad.numpy()) < 8e-3
-    assert rel(dw.cpu().numpy(), wt.grad.numpy()) < 1e-5
-    assert rel(db.cpu().numpy(), bt.grad.numpy()) < 1e-5
+    if Cin > 1:
+        dx = torch.empty_like(x)
+        _abi.call("pty_nn_conv3x3_dgrad", P(g), P(w), P(dx), None, B, H, W, Cin, Cout, None)
+        # with a ReLU mask (the producer's post-activation): dx = 0 where mask <= 0
+        mask = bf(rng.standard_normal((B, H, W, Cin)), gpu_device)
+        dxm = torch.empty_like(x)
+        _abi.call("pty_nn_conv3x3_dgrad", P(g), P(w), P(dxm), P(mask), B, H, W, Cin, Cout, None)
+        torch.cuda.synchronize()
+        assert rel(dx.float().cpu().numpy(), xt.grad.numpy()) < 8e-3
+        ref_m = xt.grad.numpy() * (mask.float().cpu().numpy() > 0)
+        assert rel(dxm.float().cpu().numpy(), ref_m) < 8e-3
+    # the library's split count and a single split must both match
+    for splits in sorted({1, _abi.load().pty_nn_wgrad_splits(B, H, W, Cin, Cout)}):
+        dw = torch.empty((9 * Cin, Cout), dtype=torch.float32, device=gpu_device)
+        db = torch.empty(Cout, dtype=torch.float32, device=gpu_device)
+        ws = torch.empty(splits * (9 * Cin + 1) * Cout, dtype=torch.float32, device=gpu_device)
+        _abi.call("pty_nn_conv3x3_wgrad", P(x), P(g), P(dw), P(db), P(ws), splits, B, H, W, Cin, Cout,
+                  None)
+        torch.cuda.synchronize()
+        assert rel(dw.cpu().numpy(), wt.grad.numpy()) < 1e-5, splits
+        assert rel(db.cpu().numpy(), bt.grad.numpy()) < 1e-5, splits
 
 
 def test_pool_up_and_relu_bwd(gpu_device):
     rng = np.random.default_rng(3)
-    x = bf(rng.standard_normal((2, 8, 6, 5)), gpu_device)
-    y = torch.empty((2, 4, 3, 5), dtype=torch.bfloat16, device=gpu_device)
-    _abi.call("pty_nn_pool2", P(x), P(y), 2, 8, 6, 5, 0, None)
+    C = 16
+    x = bf(rng.standard_normal((2, 8, 6, C)), gpu_device)
+    y = torch.empty((2, 4, 3, C), dtype=torch.bfloat16, device=gpu_device)
+    _abi.call("pty_nn_pool2", P(x), P(y), 2, 8, 6, C, 0, None)
     xd = x.double().cpu().numpy()
-    ref = xd.reshape(2, 4, 2, 3, 2, 5).mean(axis=(2, 4))
+    ref = xd.reshape(2, 4, 2, 3, 2, C).mean(axis=(2, 4))
     torch.cuda.synchronize()
     assert rel(y.float().cpu().numpy(), ref) < 8e-3
-    up = torch.empty((2, 16, 12, 5), dtype=torch.bfloat16, device=gpu_device)
-    _abi.call("pty_nn_up2", P(x), P(up), 2, 8, 6, 5, 0, None)
+    up = torch.empty((2, 16, 12, C), dtype=torch.bfloat16, device=gpu_device)
+    _abi.call("pty_nn_up2", P(x), P(up), 2, 8, 6, C, 0, None)
     torch.cuda.synchronize()
     assert np.array_equal(up.float().cpu().numpy(), np.repeat(np.repeat(xd, 2, 1), 2, 2))
     # backward of up = 2x2 sum; of pool = 1/4 spread
     dxu = torch.empty_like(x)
-    _abi.call("pty_nn_up2", P(up), P(dxu), 2, 8, 6, 5, 1, None)
+    _abi.call("pty_nn_up2", P(up), P(dxu), 2, 8, 6, C, 1, None)
     torch.cuda.synchronize()
     assert rel(dxu.float().cpu().numpy(), 4 * xd) < 8e-3
-    dy = bf(rng.standard_normal(x.shape), gpu_device)
-    yy = bf(rng.standard_normal(x.shape), gpu_device)
+    dxp = torch.empty_like(x)   # pool backward: each input pixel gets 1/4 of its output's gradient
+    _abi.call("pty_nn_pool2", P(y), P(dxp), 2, 8, 6, C, 1, None)
+    torch.cuda.synchronize()
+    spread = np.repeat(np.repeat(y.double().cpu().numpy(), 2, 1), 2, 2) / 4
+    assert rel(dxp.float().cpu().numpy(), spread) < 8e-3
+    dy = bf(rng.standard_normal((3, 7, 5, 3)), gpu_device)   # 315 elements: vector body + tail
+    yy = bf(rng.standard_normal(dy.shape), gpu_device)
     expect = dy.float().cpu().numpy() * (yy.float().cpu().numpy() > 0)
     _abi.call("pty_nn_relu_bwd", P(dy), P(yy), dy.numel(), None)
     torch.cuda.synchronize()
@@ -193,7 +233,82 @@ def test_training_reduces_loss_and_gate(gpu_device):
     from ptychography_amd import sim
     s = sim.make_scan("c5", grid=(6, 6), noise=False)
     U, V = s.O.shape
-    O0, info = initial_object(net, s.I, s.R, s.P / np.sqrt(np.abs(s.P).max() ** 2) * np.abs(s.P).max(),
-                              U, V, photons=sim.RECIPES["c5"].photons, device=gpu_device)
-    assert info["used_learned"] == (info["eMod_learned_start"] < info["eMod_flat_start"])
+    O0, info = initial_object(net, s.I, s.R, s.P, U, V, device=gpu_device)
+    assert info["used_learned"] == (info["eMod_learned_start"] < info["eMod_plain_start"])
     assert O0.shape == (U, V)
+
+
+def test_pair_overlap_sync_and_stitch(gpu_device):
+    """Patches = windows of a known object times random phases: the pair
+    products match numpy, the sync recovers the phases up to one global phase
+    and the stitched object (overlap projection of P * patch * z) is the object."""
+    from ptychography_amd import sim
+    s = sim.make_scan("c5", grid=(8, 8), noise=False)
+    off = -s.R
+    N, M = len(off), s.P.shape[0]
+    U, V = s.O.shape
+    rng = np.random.default_rng(2)
+    phi = rng.uniform(-np.pi, np.pi, N)
+    true_p = np.stack([s.O[y:y + M, x:x + M] for y, x in off])
+    patches = torch.from_numpy((true_p * np.exp(-1j * phi)[:, None, None]).astype(np.complex64)).to(gpu_device)
+    w = (np.abs(s.P) ** 2 / (np.abs(s.P) ** 2).max()).astype(np.float32)
+    pairs = overlap_pairs(off, M, M)
+    c = torch.zeros((len(pairs), 2), dtype=torch.float64, device=gpu_device)
+    # keep every argument tensor alive until the kernel has run
+    w_t = torch.from_numpy(w).to(gpu_device)
+    off_t = torch.from_numpy(off.astype(np.int32)).to(gpu_device)
+    pairs_t = torch.from_numpy(pairs).to(gpu_device)
+    _abi.call("pty_nn_pair_overlap", P(patches), P(w_t), P(off_t), P(pairs_t), len(pairs), M, M,
+              P(c), None)
+    torch.cuda.synchronize()
+    pn = patches.cpu().numpy().astype(np.complex128)
+    for q in (0, len(pairs) // 2, len(pairs) - 1):
+        i, j = pairs[q]
+        dy, dx = off[j] - off[i]
+        a = (w * pn[i])[max(0, dy):M + min(0, dy), max(0, dx):M + min(0, dx)]
+        b = (w * pn[j])[max(0, -dy):M + min(0, -dy), max(0, -dx):M + min(0, -dx)]
+        ref = np.vdot(a, b)
+        got = c[q, 0].item() + 1j * c[q, 1].item()
+        assert abs(got - ref) <= 1e-6 * abs(ref) + 1e-9
+    z = sync_phases(patches, off, w, iters=300).cpu().numpy()
+    q = z * np.exp(-1j * phi)            # should be one constant phase
+    q *= np.conj(q.mean() / abs(q.mean()))
+    assert np.abs(np.angle(q)).max() < 1e-3
+    eng = Engine(N, M, M, U, V, s.R - s.R.max(0), device=gpu_device)
+    P_t = torch.from_numpy(s.P.astype(np.complex64)).to(gpu_device)
+    exits = torch.empty_like(patches)
+    zt = torch.from_numpy(z.astype(np.complex64)).to(gpu_device)
+    _abi.call("pty_nn_phase_exits", P(patches), P(zt), P(P_t), N, M, M, P(exits), None)
+    O, H = eng.psup_O(exits, P_t)
+    eng.close()
+    O = O.cpu().numpy()
+    cov = H.cpu().numpy() > 1e-3 * H.max().item()
+    g = (O[cov] * np.conj(s.O[cov])).sum() / (np.abs(s.O[cov]) ** 2).sum()
+    assert abs(abs(g) - 1) < 1e-4
+    assert rel(O[cov] / g, s.O[cov]) < 1e-4
+
+
+def test_batch_gather_and_augmentation(gpu_device):
+    from test_learned_host import _dihedral
+    rng = np.random.default_rng(5)
+    M, n, B = 16, 10, 6
+    x = bf(rng.standard_normal((n, M, M)), gpu_device)
+    t = torch.from_numpy(rng.standard_normal((n, M, M, 2)).astype(np.float32)).to(gpu_device)
+    idx = torch.tensor([3, 0, 9, 3, 7, 1], dtype=torch.int64, device=gpu_device)
+    xn, tn = x.float().cpu().numpy(), t.cpu().numpy()
+    for aug in (0, 1):
+        xb = torch.empty((B, M, M), dtype=torch.bfloat16, device=gpu_device)
+        tb = torch.empty((B, M, M, 2), dtype=torch.float32, device=gpu_device)
+        _abi.call("pty_nn_batch_gather", P(x), P(t), P(idx), B, M, aug, 1234, P(xb), P(tb), None)
+        torch.cuda.synchronize()
+        xo, to = xb.float().cpu().numpy(), tb.cpu().numpy()
+        codes = []
+        for b, k in enumerate(idx.cpu().numpy()):
+            match = [c for c in range(8) if np.array_equal(xo[b], _dihedral(xn[k], c))
+                     and np.array_equal(to[b], _dihedral(tn[k], c))]
+            assert match, (aug, b)
+            codes.append(match[0])
+        if aug == 0:
+            assert codes == [0] * B
+        else:
+            assert len(set(codes)) > 1
