@@ -10,7 +10,14 @@ the library's own frame-formation / FFT kernels plus torch.poisson.
 A step is one ERA iteration (method 1: object update with the known probe)
 over all 4096 frames, with every input resident in HBM.  The same run also
 times the difference map with background refinement (method 4) at the same
-shape, reported under "extra".
+shape, reported under "extra", and the second half of the metric: wall time
+to eMod <= 1e-3 of ERA method 3 on the held-out 32 x 32-position scan of
+64^2 noise-free frames from the learned start (committed weights, trained by
+``python -m ptychography_amd.train_init``) against the flat start, under
+"learned_init".
+
+``--copy-calibration`` only copies 1 GiB three times (the plain copy kernel
+whose FETCH_SIZE / WRITE_SIZE counters scale the kernel traffic figures).
 
 Prints ONE JSON line (rank 0, N=1).
 """
@@ -60,7 +67,61 @@ def parse():
     ap.add_argument("--path", choices=["default", "fused", "staged"], default="default")
     ap.add_argument("--frame-threads", type=int, default=0)
     ap.add_argument("--frame-slots", type=int, default=0)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-init timing")
+    ap.add_argument("--copy-calibration", action="store_true")
     return ap.parse_args()
+
+
+WEIGHTS = os.path.join(ROOT, "ptychography_amd", "weights", "init64.npz")
+
+
+def copy_calibration():
+    import torch
+    x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 1 GiB
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 1 << 30, "copies": 3}))
+
+
+def learned_init(dev, target=1e-3, max_iters=1500, reps=5):
+    """Wall time to eMod <= target: ERA method 3 from the learned start vs the
+    flat start, held-out noise-free 32 x 32 scan of 64^2 frames, I resident."""
+    import torch
+    from ptychography_amd import ERA, sim
+    from ptychography_amd.learned import load_weights
+    net = load_weights(WEIGHTS, device=dev)
+    s = sim.make_scan("c5", noise=False)
+    I_d = torch.from_numpy(s.I.astype(np.float32)).to(dev)
+    U, V = s.O.shape
+    ones = np.ones((U, V), np.complex64)
+
+    def era(iters, init, full):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ERA(I_d, s.R.copy(), s.P.copy(), ones.copy(), iters, mask=1, method=3, device=dev,
+                init=init, init_net=net if init else None, full_output=full)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    out = {"workload": "c5-heldout-1024x64x64-method3", "target_eMod": target}
+    for name, init in (("plain", None), ("learned", "learned")):
+        _, (O, P, info) = era(max_iters, init, True)
+        e = np.asarray(info["eMod"])
+        hit = np.nonzero(e <= target)[0]
+        k = int(hit[0]) + 1 if len(hit) else None
+        out[f"{name}_iters"] = k
+        out[f"{name}_start_eMod"] = float(e[0])
+        if name == "learned":
+            out["init"] = info.get("init")
+        if k is not None:
+            era(k, init, False)
+            out[f"{name}_wall_s"] = float(np.median([era(k, init, False)[0] for _ in range(reps)]))
+    if out.get("plain_wall_s") and out.get("learned_wall_s"):
+        out["speedup"] = out["plain_wall_s"] / out["learned_wall_s"]
+    return out
 
 
 def make_scan_gpu(grid, dev):
@@ -134,6 +195,9 @@ def cpu_baseline(scan, nframes):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding "
                                    "is a design note only (--gpus must be 1)"}))
@@ -198,12 +262,16 @@ def main():
               "dm_bg_roofline_frac": (DM4_BYTES_PX * px + OBJ_BYTES_PX * U * V) * dm_its / (HBM_PEAK_GBS * 1e9),
               "dm_bg_final_eMod": float(dm_metrics[-1, 0])}
 
+    learned = None
+    if not args.no_learned and os.path.exists(WEIGHTS):
+        learned = learned_init(dev)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scan, args.cpu_frames)
 
     out = {
-        "metric": "ERA/DM iters/s (4096x256^2 frames)",
+        "metric": "ERA/DM iters/s (4096x256^2 frames); time-to-eMod 1e-3, learned init vs plain",
         "value": its,
         "unit": "iters/s",
         "n_gpus": 1,
@@ -228,6 +296,7 @@ def main():
         "gen_s": t_gen,
         "cpu_baseline": cpu,
         "extra": dm,
+        "learned_init": learned,
     }
     print(json.dumps(out))
 

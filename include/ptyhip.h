@@ -119,6 +119,39 @@ int pty_set_path(pty_ctx* ctx, int32_t fused, int32_t threads, int32_t slots);
  * recorded while graphs are on. */
 int pty_set_graphs(pty_ctx* ctx, int32_t enable);
 
+/* ---- launch shapes and the autotuner (no counterpart in the reference:
+ * its loop is NumPy on the CPU) ---- */
+/* Persistent staged kernels use up to rows_mult * CUs (1..4, default 3)
+ * work-groups for the row FFTs and cols_mult * CUs (1..4; 0 = the default per
+ * column mode) for the column FFTs. */
+int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult);
+typedef struct pty_tune_result {
+  int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult;  /* chosen */
+  int32_t candidates;                                               /* configurations timed */
+  float best_ms, default_ms;                                        /* per iteration */
+} pty_tune_result;
+/* Times `trial_iters` iterations of opts for every launch configuration
+ * (staged: rows_mult {2,3,4} x cols_mult {default,2,4}; fused per-frame path
+ * where the frame shape allows it: 512 / 1024 threads x CUs / 2*CUs slots),
+ * keeps the fastest and restores O, P and the background; the next pty_run
+ * must pass begin = 1. */
+int pty_tune(pty_ctx* ctx, const pty_opts* opts, int32_t trial_iters, pty_tune_result* out);
+
+/* ---- several scans at once (no counterpart in the reference) ----
+ * The contexts (prepared with pty_set_data / pty_set_state / ...) stay owned
+ * by the caller and must outlive the batch.  Each scan's kernels run on one
+ * of up to 8 lane streams so scans of different sizes overlap; with graphs
+ * on, one iteration of every scan is captured as a single hipGraph (fork /
+ * join over the lanes) per distinct iteration kind and replayed.  Per scan,
+ * results are identical to pty_run on that context alone. */
+typedef struct pty_batch pty_batch;
+int pty_batch_create(pty_batch** out, int32_t n, pty_ctx* const* ctxs);
+int pty_batch_destroy(pty_batch* batch);
+int pty_batch_set_graphs(pty_batch* batch, int32_t enable);
+/* opts: n entries; metrics_host: n pointers to 3 * iters doubles (or NULL). */
+int pty_batch_run(pty_batch* batch, const pty_opts* opts, int64_t iter0, int32_t iters,
+                  int32_t begin, double* const* metrics_host);
+
 /* ---- per-stage timing (HIP events on the context stream) ---- */
 enum {
   PTY_STAGE_ROWS_FWD = 0,  /* frame formation + forward row FFT */

@@ -27,6 +27,13 @@ _f32 = ctypes.c_float
 _f64 = ctypes.c_double
 
 
+class PtyTuneResult(ctypes.Structure):
+    _fields_ = [("fused", ctypes.c_int32), ("frame_threads", ctypes.c_int32),
+                ("frame_slots", ctypes.c_int32), ("rows_mult", ctypes.c_int32),
+                ("cols_mult", ctypes.c_int32), ("candidates", ctypes.c_int32),
+                ("best_ms", ctypes.c_float), ("default_ms", ctypes.c_float)]
+
+
 class PtyOpts(ctypes.Structure):
     """Mirror of ``pty_opts``."""
     _fields_ = [("algorithm", _i32), ("method", _i32), ("op_n", _i32), ("op_every", _i32),
@@ -57,6 +64,12 @@ SIGNATURES = {
     "pty_pmod1": (ctypes.c_int, [_vp, _vp, _f32, ctypes.POINTER(_f64)]),
     "pty_psup_O": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
+    "pty_set_launch": (ctypes.c_int, [_vp, _i32, _i32]),
+    "pty_tune": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
+    "pty_batch_create": (ctypes.c_int, [_vp, _i32, _vp]),
+    "pty_batch_destroy": (ctypes.c_int, [_vp]),
+    "pty_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
+    "pty_batch_run": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _i32, _i32, _vp]),
     "pty_nn_last_error": (ctypes.c_char_p, []),
     "pty_nn_gemm_probe": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp]),
     "pty_nn_conv3x3_fwd": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32,

@@ -705,6 +705,15 @@ __global__ __launch_bounds__(WG) void k_roll(const float2* src, float2* dst, int
 }
 
 // bbar[p] = (sum over chunks of partb[c][p]) / N
+// dst[i] = src[i % n_per] for i < n (per-frame copies of one background)
+__global__ __launch_bounds__(WG) void k_broadcast(float* __restrict__ dst,
+                                                  const float* __restrict__ src, size_t n_per,
+                                                  size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i % n_per];
+}
+
 __global__ __launch_bounds__(WG) void k_bg_mean(const double* partb, int nchunks, int MK, int N,
                                                 float* bbar) {
   const int p = blockIdx.x * WG + threadIdx.x;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -85,6 +86,14 @@ constexpr int ESUP_BLOCKS = 1024;
 
 }  // namespace
 
+// host-side state an iteration changes; a graph replay restores what its
+// capture left
+struct HostIterState {
+  bool H_valid, G_valid, era_bg_shared, have_metric_state;
+  float2* F;
+  float2* X;
+};
+
 struct pty_ctx {
   int N = 0, M = 0, K = 0, U = 0, V = 0;
   hipStream_t stream = nullptr;
@@ -97,6 +106,8 @@ struct pty_ctx {
   float2* twK = nullptr;
   // column-stage launch shape
   int col_blocks = 0, col_chunk = 1, col_chunks = 0;
+  int rows_mult = 3;         // persistent row kernels: up to rows_mult * CUs work-groups
+  int cols_mult = 0;         // persistent column kernels: cols_mult * CUs (0 = per-mode default)
   int cus = 256;             // compute units of the device (persistent grids)
   // probe-reduction launch shape
   int probe_chunk = 1, probe_chunks = 0;
@@ -164,9 +175,7 @@ struct pty_ctx {
   struct GraphEntry {
     std::vector<uint64_t> key;
     hipGraphExec_t exec;
-    bool H_valid, G_valid, era_bg_shared, have_metric_state;
-    float2* F;
-    float2* X;
+    HostIterState st;   // host state after the captured iteration
   };
   bool graphs = false;
   hipStream_t gstream = nullptr;     // default-priority stream used for capture / replay
@@ -239,7 +248,7 @@ template <int SRC>
 int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows) {
   StageTimer tm(c, PTY_STAGE_ROWS_FWD);
   const int T = threads_per_line(c->K), lpw = WG / T;
-  dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, 3 * c->cus));
+  dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
   switch (c->K) {
 #define C(Lv)                                                                              \
   case Lv:                                                                                 \
@@ -259,7 +268,7 @@ int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const fl
              float scale, double* partial, int nrows, int* nparts = nullptr) {
   StageTimer tm(c, PTY_STAGE_ROWS_INV);
   const int T = threads_per_line(c->K), lpw = WG / T;
-  dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, 3 * c->cus));
+  dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
   if (nparts) *nparts = grid.x;
   switch (c->K) {
 #define C(Lv)                                                                               \
@@ -299,7 +308,7 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   if (!ColTraits<MODE>::bsum) a.chunk = 1;
   const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
   const long items = (long)c->col_blocks * chunks;
-  const int grid = persistent_grid(items, (long)ColTraits<MODE>::wpe * c->cus);
+  const int grid = persistent_grid(items, (long)(c->cols_mult ? c->cols_mult : ColTraits<MODE>::wpe) * c->cus);
   if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
   switch (c->M) {
 #define C(Lv)                                                                                  \
@@ -788,10 +797,23 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)o.probe_centering, bits((double)o.alpha), bits(c->I_norm),
           (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
           (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
+          (uint64_t)c->rows_mult, (uint64_t)c->cols_mult,
           (uint64_t)(uintptr_t)c->O, (uint64_t)(uintptr_t)c->P, (uint64_t)(uintptr_t)c->F,
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
           (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp};
+}
+
+HostIterState save_state(const pty_ctx* c) {
+  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->F, c->X};
+}
+void load_state(pty_ctx* c, const HostIterState& st) {
+  c->H_valid = st.H_valid;
+  c->G_valid = st.G_valid;
+  c->era_bg_shared = st.era_bg_shared;
+  c->have_metric_state = st.have_metric_state;
+  c->F = st.F;
+  c->X = st.X;
 }
 
 void drop_graphs(pty_ctx* c) {
@@ -836,20 +858,62 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
       const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ei != hipSuccess) return fail(PTY_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
-      c->gcache.push_back({key, exec, c->H_valid, c->G_valid, c->era_bg_shared,
-                           c->have_metric_state, c->F, c->X});
+      c->gcache.push_back({key, exec, save_state(c)});
       hit = &c->gcache.back();
     } else {
-      c->H_valid = hit->H_valid;
-      c->G_valid = hit->G_valid;
-      c->era_bg_shared = hit->era_bg_shared;
-      c->have_metric_state = hit->have_metric_state;
-      c->F = hit->F;
-      c->X = hit->X;
+      load_state(c, hit->st);
     }
     PTY_HIP(hipGraphLaunch(hit->exec, c->stream));
   }
   PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+// validation and per-run set-up of pty_run (before any iteration)
+int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t begin) {
+  if (!c || !o) return fail(PTY_EINVAL, "NULL argument");
+  if (o->method < 1 || o->method > 6) return fail(PTY_EINVAL, "method must be 1..6");
+  if (o->algorithm != 0 && o->algorithm != 1) return fail(PTY_EINVAL, "algorithm must be 0 or 1");
+  if (o->op_n < 0 || o->op_every < 1) return fail(PTY_EINVAL, "bad OP_iters");
+  if (iters < 0) return fail(PTY_EINVAL, "iters < 0");
+  PTY_TRY(check_bound(c, o->method >= 4));
+  if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
+  if (iters > c->metrics_cap) {
+    dev_free(c->metrics);
+    PTY_TRY(dev_alloc(&c->metrics, 3 * (size_t)iters));
+    c->metrics_cap = iters;
+  }
+  if (begin) {
+    c->H_valid = c->G_valid = false;
+    c->have_metric_state = false;
+    c->era_bg_shared = false;
+    const size_t UV = (size_t)c->U * c->V;
+    PTY_TRY(copy_d2d(c, c->Obak, c->O, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->Pbak, c->P, c->MK() * sizeof(float2)));
+    if (o->algorithm == 1) {
+      PTY_TRY(ensure_F(c));
+      hipLaunchKernelGGL(k_make_exits, dim3(4096), dim3(WG), 0, c->stream, c->g, (const float2*)c->O,
+                         (const float2*)c->P, c->F);
+      PTY_LAUNCHED("k_make_exits");
+      c->dm_ready = true;
+    }
+  }
+  if (o->algorithm == 1 && !c->dm_ready) return fail(PTY_ESTATE, "DM run needs begin=1 first");
+  c->last_algo = o->algorithm;
+  if (c->use_frame) PTY_TRY(ensure_slots(c));  // never allocate inside a capture
+  return PTY_OK;
+}
+
+int run_one(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_dev) {
+  return o.algorithm == 0 ? era_iter(c, o, it, metrics_dev) : dm_iter(c, o, it, metrics_dev);
+}
+
+int run_finish(pty_ctx* c, int32_t iters, double* metrics_host) {
+  if (metrics_host && iters > 0)
+    PTY_HIP(hipMemcpyAsync(metrics_host, c->metrics, 3 * (size_t)iters * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  if (c->prof_on) prof_collect(c);
   return PTY_OK;
 }
 
@@ -862,6 +926,37 @@ void twiddles(int L, std::vector<float2>& out) {
 }
 
 }  // namespace
+
+// Several scans advanced together: each scan's kernels go to one of up to
+// eight lane streams (default priority), so small scans fill the device
+// side by side; with graphs on, one iteration of every scan is captured as one
+// hipGraph (fork / join through events) and replayed.
+struct pty_batch {
+  std::vector<pty_ctx*> ctx;
+  std::vector<hipStream_t> lanes;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  bool graphs = false;
+  struct Entry {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+    std::vector<HostIterState> st;
+  };
+  std::vector<Entry> cache;
+
+  void drop() {
+    for (auto& e : cache) (void)hipGraphExecDestroy(e.exec);
+    cache.clear();
+  }
+  ~pty_batch() {
+    drop();
+    for (auto l : lanes) (void)hipStreamDestroy(l);
+    if (origin) (void)hipStreamDestroy(origin);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto e : join) (void)hipEventDestroy(e);
+  }
+};
 
 // ==========================================================================
 // C ABI
@@ -1054,8 +1149,8 @@ int pty_set_background(pty_ctx* c, const float* b, int32_t per_frame) {
   if (per_frame) {
     PTY_TRY(copy_d2d(c, c->bg, b, c->NMK() * sizeof(float)));
   } else {
-    for (int n = 0; n < c->N; ++n)
-      PTY_TRY(copy_d2d(c, c->bg + (size_t)n * c->MK(), b, c->MK() * sizeof(float)));
+    hipLaunchKernelGGL(k_broadcast, dim3(4096), dim3(WG), 0, c->stream, c->bg, b, c->MK(), c->NMK());
+    PTY_LAUNCHED("k_broadcast");
   }
   c->bg_set = true;
   c->era_bg_shared = false;
@@ -1083,51 +1178,13 @@ int pty_set_pamp(pty_ctx* c, const float* pamp) {
 
 int pty_run(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t begin,
             double* metrics_host) {
-  if (!c || !o) return fail(PTY_EINVAL, "NULL argument");
-  if (o->method < 1 || o->method > 6) return fail(PTY_EINVAL, "method must be 1..6");
-  if (o->algorithm != 0 && o->algorithm != 1) return fail(PTY_EINVAL, "algorithm must be 0 or 1");
-  if (o->op_n < 0 || o->op_every < 1) return fail(PTY_EINVAL, "bad OP_iters");
-  if (iters < 0) return fail(PTY_EINVAL, "iters < 0");
-  PTY_TRY(check_bound(c, o->method >= 4));
-  if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
-  if (iters > c->metrics_cap) {
-    dev_free(c->metrics);
-    PTY_TRY(dev_alloc(&c->metrics, 3 * (size_t)iters));
-    c->metrics_cap = iters;
-  }
-  if (begin) {
-    c->H_valid = c->G_valid = false;
-    c->have_metric_state = false;
-    c->era_bg_shared = false;
-    const size_t UV = (size_t)c->U * c->V;
-    PTY_TRY(copy_d2d(c, c->Obak, c->O, UV * sizeof(float2)));
-    PTY_TRY(copy_d2d(c, c->Pbak, c->P, c->MK() * sizeof(float2)));
-    if (o->algorithm == 1) {
-      PTY_TRY(ensure_F(c));
-      hipLaunchKernelGGL(k_make_exits, dim3(4096), dim3(WG), 0, c->stream, c->g, (const float2*)c->O,
-                         (const float2*)c->P, c->F);
-      PTY_LAUNCHED("k_make_exits");
-      c->dm_ready = true;
-    }
-  }
-  if (o->algorithm == 1 && !c->dm_ready) return fail(PTY_ESTATE, "DM run needs begin=1 first");
-  c->last_algo = o->algorithm;
-  if (c->use_frame) PTY_TRY(ensure_slots(c));  // never allocate inside a capture
+  PTY_TRY(run_prepare(c, o, iter0, iters, begin));
   if (c->graphs) {
     PTY_TRY(run_graphs(c, *o, iter0, iters));
   } else {
-    for (int32_t k = 0; k < iters; ++k) {
-      const int64_t it = iter0 + k;
-      if (o->algorithm == 0) PTY_TRY(era_iter(c, *o, it, c->metrics + 3 * (size_t)k));
-      else PTY_TRY(dm_iter(c, *o, it, c->metrics + 3 * (size_t)k));
-    }
+    for (int32_t k = 0; k < iters; ++k) PTY_TRY(run_one(c, *o, iter0 + k, c->metrics + 3 * (size_t)k));
   }
-  if (metrics_host && iters > 0)
-    PTY_HIP(hipMemcpyAsync(metrics_host, c->metrics, 3 * (size_t)iters * sizeof(double),
-                           hipMemcpyDeviceToHost, c->stream));
-  PTY_HIP(hipStreamSynchronize(c->stream));
-  if (c->prof_on) prof_collect(c);
-  return PTY_OK;
+  return run_finish(c, iters, metrics_host);
 }
 
 int pty_set_path(pty_ctx* c, int32_t fused, int32_t threads, int32_t slots) {
@@ -1151,6 +1208,236 @@ int pty_set_graphs(pty_ctx* c, int32_t enable) {
   PTY_HIP(hipStreamSynchronize(c->stream));
   c->graphs = enable != 0;
   if (!c->graphs) drop_graphs(c);
+  return PTY_OK;
+}
+
+int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 4)
+    return fail(PTY_EINVAL, "rows_mult must be 1..4 and cols_mult 0..4");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->rows_mult = rows_mult;
+  c->cols_mult = cols_mult;
+  drop_graphs(c);
+  return PTY_OK;
+}
+
+// Time every launch configuration on this scan (`trial_iters` iterations of
+// `o` from the bound state, after one warm-up iteration) and keep the fastest.
+// O, P, the DM exits and the background are restored afterwards, so the
+// caller's next pty_run(begin = 1) starts from the state it set.
+int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result* out) {
+  if (!c || !o || !out || trial_iters < 1) return fail(PTY_EINVAL, "bad pty_tune arguments");
+  PTY_TRY(check_bound(c, o->method >= 4));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  const size_t UV = (size_t)c->U * c->V, MK = c->MK(), NMK = c->NMK();
+  struct Snap {
+    float2 *O = nullptr, *P = nullptr, *F = nullptr;
+    float* B = nullptr;
+    ~Snap() { dev_free(O); dev_free(P); dev_free(F); dev_free(B); }
+  } snap;
+  PTY_TRY(dev_alloc(&snap.O, UV));
+  PTY_TRY(dev_alloc(&snap.P, MK));
+  PTY_TRY(copy_d2d(c, snap.O, c->O, UV * sizeof(float2)));
+  PTY_TRY(copy_d2d(c, snap.P, c->P, MK * sizeof(float2)));
+  float2* F0 = c->F;
+  if (F0) {
+    PTY_TRY(dev_alloc(&snap.F, NMK));
+    PTY_TRY(copy_d2d(c, snap.F, F0, NMK * sizeof(float2)));
+  }
+  if (c->bg && c->bg_set) {
+    PTY_TRY(dev_alloc(&snap.B, NMK));
+    PTY_TRY(copy_d2d(c, snap.B, c->bg, NMK * sizeof(float)));
+  }
+  auto restore = [&]() -> int {
+    PTY_TRY(copy_d2d(c, c->O, snap.O, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->P, snap.P, MK * sizeof(float2)));
+    if (snap.F && c->F) PTY_TRY(copy_d2d(c, c->F, snap.F, NMK * sizeof(float2)));
+    if (snap.B) PTY_TRY(copy_d2d(c, c->bg, snap.B, NMK * sizeof(float)));
+    return PTY_OK;
+  };
+  struct Cfg { int fused, nt, slots, rows, cols; };
+  std::vector<Cfg> cands;
+  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult});
+  for (int rows : {2, 3, 4})
+    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols});
+  if (c->frame_ok)
+    for (int nt : {512, 1024})
+      for (int sl : {c->cus, 2 * c->cus})
+        if (sl <= c->cap_col && sl <= c->cap_esup) cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult});
+  auto apply = [&](const Cfg& g) {
+    c->use_frame = g.fused != 0;
+    c->frame_nt = g.nt;
+    c->frame_slots = g.slots;
+    c->rows_mult = g.rows;
+    c->cols_mult = g.cols;
+    drop_graphs(c);
+  };
+  double best = 1e300, first = 0.0;
+  int besti = 0;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    apply(cands[i]);
+    PTY_TRY(restore());
+    PTY_TRY(pty_run(c, o, 0, 1, 1, nullptr));
+    PTY_TRY(restore());
+    PTY_HIP(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    PTY_TRY(pty_run(c, o, 0, trial_iters, 1, nullptr));
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+        trial_iters;
+    if (i == 0) first = ms;
+    if (ms < best) { best = ms; besti = (int)i; }
+  }
+  apply(cands[besti]);
+  PTY_TRY(restore());
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->H_valid = c->G_valid = false;
+  c->have_metric_state = false;
+  const Cfg& g = cands[besti];
+  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, (int32_t)cands.size(),
+                         (float)best, (float)first};
+  return PTY_OK;
+}
+
+int pty_batch_create(pty_batch** out, int32_t n, pty_ctx* const* ctxs) {
+  if (!out || n < 1 || !ctxs) return fail(PTY_EINVAL, "bad pty_batch_create arguments");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    if (!ctxs[i]) return fail(PTY_EINVAL, "NULL context in the batch");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(PTY_EINVAL, "a context appears twice in the batch");
+  }
+  pty_batch* b = new pty_batch();
+  b->ctx.assign(ctxs, ctxs + n);
+  const int nl = std::min(n, 8);
+  b->lanes.resize(nl, nullptr);
+  b->join.resize(nl, nullptr);
+  hipError_t e = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  for (int l = 0; l < nl && e == hipSuccess; ++l) {
+    e = hipStreamCreateWithFlags(&b->lanes[l], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    delete b;
+    return fail(PTY_EHIP, std::string("pty_batch_create: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return PTY_OK;
+}
+
+int pty_batch_destroy(pty_batch* b) {
+  if (!b) return PTY_OK;
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
+  for (auto l : b->lanes)
+    if (l) (void)hipStreamSynchronize(l);
+  delete b;
+  return PTY_OK;
+}
+
+int pty_batch_set_graphs(pty_batch* b, int32_t enable) {
+  if (!b) return fail(PTY_EINVAL, "batch is NULL");
+  b->graphs = enable != 0;
+  if (!b->graphs) b->drop();
+  return PTY_OK;
+}
+
+// opts: one pty_opts per scan; metrics_host: one pointer per scan (3 * iters
+// doubles, or NULL).  Every scan advances `iters` iterations from iter0.
+int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t iters, int32_t begin,
+                  double* const* metrics_host) {
+  if (!b || !opts) return fail(PTY_EINVAL, "NULL argument");
+  if (iters < 0) return fail(PTY_EINVAL, "iters < 0");
+  const int n = (int)b->ctx.size(), nl = (int)b->lanes.size();
+  std::vector<hipStream_t> user(n);
+  std::vector<bool> prof(n);
+  for (int s = 0; s < n; ++s) {
+    user[s] = b->ctx[s]->stream;
+    prof[s] = b->ctx[s]->prof_on;
+  }
+  struct Restore {
+    pty_batch* b; std::vector<hipStream_t>& user; std::vector<bool>& prof;
+    ~Restore() {
+      for (size_t s = 0; s < b->ctx.size(); ++s) {
+        b->ctx[s]->stream = user[s];
+        b->ctx[s]->use_slot = false;
+        b->ctx[s]->prof_on = prof[s];
+      }
+    }
+  } restore{b, user, prof};
+  for (int s = 0; s < n; ++s) {
+    pty_ctx* c = b->ctx[s];
+    PTY_HIP(hipStreamSynchronize(user[s]));
+    c->stream = b->lanes[s % nl];
+    c->prof_on = false;   // stage events are not recorded in batches
+    PTY_TRY(run_prepare(c, &opts[s], iter0, iters, begin));
+  }
+  for (int l = 0; l < nl; ++l) PTY_HIP(hipStreamSynchronize(b->lanes[l]));
+  if (!b->graphs) {
+    for (int32_t k = 0; k < iters; ++k)
+      for (int s = 0; s < n; ++s)
+        PTY_TRY(run_one(b->ctx[s], opts[s], iter0 + k, b->ctx[s]->metrics + 3 * (size_t)k));
+  } else {
+    const int zero = 0;
+    for (int s = 0; s < n; ++s) {
+      pty_ctx* c = b->ctx[s];
+      if (!c->slot) PTY_TRY(dev_alloc(&c->slot, 1));
+      PTY_HIP(hipMemcpy(c->slot, &zero, sizeof(int), hipMemcpyHostToDevice));
+      c->use_slot = true;
+    }
+    for (int32_t k = 0; k < iters; ++k) {
+      const int64_t it = iter0 + k;
+      std::vector<uint64_t> key;
+      for (int s = 0; s < n; ++s) {
+        std::vector<uint64_t> ks = iter_key(b->ctx[s], opts[s], it);
+        key.push_back(ks.size());
+        key.insert(key.end(), ks.begin(), ks.end());
+      }
+      pty_batch::Entry* hit = nullptr;
+      for (auto& e : b->cache)
+        if (e.key == key) { hit = &e; break; }
+      if (!hit) {
+        PTY_HIP(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+        int r = PTY_OK;
+        hipError_t he = hipEventRecord(b->fork, b->origin);
+        for (int l = 0; l < nl && he == hipSuccess; ++l) he = hipStreamWaitEvent(b->lanes[l], b->fork, 0);
+        for (int s = 0; s < n && r == PTY_OK && he == hipSuccess; ++s)
+          r = run_one(b->ctx[s], opts[s], it, b->ctx[s]->metrics);
+        for (int l = 0; l < nl && he == hipSuccess; ++l) {
+          he = hipEventRecord(b->join[l], b->lanes[l]);
+          if (he == hipSuccess) he = hipStreamWaitEvent(b->origin, b->join[l], 0);
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->origin, &graph);
+        if (r != PTY_OK || he != hipSuccess || ec != hipSuccess) {
+          if (graph) (void)hipGraphDestroy(graph);
+          if (r != PTY_OK) return r;
+          return fail(PTY_EHIP, std::string("batch capture: ") +
+                                    hipGetErrorString(he != hipSuccess ? he : ec));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(PTY_EHIP, std::string("batch instantiate: ") + hipGetErrorString(ei));
+        pty_batch::Entry e{key, exec, {}};
+        for (int s = 0; s < n; ++s) e.st.push_back(save_state(b->ctx[s]));
+        b->cache.push_back(std::move(e));
+        hit = &b->cache.back();
+      } else {
+        for (int s = 0; s < n; ++s) load_state(b->ctx[s], hit->st[s]);
+      }
+      PTY_HIP(hipGraphLaunch(hit->exec, b->origin));
+    }
+    PTY_HIP(hipStreamSynchronize(b->origin));
+  }
+  for (int s = 0; s < n; ++s) {
+    pty_ctx* c = b->ctx[s];
+    if (metrics_host && metrics_host[s] && iters > 0)
+      PTY_HIP(hipMemcpyAsync(metrics_host[s], c->metrics, 3 * (size_t)iters * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  for (int l = 0; l < nl; ++l) PTY_HIP(hipStreamSynchronize(b->lanes[l]));
   return PTY_OK;
 }
 

@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from ._abi import PtyOpts, call
+from ._abi import PtyTuneResult, PtyOpts, call
 
 
 def _ptr(t):
@@ -116,16 +116,35 @@ class Engine:
         call("pty_set_pamp", self._ctx, _ptr(pamp))
 
     # -- running ------------------------------------------------------------
+    @staticmethod
+    def opts(algorithm, method, OP_iters=(1, 1), Pmod_probe_until=0, probe_centering=False,
+             alpha=1e-10):
+        """The ``pty_opts`` of one run."""
+        return PtyOpts(algorithm=int(algorithm), method=int(method), op_n=int(OP_iters[0]),
+                       op_every=int(OP_iters[1]), pmod_probe_until=int(Pmod_probe_until),
+                       probe_centering=int(bool(probe_centering)), alpha=float(alpha))
+
     def run(self, algorithm, method, iters, iter0=0, begin=False, OP_iters=(1, 1),
             Pmod_probe_until=0, probe_centering=False, alpha=1e-10, metrics=True):
         """Run ``iters`` iterations; returns an (iters, 3) array of eMod, eCon, eSup."""
-        o = PtyOpts(algorithm=int(algorithm), method=int(method), op_n=int(OP_iters[0]),
-                    op_every=int(OP_iters[1]), pmod_probe_until=int(Pmod_probe_until),
-                    probe_centering=int(bool(probe_centering)), alpha=float(alpha))
+        o = self.opts(algorithm, method, OP_iters, Pmod_probe_until, probe_centering, alpha)
         out = np.zeros((max(iters, 0), 3), dtype=np.float64) if metrics else None
         ptr = out.ctypes.data_as(ctypes.c_void_p) if (metrics and iters > 0) else None
         call("pty_run", self._ctx, ctypes.byref(o), int(iter0), int(iters), int(bool(begin)), ptr)
         return out
+
+    def set_launch(self, rows_mult=3, cols_mult=0):
+        """Persistent grid sizes of the staged row / column kernels (x CUs)."""
+        call("pty_set_launch", self._ctx, int(rows_mult), int(cols_mult))
+
+    def tune(self, algorithm, method, trial_iters=3, OP_iters=(1, 1), Pmod_probe_until=0,
+             probe_centering=False, alpha=1e-10):
+        """Time every launch configuration on this scan and keep the fastest
+        (state restored; the next run must pass ``begin=True``)."""
+        o = self.opts(algorithm, method, OP_iters, Pmod_probe_until, probe_centering, alpha)
+        r = PtyTuneResult()
+        call("pty_tune", self._ctx, ctypes.byref(o), int(trial_iters), ctypes.byref(r))
+        return {f: getattr(r, f) for f, _ in PtyTuneResult._fields_}
 
     def set_path(self, fused, threads=0, slots=0):
         """Select the fused per-frame kernel (True) or the staged kernels (False)."""

@@ -67,15 +67,25 @@ def _random(rng, shape):
     return np.random.random(shape) if rng is None else rng.random(shape)
 
 
-def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
-          Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
-          full_output=True, device=None, callback=None, rng=None, init=None, init_net=None, graphs=False):
-    """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns.
+class Job:
+    """One prepared scan: the engine with data and state bound, and what the
+    returns need (``prepare`` / ``finish``)."""
 
-    ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
-    the starting object by the network's stitched estimate when its
-    first-iteration eMod beats that of the ``O`` passed in; ``info['init']``
-    records both eMods, the choice and the time spent."""
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def run_kwargs(self):
+        return dict(OP_iters=self.OP_iters, Pmod_probe_until=self.until,
+                    probe_centering=self.probe_centering, alpha=self.alpha)
+
+    def opts(self):
+        return Engine.opts(self.algorithm, self.method, **self.run_kwargs())
+
+
+def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
+            Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None, device=None,
+            rng=None, init=None, init_net=None, graphs=False):
+    """Host preamble of ERA / DM (era.py:435-548) on a new engine; returns a ``Job``."""
     dev = require_gpu(device)
     if algorithm not in (ALGO_ERA, ALGO_DM):
         raise ValueError("algorithm must be 0 (ERA) or 1 (DM)")
@@ -166,43 +176,45 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
         until = pmod_probe_until(Pmod_probe)
         if until > 0:
             eng.set_pamp(None)   # |FFT(P0)| (era.py:546-547)
-
         if graphs:
             eng.set_graphs(True)
-        kw = dict(OP_iters=OP_iters, Pmod_probe_until=until, probe_centering=probe_centering,
-                  alpha=alpha)
-        if callback is None:
-            metrics = eng.run(algorithm, method, iters, iter0=0, begin=True, **kw)
-        else:
-            rows = []
-            for i in range(iters):
-                rows.append(eng.run(algorithm, method, 1, iter0=i, begin=(i == 0), **kw))
-                callback(i, O_t.cpu().numpy().astype(c_dtype), P_t.cpu().numpy().astype(c_dtype))
-            metrics = np.concatenate(rows, axis=0) if rows else np.zeros((0, 3))
-            if iters == 0:
-                eng.run(algorithm, method, 0, iter0=0, begin=True, **kw)
+    except BaseException:
+        eng.close()
+        raise
+    return Job(algorithm=algorithm, eng=eng, O_t=O_t, P_t=P_t, method=method, update=update,
+               r_dtype=r_dtype, c_dtype=c_dtype, bg_init=bg_init, background=background,
+               OP_iters=OP_iters, until=until, probe_centering=probe_centering, alpha=alpha,
+               init_info=init_info, iters=iters)
 
-        O_out = O_t.cpu().numpy().astype(c_dtype)
-        P_out = P_t.cpu().numpy().astype(c_dtype)
+
+def finish(job, metrics, full_output=True):
+    """Returns of ERA / DM (era.py:315-332) from a run job; closes its engine."""
+    eng = job.eng
+    try:
+        c_dtype, r_dtype = job.c_dtype, job.r_dtype
+        O_out = job.O_t.cpu().numpy().astype(c_dtype)
+        P_out = job.P_t.cpu().numpy().astype(c_dtype)
+        update = job.update
         if not full_output:
             return {'O': O_out, 'P': P_out}[update] if update != 'OP' else (O_out, P_out)
 
         info = {}
-        if init_info is not None:
-            info['init'] = init_info
+        if job.init_info is not None:
+            info['init'] = job.init_info
         info['I'] = eng.intensity().cpu().numpy().astype(r_dtype)
         info['eMod'] = [float(x) for x in metrics[:, 0]]
         info['eCon'] = [float(x) for x in metrics[:, 1]]
         info['eSup'] = [float(x) for x in metrics[:, 2]]
         H = eng.heatmap()
         info['heatmap'] = None if H is None else H.cpu().numpy().astype(r_dtype)
-        if method >= 4:
-            if iters > 0:
+        background = job.background
+        if job.method >= 4:
+            if job.iters > 0:
                 b = eng.background().cpu().numpy().astype(r_dtype)
             else:
-                b = np.asarray(bg_init, dtype=r_dtype)
+                b = np.asarray(job.bg_init, dtype=r_dtype)
             info['background'] = np.fft.fftshift(b) ** 2
-        elif background is not None and algorithm == ALGO_ERA:
+        elif background is not None and job.algorithm == ALGO_ERA:
             # era.py:311-314 reports a passed background even when it is unused
             B = _as_numpy(background)
             info['background'] = np.fft.fftshift(B[0] if B.ndim == 3 else B) ** 2
@@ -213,6 +225,39 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
         return O_out, P_out, info
     finally:
         eng.close()
+
+
+def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
+          Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
+          full_output=True, device=None, callback=None, rng=None, init=None, init_net=None,
+          graphs=False):
+    """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns.
+
+    ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
+    the starting object by the network's stitched estimate when its
+    first-iteration eMod beats that of the ``O`` passed in; ``info['init']``
+    records both eMods, the choice and the time spent."""
+    job = prepare(algorithm, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
+                  background=background, method=method, Pmod_probe=Pmod_probe,
+                  probe_centering=probe_centering, alpha=alpha, dtype=dtype, device=device,
+                  rng=rng, init=init, init_net=init_net, graphs=graphs)
+    try:
+        eng, kw = job.eng, job.run_kwargs()
+        if callback is None:
+            metrics = eng.run(algorithm, job.method, iters, iter0=0, begin=True, **kw)
+        else:
+            rows = []
+            for i in range(iters):
+                rows.append(eng.run(algorithm, job.method, 1, iter0=i, begin=(i == 0), **kw))
+                callback(i, job.O_t.cpu().numpy().astype(job.c_dtype),
+                         job.P_t.cpu().numpy().astype(job.c_dtype))
+            metrics = np.concatenate(rows, axis=0) if rows else np.zeros((0, 3))
+            if iters == 0:
+                eng.run(algorithm, job.method, 0, iter0=0, begin=True, **kw)
+    except BaseException:
+        job.eng.close()
+        raise
+    return finish(job, metrics, full_output)
 
 
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmod_probe=False,

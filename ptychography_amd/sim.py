@@ -211,3 +211,69 @@ def make_scan(spec="c1", noise=None, **overrides):
     R = -offsets
     R -= R.max(axis=0)
     return Scan(I=I, R=R, mask=mask, P=P, O=O, B=B, spec=spec)
+
+
+def c4_specs(count=16, seed0=100, frames=(64, 96, 128, 192, 256), positions=(100, 2000)):
+    """The mixed multi-scan workload: scan s (seed seed0 + s) draws its frame
+    side from ``frames`` and its position count from ``positions``, a raster
+    step giving 30-50 % overlap of the probe footprint (half the frame), and
+    follows the 128^2 Zernike-probe recipe scaled to its frame size (pupil
+    radius frame/4, same maximum defocus phase, 10^7 photons, beamstop and
+    8.6 % dead pixels)."""
+    out = []
+    for s in range(count):
+        rng = np.random.default_rng(seed0 + s)
+        frame = int(rng.choice(frames))
+        n = int(rng.integers(positions[0], positions[1] + 1))
+        gy = max(1, int(round(np.sqrt(n))))
+        gx = max(1, int(round(n / gy)))
+        overlap = rng.uniform(0.3, 0.5)
+        f = frame / 128.0
+        out.append(ScanSpec(frame=frame, grid=(gy, gx), step=max(1, int(round(frame / 2 * (1 - overlap)))),
+                            jitter=max(1, int(round(2 * f))), pupil_radius=32.0 * f,
+                            defocus=0.012 / f ** 2, zernike_rms=0.6, photons=1e7,
+                            beamstop_radius=8.0 * f, dead_frac=0.086, seed=seed0 + s))
+    return out
+
+
+def make_scan_gpu(spec, device, poisson=True):
+    """``make_scan`` for large scans: the same draws, with the intensities
+    formed by the engine's own frame kernels on ``device`` (complex64) and
+    returned as a float32 device tensor.  Returns a dict with I, R (int64,
+    max 0), P (scaled like ``make_scan``), O, B, mask, spec."""
+    import torch
+    from .engine import Engine
+
+    rng = np.random.default_rng(spec.seed)
+    M = spec.frame
+    off = make_positions(spec.grid, spec.step, spec.jitter, rng)
+    U = max(int(off[:, 0].max()) + M, spec.obj_size)
+    V = max(int(off[:, 1].max()) + M, spec.obj_size)
+    O = make_object((U, V), rng, spec.amp_range, spec.sigma)
+    P = make_probe(M, spec.pupil_radius, spec.defocus, spec.zernike_rms, rng)
+    R = -off
+    R -= R.max(axis=0)
+    N = len(R)
+    gen = Engine(N, M, M, U, V, R, device=device)
+    try:
+        O_t = torch.from_numpy(O.astype(np.complex64)).to(device)
+        P_t = torch.from_numpy(P.astype(np.complex64)).to(device)
+        gen.set_state(O_t, P_t)
+        I = gen.intensity()
+    finally:
+        gen.close()
+    scale = spec.photons / max(float(I.sum(dtype=torch.float64).item()) / N, 1e-300)
+    I *= scale
+    P = P * np.sqrt(scale)
+    B = None
+    if spec.background_frac > 0:
+        B = _rescale(_gauss_smooth(rng.random((M, M)), 2.0 * spec.sigma), 0.5, 1.5)
+        B *= spec.background_frac * float(I.mean().item()) / B.mean()
+        I += torch.from_numpy(B.astype(np.float32)).to(device)[None]
+    if poisson:
+        g = torch.Generator(device=device).manual_seed(spec.seed)
+        I = torch.poisson(I, generator=g)
+    mask = make_mask(M, spec.beamstop_radius, spec.dead_frac, rng)
+    if mask.all():
+        mask = 1
+    return dict(I=I, R=R, P=P, O=O, B=B, mask=mask, spec=spec)

@@ -1,0 +1,107 @@
+"""Several scans in one batch (lane streams, one hipGraph per iteration) and the
+launch-shape autotuner.
+
+Per scan, a batched run must equal the same scan run alone bit for bit (the
+same kernels, only on another stream / replayed from a graph), and must match
+the oracle run scan by scan."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from ptychography_amd import DM, ERA, DM_many, ERA_many
+from ptychography_amd.engine import Engine
+from scans import broadband_start, rel_l2, small_scan
+
+pytestmark = pytest.mark.gpu
+
+# mixed frame sizes and position counts (the C4 shape, scaled down)
+MIX = [dict(frame=32, grid=(5, 5), step=8, seed=21), dict(frame=64, grid=(4, 6), step=16, seed=22),
+       dict(frame=96, grid=(3, 4), step=24, seed=23), dict(frame=128, grid=(3, 3), step=32, seed=24)]
+
+
+def _jobs(methods):
+    out = []
+    for spec, m in zip(MIX, methods):
+        s = small_scan(**spec)
+        O0 = broadband_start(s.O.shape, seed=spec["seed"])
+        out.append((s, dict(I=s.I, R=s.R.copy(), P=s.P.copy(), O=O0, method=m)))
+    return out
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_era_batch_equals_single_scans(graphs, gpu_device):
+    jobs = _jobs([1, 3, 2, 3])
+    iters = 12
+    res = ERA_many([j for _, j in jobs], iters, graphs=graphs, device=gpu_device)
+    for (s, j), r in zip(jobs, res):
+        single = ERA(s.I, s.R.copy(), j["P"], j["O"], iters, method=j["method"], device=gpu_device)
+        assert len(r) == len(single)
+        for a, b in zip(r[:-1], single[:-1]):
+            assert np.array_equal(a, b)
+        assert r[-1]["eMod"] == single[-1]["eMod"]
+        assert r[-1]["eSup"] == single[-1]["eSup"]
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_dm_background_batch_equals_single_scans(graphs, gpu_device):
+    jobs = []
+    for spec in MIX[:3]:
+        s = small_scan(background_frac=0.01, **spec)
+        O0 = broadband_start(s.O.shape, seed=spec["seed"])
+        B0 = np.full(s.I.shape[1:], 0.01 * s.I.mean())
+        jobs.append((s, dict(I=s.I, R=s.R.copy(), P=s.P.copy(), O=O0, method=4, background=B0)))
+    iters = 8
+    res = DM_many([j for _, j in jobs], iters, graphs=graphs, device=gpu_device)
+    for (s, j), r in zip(jobs, res):
+        single = DM(s.I, s.R.copy(), j["P"], j["O"], iters, method=4, background=j["background"],
+                    device=gpu_device)
+        assert np.array_equal(r[0], single[0])
+        assert r[-1]["eMod"] == single[-1]["eMod"]
+        assert np.array_equal(r[-1]["background"], single[-1]["background"])
+
+
+def test_batch_matches_the_oracle_scan_by_scan(gpu_device):
+    jobs = _jobs([1, 1, 1, 1])
+    iters = 5
+    res = ERA_many([j for _, j in jobs], iters, graphs=True, device=gpu_device)
+    for (s, j), r in zip(jobs, res):
+        O_ref, info_ref = oracle.ERA(s.I, s.R.copy(), j["P"], j["O"], iters, method=1)
+        O_gpu, info = r
+        e_ref, e_gpu = np.asarray(info_ref["eMod"]), np.asarray(info["eMod"])
+        assert np.max(np.abs(e_gpu - e_ref) / e_ref) < 1e-4
+        H = info_ref["heatmap"]
+        cov = H > 1e-3 * H.max()
+        assert rel_l2(O_gpu[cov], O_ref[cov]) < 1e-5
+
+
+def test_tuner_keeps_the_fastest_configuration(gpu_device):
+    s = small_scan(frame=128, grid=(6, 6), step=32, seed=25)
+    O0 = broadband_start(s.O.shape, seed=25).astype(np.complex64)
+    R = s.R - s.R.max(axis=0)
+    N, M, K = s.I.shape
+    U, V = O0.shape
+
+    def fresh():
+        e = Engine(N, M, K, U, V, R, device=gpu_device)
+        amp, I_norm = e.prepare_amp(torch.from_numpy(s.I.astype(np.float32)).to(gpu_device), None)
+        e.set_data(amp, None, I_norm)
+        O_t = torch.from_numpy(O0).to(gpu_device)
+        P_t = torch.from_numpy(s.P.astype(np.complex64)).to(gpu_device)
+        e.set_state(O_t, P_t)
+        return e, O_t, P_t
+
+    e0, O_a, P_a = fresh()
+    m_ref = e0.run(0, 3, 6, begin=True)
+    e0.close()
+    e1, O_b, P_b = fresh()
+    r = e1.tune(0, 3, trial_iters=2)
+    assert r["candidates"] >= 10 and r["best_ms"] <= r["default_ms"]
+    assert r["rows_mult"] in (1, 2, 3, 4) and r["frame_threads"] in (512, 1024)
+    # the tuner restored the state: the run from it matches the untuned run
+    m = e1.run(0, 3, 6, begin=True)
+    e1.close()
+    assert np.max(np.abs(m[:, 0] - m_ref[:, 0]) / m_ref[:, 0]) < 1e-5
+    assert rel_l2(O_b.cpu().numpy(), O_a.cpu().numpy()) < 1e-5
+    assert rel_l2(P_b.cpu().numpy(), P_a.cpu().numpy()) < 1e-5

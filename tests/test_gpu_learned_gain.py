@@ -1,0 +1,80 @@
+"""What the trained initialiser gains over the plain start (held-out scan).
+
+The held-out scan is 32 x 32 positions of noise-free 64^2 frames from an
+object seed the network never saw (training objects use seed 5); ERA method 3
+runs from the flat object (the plain start) and from the gated learned start,
+with the same probe and the frames resident on the GPU.  The weights are the
+committed ones, trained on the GPU by ``python -m ptychography_amd.train_init``.
+"""
+
+import os
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from ptychography_amd import ERA, sim
+from ptychography_amd.learned import InitNet, load_weights
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(__file__), "..", "ptychography_amd", "weights", "init64.npz")
+TARGET = 1e-3
+
+
+@pytest.fixture(scope="module")
+def heldout():
+    s = sim.make_scan("c5", noise=False)
+    return s
+
+
+def _era(s, I_d, dev, iters, init=None, net=None, full=True):
+    U, V = s.O.shape
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = ERA(I_d, s.R.copy(), s.P.copy(), np.ones((U, V), np.complex64), iters, mask=1, method=3,
+            device=dev, init=init, init_net=net, full_output=full)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, r
+
+
+def _iters_to(e, target):
+    hit = np.nonzero(np.asarray(e) <= target)[0]
+    return int(hit[0]) + 1 if len(hit) else None
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
+def test_learned_start_reaches_the_target_faster(heldout, gpu_device):
+    s = heldout
+    net = load_weights(WEIGHTS, device=gpu_device)
+    I_d = torch.from_numpy(s.I.astype(np.float32)).to(gpu_device)
+    _, (_, _, info_p) = _era(s, I_d, gpu_device, 1500)
+    _, (_, _, info_l) = _era(s, I_d, gpu_device, 1500, "learned", net)
+    init = info_l["init"]
+    print("plain start eMod %.3e, learned %.3e, init %.1f ms" %
+          (init["eMod_plain_start"], init["eMod_learned_start"], 1e3 * init["init_seconds"]))
+    assert init["used_learned"] and init["eMod_learned_start"] < 0.2 * init["eMod_plain_start"]
+    k_p, k_l = _iters_to(info_p["eMod"], TARGET), _iters_to(info_l["eMod"], TARGET)
+    print(f"iterations to eMod <= {TARGET}: plain {k_p}, learned {k_l}")
+    assert k_p is not None and k_l is not None and 3 * k_l <= k_p
+    # wall time to the target, everything resident; median of 3
+    _era(s, I_d, gpu_device, k_p, full=False)
+    _era(s, I_d, gpu_device, k_l, "learned", net, full=False)
+    t_p = np.median([_era(s, I_d, gpu_device, k_p, full=False)[0] for _ in range(3)])
+    t_l = np.median([_era(s, I_d, gpu_device, k_l, "learned", net, full=False)[0] for _ in range(3)])
+    print(f"wall to target: plain {1e3 * t_p:.1f} ms, learned {1e3 * t_l:.1f} ms ({t_p / t_l:.2f}x)")
+    assert t_p > 2.0 * t_l
+
+
+def test_untrained_network_is_rejected_by_the_gate(heldout, gpu_device):
+    """Random weights give a worse start than the flat object: the gate keeps
+    the plain start and the run is exactly the plain run."""
+    s = heldout
+    net = InitNet(seed=123, device=gpu_device)
+    I_d = torch.from_numpy(s.I.astype(np.float32)).to(gpu_device)
+    _, (O_p, P_p, info_p) = _era(s, I_d, gpu_device, 5)
+    _, (O_l, P_l, info_l) = _era(s, I_d, gpu_device, 5, "learned", net)
+    assert not info_l["init"]["used_learned"]
+    assert np.array_equal(O_l, O_p) and np.array_equal(P_l, P_p)
+    assert info_l["eMod"] == info_p["eMod"]
