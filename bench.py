@@ -53,6 +53,16 @@ STAGE_BYTES_PX_STAGED = {
 # fused per-frame kernel (timed under the "cols" stage): amplitude 4 in, psi' 8 out
 STAGE_BYTES_PX_FUSED = {"cols": 12, "gather": 8}
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+# scaled x2 / x1 (what the counters read for a 1 GiB copy, --copy-calibration);
+# summary in profiles/era_4096x256_pmc_traffic.md.  Key: (stage, fused, frames, side).
+MEASURED_TRAFFIC = {
+    ("rows_fwd", False, 4096, 256): 3.313e9,
+    ("cols", False, 4096, 256): 5.508e9,
+    ("rows_inv", False, 4096, 256): 4.295e9,
+    ("gather", False, 4096, 256): 2.587e9,
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -69,6 +79,9 @@ def parse():
     ap.add_argument("--frame-slots", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-init timing")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--nn", action="store_true",
+                    help="time the learned-init network only (inference 1024 frames, "
+                         "training steps of 64)")
     return ap.parse_args()
 
 
@@ -84,6 +97,54 @@ def copy_calibration():
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": 1 << 30, "copies": 3}))
+
+
+def nn_bench(dev, reps=10):
+    """Throughput of the init network on its own kernels: inference of 1024
+    64^2 frames and training steps (forward + loss + backward + Adam) of 64."""
+    import torch
+    from ptychography_amd.learned import LAYERS, InitNet
+    H, flops = 64, 0
+    for l in LAYERS:
+        if l[0] == "conv":
+            flops += 2 * 9 * l[1] * l[2] * H * H
+        else:
+            H = H // 2 if l[0] == "pool" else H * 2
+    net = InitNet(seed=1, device=dev)
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.rand((1024, 64, 64, 1), device=dev, generator=g).to(torch.bfloat16)
+    for _ in range(2):
+        net.forward(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        net.forward(x)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t0) / reps
+    xb = x[:64].contiguous()
+    tb = torch.rand((64, 64, 64, 2), device=dev, generator=g)
+    wt = torch.ones(64 * 64 * 2, device=dev)
+
+    def step():
+        pred = net.forward(xb, keep=True)
+        _, dy = net.loss(pred, tb, wt, grad=True, gscale=1024.0, sync=False)
+        net.backward(dy)
+        net.adam(lr=1e-4, gscale=1024.0)
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / reps
+    print(json.dumps({"nn_inference_ms_1024_frames": 1e3 * t_inf,
+                      "nn_inference_tflops": 1024 * flops / t_inf / 1e12,
+                      "nn_train_ms_per_step_64": 1e3 * t_step,
+                      "nn_train_frames_per_s": 64 / t_step,
+                      "nn_train_tflops": 3 * 64 * flops / t_step / 1e12,
+                      "forward_gflop_per_frame": flops / 1e9}))
 
 
 def learned_init(dev, target=1e-3, max_iters=1500, reps=5):
@@ -206,6 +267,9 @@ def main():
     from ptychography_amd.engine import Engine
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.nn:
+        nn_bench(dev)
+        return
 
     t_gen = time.perf_counter()
     scan = make_scan_gpu(args.grid, dev)
@@ -286,7 +350,8 @@ def main():
         "config": {"workload": f"era-method1-{N}x{M}x{M}-object{U}x{V}", "frames": N,
                    "frame": [M, M], "object": [U, V], "method": 1, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": dom_gbs / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((dom, fused, N, M)),
                      "kernel": ("k_frame (fused rows+columns+modulus)" if fused and dom == "cols" else dom),
                      "kernel_ms": stage_avg[dom],
                      "algorithmic_bytes_per_launch": dom_bytes},
