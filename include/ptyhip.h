@@ -123,15 +123,17 @@ int pty_set_graphs(pty_ctx* ctx, int32_t enable);
  * its loop is NumPy on the CPU) ---- */
 /* Persistent staged kernels use up to rows_mult * CUs (1..4, default 3)
  * work-groups for the row FFTs and cols_mult * CUs (1..4; 0 = the default per
- * column mode) for the column FFTs. */
-int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult);
+ * column mode) for the column FFTs; cols_wg (256 or 512) threads per column
+ * work-group (512 = 32-column blocks, 256^2 frames only). */
+int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg);
 typedef struct pty_tune_result {
-  int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult;  /* chosen */
+  int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult, cols_wg;  /* chosen */
   int32_t candidates;                                               /* configurations timed */
   float best_ms, default_ms;                                        /* per iteration */
 } pty_tune_result;
 /* Times `trial_iters` iterations of opts for every launch configuration
- * (staged: rows_mult {2,3,4} x cols_mult {default,2,4}; fused per-frame path
+ * (staged: rows_mult {2,3,4} x cols_mult {default,2,4}, and at 256^2 512-thread
+ * column work-groups x cols_mult {default,1,3}; fused per-frame path
  * where the frame shape allows it: 512 / 1024 threads x CUs / 2*CUs slots),
  * keeps the fastest and restores O, P and the background; the next pty_run
  * must pass begin = 1. */

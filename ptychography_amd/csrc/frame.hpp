@@ -130,15 +130,15 @@ __global__ __launch_bounds__(NT) void k_frame(Geom g, FrameArgs a) {
               const float m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;
               const float am = (ampn + uo)[toff];
               if constexpr (MODE == FR_ERR1) {
-                const float d = sqrtf(z2) - am;
+                const float d = msqrt(z2) - am;
                 acc[0] += (double)(m * d * d);
               } else {
-                const float mg = sqrtf(z2) + a.alpha;
+                const float mg = msqrt(z2) + a.alpha;
                 if constexpr (MODE == FR_ERA1) {
                   const float d = mg - am;
                   acc[0] += (double)(m * d * d);
                 }
-                const float fac = (m != 0.f) ? am / mg : 1.f;
+                const float fac = (m != 0.f) ? am * mrcp(mg) : 1.f;
                 const float2 zn = z * fac;
                 if constexpr (MODE == FR_ERA1) acc[1] += (double)norm2(zn);
                 v[e] = zn * a.scale;

@@ -264,6 +264,12 @@ template <int MODE> struct ColTraits {
                               MODE == COL_DMB_UPD || MODE == COL_DMB_LEAD) ? 2 : 3;
 };
 
+// Modulus-step arithmetic: the hardware square root and reciprocal (1 ulp;
+// the divisor is |psi| + alpha >= alpha > 0) instead of the correctly rounded
+// library sequences, which cost ~10 extra VALU instructions per element.
+__device__ __forceinline__ float msqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float mrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Uniform part of the row index held in register slot e (layoutA / layoutB);
 // the per-thread part is t for both layouts.
 template <int Ra, int Rb, int T>
@@ -271,15 +277,18 @@ __device__ __forceinline__ constexpr int urowA(int e) { return (e / Ra) * T + (e
 template <int Ra, int Rb, int T>
 __device__ __forceinline__ constexpr int urowB(int e) { return (e / Rb) * T + (e % Rb) * Ra; }
 
-template <int L, int MODE>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, WG),
+// CWG threads per work-group: CPW = CWG / T columns per block, so a wave's
+// load reads (64 / T) rows of CPW * 8 contiguous bytes (CWG = 512 at L = 256:
+// 256-byte row segments).
+template <int L, int MODE, int CWG = WG>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                           amdgpu_waves_per_eu(ColTraits<MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
   using TR = ColTraits<MODE>;
-  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = WG / T;
+  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
   __shared__ float2 s_tw[L];
-  __shared__ double red[8];
+  __shared__ double red[2 * (CWG / 64)];
   stage_twiddles<L>(s_tw, a.tw);
   const int tid = threadIdx.x, l = tid % CPW, t = tid / CPW;
   const int K = g.K;
@@ -347,31 +356,31 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WG),
               const float m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;
               const float am = (ampn + uo)[toff];
               if constexpr (MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP) {
-                const float mg = sqrtf(z2) + a.alpha;
+                const float mg = msqrt(z2) + a.alpha;
                 if constexpr (MODE == COL_ERA1) {
                   const float d = mg - am;
                   acc[0] += (double)(m * d * d);
                 }
-                const float fac = (m != 0.f) ? am / mg : 1.f;
+                const float fac = (m != 0.f) ? am * mrcp(mg) : 1.f;
                 const float2 zn = z * fac;
                 if constexpr (MODE == COL_ERA1) acc[1] += (double)norm2(zn);
                 v[e] = zn * a.scale;
               } else if constexpr (MODE == COL_ERR1) {
-                const float d = sqrtf(z2) - am;
+                const float d = msqrt(z2) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERR7) {
                 const float b = (a.bbar + uo)[toff];
-                const float d = sqrtf(z2 + b * b) - am;
+                const float d = msqrt(z2 + b * b) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
                 float* bp = (bgn + uo) + toff;
                 const float b = *bp;
-                const float mg = sqrtf(z2 + b * b + a.alpha);
+                const float mg = msqrt(z2 + b * b + a.alpha);
                 if constexpr (MODE == COL_ERAB) {
                   const float d = mg - am;
                   acc[0] += (double)(m * d * d);
                 }
-                const float fac = m * am / mg;
+                const float fac = m * am * mrcp(mg);
                 const float bn = b * fac;
                 bacc[e] += (double)bn;
                 if constexpr (MODE == COL_DMB_LEAD) *bp = bn;
@@ -383,8 +392,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, WG),
                 const float b = *bp;
                 const float bb = (a.bbar + uo)[toff];
                 const float b0 = 2.f * bb - b;
-                const float mg = sqrtf(z2 + b0 * b0 + a.alpha);
-                const float fac = m * am / mg;
+                const float mg = msqrt(z2 + b0 * b0 + a.alpha);
+                const float fac = m * am * mrcp(mg);
                 const float bn = (b - bb) + b0 * fac;
                 *bp = bn;
                 bacc[e] += (double)bn;

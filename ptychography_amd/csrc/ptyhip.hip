@@ -108,6 +108,7 @@ struct pty_ctx {
   int col_blocks = 0, col_chunk = 1, col_chunks = 0;
   int rows_mult = 3;         // persistent row kernels: up to rows_mult * CUs work-groups
   int cols_mult = 0;         // persistent column kernels: cols_mult * CUs (0 = per-mode default)
+  int cols_wg = 256;         // threads per column work-group (512: wide blocks, 256^2 frames)
   int cus = 256;             // compute units of the device (persistent grids)
   // probe-reduction launch shape
   int probe_chunk = 1, probe_chunks = 0;
@@ -307,17 +308,24 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   // frame chunks only matter where per-chunk background sums are accumulated
   if (!ColTraits<MODE>::bsum) a.chunk = 1;
   const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
-  const long items = (long)c->col_blocks * chunks;
-  const int grid = persistent_grid(items, (long)(c->cols_mult ? c->cols_mult : ColTraits<MODE>::wpe) * c->cus);
+  const bool wide = c->cols_wg == 512 && c->M == 256;
+  const int cpw = (wide ? 512 : WG) / threads_per_line(c->M);
+  const long items = (long)((c->K + cpw - 1) / cpw) * chunks;
+  const int mult = c->cols_mult ? c->cols_mult : (wide ? 2 : ColTraits<MODE>::wpe);
+  const int grid = persistent_grid(items, (long)mult * c->cus);
   if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
-  switch (c->M) {
+  if (wide) {
+    hipLaunchKernelGGL((k_cols<256, MODE, 512>), dim3(grid), dim3(512), 0, c->stream, c->g, a);
+  } else {
+    switch (c->M) {
 #define C(Lv)                                                                                  \
   case Lv:                                                                                     \
     hipLaunchKernelGGL((k_cols<Lv, MODE>), dim3(grid), dim3(WG), 0, c->stream, c->g, a);       \
     break;
-    PTY_SIZES(C)
+      PTY_SIZES(C)
 #undef C
-    default: return fail(PTY_EUNSUP, "unsupported M");
+      default: return fail(PTY_EUNSUP, "unsupported M");
+    }
   }
   PTY_LAUNCHED("k_cols");
   if (nparts) *nparts = grid;
@@ -797,7 +805,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)o.probe_centering, bits((double)o.alpha), bits(c->I_norm),
           (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
           (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
-          (uint64_t)c->rows_mult, (uint64_t)c->cols_mult,
+          (uint64_t)c->rows_mult, (uint64_t)c->cols_mult, (uint64_t)c->cols_wg,
           (uint64_t)(uintptr_t)c->O, (uint64_t)(uintptr_t)c->P, (uint64_t)(uintptr_t)c->F,
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
@@ -1031,6 +1039,8 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->frame_nt = (e_nt && atoi(e_nt) == 512) ? 512 : 1024;
     c->frame_slots = e_sl ? std::max(1, atoi(e_sl)) : cus;
     c->cus = cus;
+    const char* e_cw = getenv("PTY_COLS_WG");
+    if (e_cw && atoi(e_cw) == 512) c->cols_wg = 512;
     // measured: at 256^2 the staged kernels are faster than the fused one
     c->use_frame = c->frame_ok && M == 128 && !(e_on && atoi(e_on) == 0);
     if (e_on && atoi(e_on) == 1) c->use_frame = c->frame_ok;
@@ -1211,13 +1221,15 @@ int pty_set_graphs(pty_ctx* c, int32_t enable) {
   return PTY_OK;
 }
 
-int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult) {
+int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 4)
     return fail(PTY_EINVAL, "rows_mult must be 1..4 and cols_mult 0..4");
+  if (cols_wg != 256 && cols_wg != 512) return fail(PTY_EINVAL, "cols_wg must be 256 or 512");
   PTY_HIP(hipStreamSynchronize(c->stream));
   c->rows_mult = rows_mult;
   c->cols_mult = cols_mult;
+  c->cols_wg = cols_wg;
   drop_graphs(c);
   return PTY_OK;
 }
@@ -1256,21 +1268,25 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     if (snap.B) PTY_TRY(copy_d2d(c, c->bg, snap.B, NMK * sizeof(float)));
     return PTY_OK;
   };
-  struct Cfg { int fused, nt, slots, rows, cols; };
+  struct Cfg { int fused, nt, slots, rows, cols, cwg; };
   std::vector<Cfg> cands;
-  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult});
+  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg});
   for (int rows : {2, 3, 4})
-    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols});
+    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256});
+  if (c->M == 256)
+    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512});
   if (c->frame_ok)
     for (int nt : {512, 1024})
       for (int sl : {c->cus, 2 * c->cus})
-        if (sl <= c->cap_col && sl <= c->cap_esup) cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult});
+        if (sl <= c->cap_col && sl <= c->cap_esup)
+          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg});
   auto apply = [&](const Cfg& g) {
     c->use_frame = g.fused != 0;
     c->frame_nt = g.nt;
     c->frame_slots = g.slots;
     c->rows_mult = g.rows;
     c->cols_mult = g.cols;
+    c->cols_wg = g.cwg;
     drop_graphs(c);
   };
   double best = 1e300, first = 0.0;
@@ -1295,7 +1311,7 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
   c->H_valid = c->G_valid = false;
   c->have_metric_state = false;
   const Cfg& g = cands[besti];
-  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, (int32_t)cands.size(),
+  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, (int32_t)cands.size(),
                          (float)best, (float)first};
   return PTY_OK;
 }

@@ -99,9 +99,12 @@ def test_tuner_keeps_the_fastest_configuration(gpu_device):
     r = e1.tune(0, 3, trial_iters=2)
     assert r["candidates"] >= 10 and r["best_ms"] <= r["default_ms"]
     assert r["rows_mult"] in (1, 2, 3, 4) and r["frame_threads"] in (512, 1024)
-    # the tuner restored the state: the run from it matches the untuned run
+    # the tuner restored the state: the run from it matches the untuned run up
+    # to float32 round-off (the chosen configuration may take the other code
+    # path -- fused per-frame vs staged kernels -- and reduce in another order;
+    # each path is checked against the oracle on its own)
     m = e1.run(0, 3, 6, begin=True)
     e1.close()
-    assert np.max(np.abs(m[:, 0] - m_ref[:, 0]) / m_ref[:, 0]) < 1e-5
-    assert rel_l2(O_b.cpu().numpy(), O_a.cpu().numpy()) < 1e-5
-    assert rel_l2(P_b.cpu().numpy(), P_a.cpu().numpy()) < 1e-5
+    assert np.max(np.abs(m[:, 0] - m_ref[:, 0]) / m_ref[:, 0]) < 1e-4
+    assert rel_l2(O_b.cpu().numpy(), O_a.cpu().numpy()) < 1e-4
+    assert rel_l2(P_b.cpu().numpy(), P_a.cpu().numpy()) < 1e-4
