@@ -16,6 +16,11 @@ to eMod <= 1e-3 of ERA method 3 on the held-out 32 x 32-position scan of
 ``python -m ptychography_amd.train_init``) against the flat start, under
 "learned_init".
 
+"multi_scan" reports the batched path: the 16 seeded scans of
+``sim.c4_specs()`` (frames 64-192^2, 240-1980 positions, 334 M frame pixels
+per iteration), ERA method 3, as sequential single-scan calls and as one batch
+with one hipGraph per iteration, inputs resident.
+
 ``--copy-calibration`` only copies 1 GiB three times (the plain copy kernel
 whose FETCH_SIZE / WRITE_SIZE counters scale the kernel traffic figures).
 
@@ -78,6 +83,8 @@ def parse():
     ap.add_argument("--frame-threads", type=int, default=0)
     ap.add_argument("--frame-slots", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-init timing")
+    ap.add_argument("--no-multi", action="store_true", help="skip the multi-scan timing")
+    ap.add_argument("--multi-iters", type=int, default=150)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--nn", action="store_true",
                     help="time the learned-init network only (inference 1024 frames, "
@@ -145,6 +152,44 @@ def nn_bench(dev, reps=10):
                       "nn_train_frames_per_s": 64 / t_step,
                       "nn_train_tflops": 3 * 64 * flops / t_step / 1e12,
                       "forward_gflop_per_frame": flops / 1e9}))
+
+
+def multi_scan(dev, iters):
+    """16 mixed scans, ERA method 3: sequential calls vs one batch with graphs."""
+    import torch
+    from ptychography_amd import ERA, ERA_many, sim
+    scans = [sim.make_scan_gpu(sp, dev) for sp in sim.c4_specs()]
+    px = sum(int(np.prod(sc["I"].shape)) for sc in scans)
+
+    def job(sc):
+        U, V = sc["O"].shape
+        return dict(I=sc["I"], R=sc["R"].copy(), P=sc["P"].copy(),
+                    O=np.ones((U, V), np.complex64), mask=sc["mask"], method=3)
+
+    def sequential():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for sc in scans:
+            j = job(sc)
+            ERA(j.pop("I"), j.pop("R"), j.pop("P"), j.pop("O"), iters, device=dev,
+                full_output=False, **j)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def batched():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ERA_many([job(sc) for sc in scans], iters, device=dev, full_output=False, graphs=True)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    sequential(), batched()
+    t_seq = min(sequential() for _ in range(2))
+    t_bat = min(batched() for _ in range(2))
+    return {"workload": "c4-16-scans-method3", "scans": len(scans), "frame_px_per_iter": px,
+            "iters": iters, "sequential_s": t_seq, "batched_graphs_s": t_bat,
+            "sequential_gpx_iter_per_s": px * iters / t_seq / 1e9,
+            "batched_gpx_iter_per_s": px * iters / t_bat / 1e9, "speedup": t_seq / t_bat}
 
 
 def learned_init(dev, target=1e-3, max_iters=1500, reps=5):
@@ -329,6 +374,9 @@ def main():
     learned = None
     if not args.no_learned and os.path.exists(WEIGHTS):
         learned = learned_init(dev)
+    multi = None
+    if not args.no_multi:
+        multi = multi_scan(dev, args.multi_iters)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -362,6 +410,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": dm,
         "learned_init": learned,
+        "multi_scan": multi,
     }
     print(json.dumps(out))
 
