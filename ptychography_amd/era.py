@@ -151,7 +151,11 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
         O_t = torch.from_numpy(np.ascontiguousarray(O.astype(np.complex64))).to(dev)
         P_t = torch.from_numpy(np.ascontiguousarray(P.astype(np.complex64))).to(dev)
         init_info = None
-        if init == 'learned':
+        if init == 'learned' and (M, K) != (init_net.frame, init_net.frame):
+            init_info = dict(used_learned=False,
+                             reason=f"the network takes {init_net.frame}x{init_net.frame} frames, "
+                                    f"the scan has {M}x{K}")
+        elif init == 'learned':
             from .learned import learned_start
             O_t, init_info = learned_start(init_net, eng, I_t.to(dev, torch.float32).contiguous(),
                                            P_t, O_t, alpha=alpha)
@@ -236,7 +240,8 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
     ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
     the starting object by the network's stitched estimate when its
     first-iteration eMod beats that of the ``O`` passed in; ``info['init']``
-    records both eMods, the choice and the time spent."""
+    records both eMods, the choice and the time spent (or, for frames of
+    another size than the network's, that the plain start was kept and why)."""
     job = prepare(algorithm, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
                   background=background, method=method, Pmod_probe=Pmod_probe,
                   probe_centering=probe_centering, alpha=alpha, dtype=dtype, device=device,

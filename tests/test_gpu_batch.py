@@ -17,8 +17,12 @@ from scans import broadband_start, rel_l2, small_scan
 pytestmark = pytest.mark.gpu
 
 # mixed frame sizes and position counts (the C4 shape, scaled down)
-MIX = [dict(frame=32, grid=(5, 5), step=8, seed=21), dict(frame=64, grid=(4, 6), step=16, seed=22),
-       dict(frame=96, grid=(3, 4), step=24, seed=23), dict(frame=128, grid=(3, 3), step=32, seed=24)]
+# mixed frame sizes and position counts (the multi-scan shape, scaled down);
+# two scans carry a beamstop and dead pixels
+MIX = [dict(frame=32, grid=(5, 5), step=8, seed=21),
+       dict(frame=64, grid=(4, 6), step=16, seed=22, beamstop_radius=3.0, dead_frac=0.05),
+       dict(frame=96, grid=(3, 4), step=24, seed=23),
+       dict(frame=128, grid=(3, 3), step=32, seed=24, beamstop_radius=6.0, dead_frac=0.05)]
 
 
 def _jobs(methods):
@@ -26,7 +30,7 @@ def _jobs(methods):
     for spec, m in zip(MIX, methods):
         s = small_scan(**spec)
         O0 = broadband_start(s.O.shape, seed=spec["seed"])
-        out.append((s, dict(I=s.I, R=s.R.copy(), P=s.P.copy(), O=O0, method=m)))
+        out.append((s, dict(I=s.I, R=s.R.copy(), P=s.P.copy(), O=O0, method=m, mask=s.mask)))
     return out
 
 
@@ -36,7 +40,8 @@ def test_era_batch_equals_single_scans(graphs, gpu_device):
     iters = 12
     res = ERA_many([j for _, j in jobs], iters, graphs=graphs, device=gpu_device)
     for (s, j), r in zip(jobs, res):
-        single = ERA(s.I, s.R.copy(), j["P"], j["O"], iters, method=j["method"], device=gpu_device)
+        single = ERA(s.I, s.R.copy(), j["P"], j["O"], iters, method=j["method"], mask=j["mask"],
+                     device=gpu_device)
         assert len(r) == len(single)
         for a, b in zip(r[:-1], single[:-1]):
             assert np.array_equal(a, b)
@@ -67,7 +72,7 @@ def test_batch_matches_the_oracle_scan_by_scan(gpu_device):
     iters = 5
     res = ERA_many([j for _, j in jobs], iters, graphs=True, device=gpu_device)
     for (s, j), r in zip(jobs, res):
-        O_ref, info_ref = oracle.ERA(s.I, s.R.copy(), j["P"], j["O"], iters, method=1)
+        O_ref, info_ref = oracle.ERA(s.I, s.R.copy(), j["P"], j["O"], iters, method=1, mask=j["mask"])
         O_gpu, info = r
         e_ref, e_gpu = np.asarray(info_ref["eMod"]), np.asarray(info["eMod"])
         assert np.max(np.abs(e_gpu - e_ref) / e_ref) < 1e-4

@@ -78,3 +78,15 @@ def test_untrained_network_is_rejected_by_the_gate(heldout, gpu_device):
     assert not info_l["init"]["used_learned"]
     assert np.array_equal(O_l, O_p) and np.array_equal(P_l, P_p)
     assert info_l["eMod"] == info_p["eMod"]
+
+
+def test_other_frame_sizes_keep_the_plain_start(gpu_device):
+    from scans import small_scan
+    s = small_scan(frame=32, grid=(4, 4), step=8)
+    net = InitNet(seed=1, device=gpu_device)
+    O0 = np.ones(s.O.shape, np.complex128)
+    O_l, info_l = ERA(s.I, s.R.copy(), s.P, O0, 3, method=1, device=gpu_device, init="learned",
+                      init_net=net)
+    O_p, info_p = ERA(s.I, s.R.copy(), s.P, O0, 3, method=1, device=gpu_device)
+    assert not info_l["init"]["used_learned"] and "64x64" in info_l["init"]["reason"]
+    assert np.array_equal(O_l, O_p)
