@@ -1169,13 +1169,15 @@ int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const uint16_t* wT,
     NN_LAUNCHED("k_conv_c1");
     return PTY_OK;
   }
-  if (Cout <= 8 && Cin % 8 == 0 && Cin <= 64) {
+  // the vector kernels read and write 16-byte groups of channels
+  const bool al = aligned16(x) && aligned16(wT) && (!y || aligned16(y));
+  if (Cout <= 8 && Cin % 8 == 0 && Cin <= 64 && aligned16(x)) {
     hipLaunchKernelGGL(k_conv_nsmall, dim3(grid_for(P)), dim3(256), 0, s, x, w, B, H, W, Cin,
                        Cout, o);
     NN_LAUNCHED("k_conv_nsmall");
     return PTY_OK;
   }
-  if (wT && direct_ok(W, Cin, Cout)) return launch_conv<false>(x, wT, B * H, H, W, Cin, Cout, o, s);
+  if (wT && al && direct_ok(W, Cin, Cout)) return launch_conv<false>(x, wT, B * H, H, W, Cin, Cout, o, s);
   ConvGeom g{B, H, W, Cin, Cout};
   return igemm(B * H * W, Cout, 9 * Cin, LoadFwdA{x, g}, LoadFwdB{w, g},
                EpiConv{y, yf, bias, Cout, relu}, s);
@@ -1188,13 +1190,13 @@ int pty_nn_conv3x3_dgrad(const uint16_t* dy, const uint16_t* w, uint16_t* dx,
     return nn_fail(PTY_EINVAL, "bad dgrad args");
   hipStream_t s = (hipStream_t)stream;
   const size_t P = (size_t)B * H * W;
-  if (Cout <= 8 && Cin % 8 == 0 && Cin <= 64) {
+  if (Cout <= 8 && Cin % 8 == 0 && Cin <= 64 && aligned16(dx)) {
     hipLaunchKernelGGL(k_dgrad_nsmall, dim3(grid_for(P)), dim3(256), 0, s, dy, w, B, H, W, Cin,
                        Cout, dx, mask);
     NN_LAUNCHED("k_dgrad_nsmall");
     return PTY_OK;
   }
-  if (direct_ok(W, Cout, Cin))
+  if (direct_ok(W, Cout, Cin) && aligned16(dy) && aligned16(w) && aligned16(dx))
     return launch_conv<true>(dy, w, B * H, H, W, Cout, Cin, ConvOut{dx, nullptr, nullptr, mask, 0},
                              s);
   ConvGeom g{B, H, W, Cin, Cout};
@@ -1247,7 +1249,7 @@ int pty_nn_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float
   const long P = (long)B * H * W;
   const size_t cnt = (size_t)9 * Cin * Cout;
   float* wsb = ws + (size_t)splits * cnt;
-  if (direct_ok(W, Cin, Cout)) {
+  if (direct_ok(W, Cin, Cout) && aligned16(x) && aligned16(dy)) {
     const int R = 128 / W;
     const int ntiles = (B * H + R - 1) / R;
     if (Cout % 64 == 0)

@@ -6,10 +6,12 @@ No counterpart in the reference: this is the initial-guess stage that feeds
 (64 x 64) to the complex object patch seen by that frame (real / imaginary
 channels, gauge-fixed so the illumination-weighted patch mean is real and
 positive -- a diffraction pattern cannot tell an object from the same object
-times a global phase).  Patches are stitched into O0 by a weighted overlap
-average that rotates each patch onto what is already stitched.  The plain
-start stays the fallback and the judge: ``initial_object`` keeps the learned
-O0 only if its first-iteration eMod beats the flat start's.
+times a global phase).  The patches' unknown phases are synchronised globally
+from the pairwise overlap products of neighbouring windows (projected power
+iterations, ``sync_phases``), and O0 is the engine's overlap projection of the
+exits P * patch * phase (the |P|^2-weighted average).  The plain start stays
+the fallback and the judge: ``learned_start`` keeps the learned O0 only if its
+first-iteration eMod beats the plain start's.
 
 Layers (NHWC, bf16 activations and weights, FP32 accumulate):
   enc  64^2: 1->32, 32->32 | pool | 32^2: 32->64, 64->64 | pool |
