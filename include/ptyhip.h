@@ -197,6 +197,13 @@ int pty_psup_O(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* P_de
 int pty_psup_P(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* O_dev,
                pty_cfloat* P_out_dev, double* G_dev, int32_t fresh, float alpha);
 
+/* Back-projection initial object (back_projection.py:4-88, with the ERA
+ * preamble's centring): exits_n = IFFT2(mask * amp_n * FFT2(P) / |FFT2(P)|)
+ * from the bound amplitudes / mask, O_out = psup_O(exits, P) and its fresh
+ * heatmap H (U x V float64); exits_out (N x M x K) is optional. */
+int pty_back_projection(pty_ctx* ctx, const pty_cfloat* P_dev, pty_cfloat* exits_out_dev,
+                        pty_cfloat* O_out_dev, double* H_dev, float alpha);
+
 /* ---- learned initialiser (no counterpart in the reference; the north-star
  * conv encoder-decoder whose output seeds O) -------------------------------
  * bf16 tensors are uint16 bit patterns; activations NHWC; weights

@@ -25,6 +25,10 @@ own formula; the tests live in ``tests/test_oracle_*.py``):
                                    background scaled by the same factor).
 * ``Pmod_P``                    -- modulus invariant on the probe.
 * ``center_of_mass``            -- ``scipy.ndimage.center_of_mass``.
+* ``back_projection.Back_projection`` -- exact-data closed form (flat object:
+                                   the exits are P, O = H / (H + alpha)) and the
+                                   exit spectra's modulus (mask * sqrt I) and
+                                   phase (the probe's far field).
 * ``ERA`` (methods 1-6)         -- monotone eMod for method 1, exact-data fixed
                                    point, an independent pure-loop ERA on tiny
                                    inputs (direct DFT, explicit loops).

@@ -8,6 +8,7 @@ hand-written gfx950 HIP kernels in ``libptyhip.so`` through a ctypes C ABI.
 from .era import ERA
 from .dm import DM
 from .batch import ERA_many, DM_many
+from .back_projection import Back_projection
 from . import sim
 
-__all__ = ["ERA", "DM", "ERA_many", "DM_many", "sim"]
+__all__ = ["ERA", "DM", "ERA_many", "DM_many", "Back_projection", "sim"]

@@ -65,6 +65,7 @@ SIGNATURES = {
     "pty_pmod1": (ctypes.c_int, [_vp, _vp, _f32, ctypes.POINTER(_f64)]),
     "pty_psup_O": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
+    "pty_back_projection": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _f32]),
     "pty_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "pty_tune": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
     "pty_batch_create": (ctypes.c_int, [_vp, _i32, _vp]),

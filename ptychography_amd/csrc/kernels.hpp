@@ -754,6 +754,25 @@ __global__ __launch_bounds__(WG) void k_prepare_amp(const float* I, int N, int M
   if (threadIdx.x == 0) part[blockIdx.x] = r[0];
 }
 
+// Back-projection spectra (back_projection.py:75-78): X[n][p] = mask[p] amp[n][p]
+// Pf[p] / |Pf[p]| with Pf = FFT2(P) (natural order, like amp and mask); a zero
+// Pf gives phase 1 (numpy's angle(0) = 0).
+__global__ __launch_bounds__(WG) void k_bp_spectra(float2* __restrict__ X,
+                                                   const float* __restrict__ amp,
+                                                   const uint8_t* __restrict__ mask,
+                                                   const float2* __restrict__ Pf, size_t MK,
+                                                   size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t p = i % MK;
+    const float2 f = Pf[p];
+    const float m = sqrtf(norm2(f));
+    const float2 ph = m > 0.f ? make_float2(f.x / m, f.y / m) : make_float2(1.f, 0.f);
+    const float a = amp[i] * (mask ? (float)mask[p] : 1.f);
+    X[i] = make_float2(a * ph.x, a * ph.y);
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_abs(const float2* X, float* out, size_t count) {
   for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < count;
        idx += (size_t)gridDim.x * WG)

@@ -1589,4 +1589,35 @@ int pty_psup_P(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* O, pty_cfloat
   return PTY_OK;
 }
 
+// Back-projection initial object (back_projection.py:4-88): exits_n =
+// IFFT2(mask * amp_n * FFT2(P) / |FFT2(P)|), O = psup_O(exits, P) with a fresh
+// heatmap H.  Uses the bound amplitudes / mask (pty_set_data); exits_out
+// (N x M x K, optional) receives the exit waves.
+int pty_back_projection(pty_ctx* c, const pty_cfloat* P, pty_cfloat* exits_out, pty_cfloat* O_out,
+                        double* H, float alpha) {
+  if (!c || !P || !O_out || !H) return fail(PTY_EINVAL, "NULL argument");
+  if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
+  const float2* Pd = reinterpret_cast<const float2*>(P);
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, Pd, c->Xp, c->M));
+  ColArgs a = col_args(c);
+  a.X = c->Xp;
+  a.nframes = 1;
+  a.chunk = 1;
+  PTY_TRY(cols<COL_FWD>(c, a));
+  hipLaunchKernelGGL(k_bp_spectra, dim3(4096), dim3(WG), 0, c->stream, c->X, (const float*)c->amp,
+                     c->mask, (const float2*)c->Xp, c->MK(), c->NMK());
+  PTY_LAUNCHED("k_bp_spectra");
+  ColArgs b = col_args(c);
+  b.X = c->X;
+  b.nframes = c->N;
+  b.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+  PTY_TRY(cols<COL_INV>(c, b));
+  PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->X, nullptr, nullptr, 1.0f, nullptr, c->N * c->M));
+  if (exits_out) PTY_TRY(copy_d2d(c, exits_out, c->X, c->NMK() * sizeof(float2)));
+  PTY_TRY(gather(c, c->X, Pd, nullptr, reinterpret_cast<float2*>(O_out), H, 1, 1, alpha, false,
+                 false));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
 }  // extern "C"

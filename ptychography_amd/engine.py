@@ -212,6 +212,16 @@ class Engine:
         call("pty_psup_O", self._ctx, _ptr(exits), _ptr(P), _ptr(O), _ptr(H), int(fresh), float(alpha))
         return O, H
 
+    def back_projection(self, P, alpha=1e-10, exits=False):
+        """Back-projection object from the bound data: (O, H, exits or None)."""
+        O = torch.empty((self.U, self.V), dtype=torch.complex64, device=self.device)
+        H = torch.empty((self.U, self.V), dtype=torch.float64, device=self.device)
+        ex = torch.empty((self.N, self.M, self.K), dtype=torch.complex64, device=self.device) \
+            if exits else None
+        self._sync_torch()
+        call("pty_back_projection", self._ctx, _ptr(P), _ptr(ex), _ptr(O), _ptr(H), float(alpha))
+        return O, H, ex
+
     def psup_P(self, exits, O, G=None, alpha=1e-10):
         P = torch.empty((self.M, self.K), dtype=torch.complex64, device=self.device)
         fresh = G is None
