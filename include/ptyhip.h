@@ -118,6 +118,10 @@ int pty_set_path(pty_ctx* ctx, int32_t fused, int32_t threads, int32_t slots);
  * replays it; results are identical to eager launches.  Stage timing is not
  * recorded while graphs are on. */
 int pty_set_graphs(pty_ctx* ctx, int32_t enable);
+/* Whether graphs are on, and whether a capture or instantiation failed during
+ * a run (the run then finished with eager launches and graphs were switched
+ * off; nothing of the failed capture ran). */
+int pty_get_graphs(pty_ctx* ctx, int32_t* enabled, int32_t* fell_back);
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference:
  * its loop is NumPy on the CPU) ---- */
@@ -150,6 +154,7 @@ typedef struct pty_batch pty_batch;
 int pty_batch_create(pty_batch** out, int32_t n, pty_ctx* const* ctxs);
 int pty_batch_destroy(pty_batch* batch);
 int pty_batch_set_graphs(pty_batch* batch, int32_t enable);
+int pty_batch_get_graphs(pty_batch* batch, int32_t* enabled, int32_t* fell_back);
 /* opts: n entries; metrics_host: n pointers to 3 * iters doubles (or NULL). */
 int pty_batch_run(pty_batch* batch, const pty_opts* opts, int64_t iter0, int32_t iters,
                   int32_t begin, double* const* metrics_host);

@@ -71,6 +71,8 @@ SIGNATURES = {
     "pty_batch_create": (ctypes.c_int, [_vp, _i32, _vp]),
     "pty_batch_destroy": (ctypes.c_int, [_vp]),
     "pty_batch_set_graphs": (ctypes.c_int, [_vp, _i32]),
+    "pty_batch_get_graphs": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
+    "pty_get_graphs": (ctypes.c_int, [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "pty_batch_run": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _i32, _i32, _vp]),
     "pty_nn_last_error": (ctypes.c_char_p, []),
     "pty_nn_gemm_probe": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp]),

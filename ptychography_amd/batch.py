@@ -48,6 +48,12 @@ class Batch:
         call("pty_batch_run", self._b, oa, int(iter0), int(iters), int(bool(begin)), ptrs)
         return out
 
+    def graph_status(self):
+        """(graphs on, a capture failed and the batch fell back to eager launches)."""
+        en, fb = ctypes.c_int32(), ctypes.c_int32()
+        call("pty_batch_get_graphs", self._b, ctypes.byref(en), ctypes.byref(fb))
+        return bool(en.value), bool(fb.value)
+
     def close(self):
         if self._b:
             load().pty_batch_destroy(self._b)
@@ -61,7 +67,7 @@ class Batch:
 
 
 def solve_many(algorithm, scans, iters, graphs=False, tune=False, full_output=True, device=None,
-               **common):
+               check_finite=True, **common):
     """Run ``algorithm`` (0 = ERA, 1 = DM) on every scan of ``scans`` together.
 
     ``scans``: list of dicts of ERA / DM arguments (``I``, ``R``, ``P``, ``O``,
@@ -81,13 +87,21 @@ def solve_many(algorithm, scans, iters, graphs=False, tune=False, full_output=Tr
         batch = Batch([j.eng for j in jobs], graphs=graphs)
         try:
             metrics = batch.run([j.opts() for j in jobs], iters, begin=True)
+            status = batch.graph_status()
         finally:
             batch.close()
     except BaseException:
         for j in jobs:
             j.eng.close()
         raise
-    return [finish(j, m, full_output) for j, m in zip(jobs, metrics)]
+    out = []
+    for k, (j, m) in enumerate(zip(jobs, metrics)):
+        j.graphs = False   # the batch, not the scan's engine, replayed the graphs
+        r = finish(j, m, full_output, check_finite)
+        if full_output and graphs:
+            r[-1]['graphs'] = {'enabled': status[0], 'fell_back': status[1]}
+        out.append(r)
+    return out
 
 
 def ERA_many(scans, iters, **kw):

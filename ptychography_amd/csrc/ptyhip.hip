@@ -179,6 +179,7 @@ struct pty_ctx {
     HostIterState st;   // host state after the captured iteration
   };
   bool graphs = false;
+  bool graphs_fell_back = false;     // a capture failed; iterations run eagerly since
   hipStream_t gstream = nullptr;     // default-priority stream used for capture / replay
   int* slot = nullptr;               // device metrics slot counter
   bool use_slot = false;             // k_metrics writes through the slot counter
@@ -829,6 +830,15 @@ void drop_graphs(pty_ctx* c) {
   c->gcache.clear();
 }
 
+// Host-side switch that treats every capture as failed (tests of the eager
+// fallback): PTY_FORCE_CAPTURE_FAILURE=1.  Read on every capture.
+bool capture_fail_forced() {
+  const char* e = getenv("PTY_FORCE_CAPTURE_FAILURE");
+  return e && atoi(e) == 1;
+}
+
+int run_one(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_dev);
+
 // Iterations as replays of captured graphs, one graph per distinct iteration
 // kind (cached with the host-state transitions the iteration implies).
 int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
@@ -848,24 +858,40 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
   } restore{c, user, prof};
   for (int32_t k = 0; k < iters; ++k) {
     const int64_t it = iter0 + k;
+    if (!c->graphs) {   // a capture failed earlier in this run: continue eagerly
+      PTY_TRY(run_one(c, o, it, c->metrics + 3 * (size_t)k));
+      continue;
+    }
     std::vector<uint64_t> key = iter_key(c, o, it);
     pty_ctx::GraphEntry* hit = nullptr;
     for (auto& e : c->gcache)
       if (e.key == key) { hit = &e; break; }
     if (!hit) {
+      const HostIterState before = save_state(c);
       PTY_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int r = o.algorithm == 0 ? era_iter(c, o, it, c->metrics) : dm_iter(c, o, it, c->metrics);
+      const int r = run_one(c, o, it, c->metrics);
       hipGraph_t graph = nullptr;
       const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
       if (r != PTY_OK) {
         if (graph) (void)hipGraphDestroy(graph);
         return r;
       }
-      if (ec != hipSuccess) return fail(PTY_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
       hipGraphExec_t exec = nullptr;
-      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ei != hipSuccess) return fail(PTY_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      hipError_t ei = ec;
+      if (ec == hipSuccess) ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess || capture_fail_forced()) {
+        // nothing of the captured iteration ran: undo its host-side effects,
+        // drop the graphs and run this and the remaining iterations eagerly
+        if (exec) (void)hipGraphExecDestroy(exec);
+        load_state(c, before);
+        drop_graphs(c);
+        c->graphs = false;
+        c->graphs_fell_back = true;
+        c->use_slot = false;
+        PTY_TRY(run_one(c, o, it, c->metrics + 3 * (size_t)k));
+        continue;
+      }
       c->gcache.push_back({key, exec, save_state(c)});
       hit = &c->gcache.back();
     } else {
@@ -946,6 +972,7 @@ struct pty_batch {
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
   bool graphs = false;
+  bool fell_back = false;   // a capture failed; the batch runs eagerly since
   struct Entry {
     std::vector<uint64_t> key;
     hipGraphExec_t exec;
@@ -1221,6 +1248,13 @@ int pty_set_graphs(pty_ctx* c, int32_t enable) {
   return PTY_OK;
 }
 
+int pty_get_graphs(pty_ctx* c, int32_t* enabled, int32_t* fell_back) {
+  if (!c || !enabled || !fell_back) return fail(PTY_EINVAL, "NULL argument");
+  *enabled = c->graphs ? 1 : 0;
+  *fell_back = c->graphs_fell_back ? 1 : 0;
+  return PTY_OK;
+}
+
 int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 4)
@@ -1352,6 +1386,13 @@ int pty_batch_destroy(pty_batch* b) {
   return PTY_OK;
 }
 
+int pty_batch_get_graphs(pty_batch* b, int32_t* enabled, int32_t* fell_back) {
+  if (!b || !enabled || !fell_back) return fail(PTY_EINVAL, "NULL argument");
+  *enabled = b->graphs ? 1 : 0;
+  *fell_back = b->fell_back ? 1 : 0;
+  return PTY_OK;
+}
+
 int pty_batch_set_graphs(pty_batch* b, int32_t enable) {
   if (!b) return fail(PTY_EINVAL, "batch is NULL");
   b->graphs = enable != 0;
@@ -1404,6 +1445,11 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
     }
     for (int32_t k = 0; k < iters; ++k) {
       const int64_t it = iter0 + k;
+      if (!b->graphs) {   // a capture failed earlier in this run: continue eagerly
+        for (int s = 0; s < n; ++s)
+          PTY_TRY(run_one(b->ctx[s], opts[s], it, b->ctx[s]->metrics + 3 * (size_t)k));
+        continue;
+      }
       std::vector<uint64_t> key;
       for (int s = 0; s < n; ++s) {
         std::vector<uint64_t> ks = iter_key(b->ctx[s], opts[s], it);
@@ -1414,6 +1460,8 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
       for (auto& e : b->cache)
         if (e.key == key) { hit = &e; break; }
       if (!hit) {
+        std::vector<HostIterState> before;
+        for (int s = 0; s < n; ++s) before.push_back(save_state(b->ctx[s]));
         PTY_HIP(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
         int r = PTY_OK;
         hipError_t he = hipEventRecord(b->fork, b->origin);
@@ -1426,16 +1474,30 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
         }
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(b->origin, &graph);
-        if (r != PTY_OK || he != hipSuccess || ec != hipSuccess) {
+        if (r != PTY_OK) {
           if (graph) (void)hipGraphDestroy(graph);
-          if (r != PTY_OK) return r;
-          return fail(PTY_EHIP, std::string("batch capture: ") +
-                                    hipGetErrorString(he != hipSuccess ? he : ec));
+          return r;
         }
         hipGraphExec_t exec = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess) return fail(PTY_EHIP, std::string("batch instantiate: ") + hipGetErrorString(ei));
+        hipError_t ei = he != hipSuccess ? he : ec;
+        if (ei == hipSuccess) ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess || capture_fail_forced()) {
+          // nothing of the captured iteration ran: undo its host-side effects,
+          // drop the graphs and run this and the remaining iterations eagerly
+          if (exec) (void)hipGraphExecDestroy(exec);
+          PTY_HIP(hipStreamSynchronize(b->origin));
+          for (int s = 0; s < n; ++s) {
+            load_state(b->ctx[s], before[s]);
+            b->ctx[s]->use_slot = false;
+          }
+          b->drop();
+          b->graphs = false;
+          b->fell_back = true;
+          for (int s = 0; s < n; ++s)
+            PTY_TRY(run_one(b->ctx[s], opts[s], it, b->ctx[s]->metrics + 3 * (size_t)k));
+          continue;
+        }
         pty_batch::Entry e{key, exec, {}};
         for (int s = 0; s < n; ++s) e.st.push_back(save_state(b->ctx[s]));
         b->cache.push_back(std::move(e));

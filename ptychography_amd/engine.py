@@ -155,6 +155,12 @@ class Engine:
         """Replay each kind of iteration as a captured hipGraph."""
         call("pty_set_graphs", self._ctx, int(bool(enable)))
 
+    def graph_status(self):
+        """(graphs on, a capture failed and the run fell back to eager launches)."""
+        en, fb = ctypes.c_int32(), ctypes.c_int32()
+        call("pty_get_graphs", self._ctx, ctypes.byref(en), ctypes.byref(fb))
+        return bool(en.value), bool(fb.value)
+
     def profile(self, enable=True):
         call("pty_profile", self._ctx, int(bool(enable)))
 

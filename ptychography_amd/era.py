@@ -188,13 +188,25 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
     return Job(algorithm=algorithm, eng=eng, O_t=O_t, P_t=P_t, method=method, update=update,
                r_dtype=r_dtype, c_dtype=c_dtype, bg_init=bg_init, background=background,
                OP_iters=OP_iters, until=until, probe_centering=probe_centering, alpha=alpha,
-               init_info=init_info, iters=iters)
+               init_info=init_info, iters=iters, graphs=bool(graphs))
 
 
-def finish(job, metrics, full_output=True):
+def check_metrics(metrics):
+    """Raise if an iteration's eMod is not finite (NaN / inf in the data or the
+    state): the reference would carry the NaNs on silently (era.py:561-571)."""
+    e = np.asarray(metrics)[:, 0] if len(metrics) else np.zeros(0)
+    bad = np.nonzero(~np.isfinite(e))[0]
+    if len(bad):
+        raise FloatingPointError(f"eMod is not finite from iteration {int(bad[0])} on: the "
+                                 "intensities or the state hold NaN / inf")
+
+
+def finish(job, metrics, full_output=True, check_finite=True):
     """Returns of ERA / DM (era.py:315-332) from a run job; closes its engine."""
     eng = job.eng
     try:
+        if check_finite:
+            check_metrics(metrics)
         c_dtype, r_dtype = job.c_dtype, job.r_dtype
         O_out = job.O_t.cpu().numpy().astype(c_dtype)
         P_out = job.P_t.cpu().numpy().astype(c_dtype)
@@ -205,6 +217,9 @@ def finish(job, metrics, full_output=True):
         info = {}
         if job.init_info is not None:
             info['init'] = job.init_info
+        if job.graphs:
+            en, fb = eng.graph_status()
+            info['graphs'] = {'enabled': en, 'fell_back': fb}
         info['I'] = eng.intensity().cpu().numpy().astype(r_dtype)
         info['eMod'] = [float(x) for x in metrics[:, 0]]
         info['eCon'] = [float(x) for x in metrics[:, 1]]
@@ -234,7 +249,7 @@ def finish(job, metrics, full_output=True):
 def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
           Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
           full_output=True, device=None, callback=None, rng=None, init=None, init_net=None,
-          graphs=False):
+          graphs=False, check_finite=True):
     """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns.
 
     ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
@@ -262,12 +277,13 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
     except BaseException:
         job.eng.close()
         raise
-    return finish(job, metrics, full_output)
+    return finish(job, metrics, full_output, check_finite)
 
 
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmod_probe=False,
         probe_centering=False, hardware='cpu', alpha=1.0e-10, dtype=None, full_output=True,
-        device=None, callback=None, rng=None, init=None, init_net=None, graphs=False):
+        device=None, callback=None, rng=None, init=None, init_net=None, graphs=False,
+        check_finite=True):
     """Error-reduction algorithm (ptychography/era.py:13).  ``hardware`` is accepted and
     ignored, as in the reference; the work always runs on the GPU ``device``.
 
@@ -276,10 +292,13 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmo
     ``.random(shape)``) in place of ``np.random`` for the default P / background,
     ``init='learned'`` with ``init_net`` for the gated learned starting object,
     ``graphs=True`` to replay each kind of iteration as a captured hipGraph
-    (bit-identical to launching the kernels one by one).
+    (bit-identical to launching the kernels one by one; a failed capture falls
+    back to eager launches, reported in ``info['graphs']``),
+    ``check_finite`` (default True) to raise ``FloatingPointError`` when eMod
+    becomes NaN / inf instead of returning NaNs.
     """
     return solve(ALGO_ERA, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
                  background=background, method=method, Pmod_probe=Pmod_probe,
                  probe_centering=probe_centering, alpha=alpha, dtype=dtype,
                  full_output=full_output, device=device, callback=callback, rng=rng,
-                 init=init, init_net=init_net, graphs=graphs)
+                 init=init, init_net=init_net, graphs=graphs, check_finite=check_finite)

@@ -66,3 +66,33 @@ def test_dm_centering_graphs_bit_identical(gpu_device):
     b = _run(s, gpu_device, 1, 3, True, P0, **kw)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_failed_capture_falls_back_to_eager(gpu_device, monkeypatch):
+    """A capture that fails (forced from the host) leaves nothing half-run:
+    the run finishes eagerly with the same results and reports the fallback."""
+    import ptychography_amd as pty
+    s = small_scan(frame=64, grid=(4, 4), step=16)
+    O0 = broadband_start(s.O.shape)
+    O_e, P_e, i_e = pty.ERA(s.I, s.R.copy(), s.P, O0, 6, method=3, device=gpu_device)
+    monkeypatch.setenv("PTY_FORCE_CAPTURE_FAILURE", "1")
+    O_g, P_g, i_g = pty.ERA(s.I, s.R.copy(), s.P, O0, 6, method=3, device=gpu_device, graphs=True)
+    res = pty.ERA_many([dict(I=s.I, R=s.R.copy(), P=s.P, O=O0, method=3)], 6, graphs=True,
+                       device=gpu_device)
+    monkeypatch.delenv("PTY_FORCE_CAPTURE_FAILURE")
+    assert i_g["graphs"] == {"enabled": False, "fell_back": True}
+    assert np.array_equal(O_g, O_e) and np.array_equal(P_g, P_e) and i_g["eMod"] == i_e["eMod"]
+    O_b, P_b, i_b = res[0]
+    assert i_b["graphs"] == {"enabled": False, "fell_back": True}
+    assert np.array_equal(O_b, O_e) and i_b["eMod"] == i_e["eMod"]
+
+
+def test_non_finite_data_raise(gpu_device):
+    import ptychography_amd as pty
+    s = small_scan(frame=32, grid=(3, 3), step=8)
+    I = s.I.copy()
+    I[2, 5, 7] = np.nan
+    with pytest.raises(FloatingPointError):
+        pty.ERA(I, s.R.copy(), s.P, None, 3, method=1, device=gpu_device)
+    O, info = pty.ERA(I, s.R.copy(), s.P, None, 3, method=1, device=gpu_device, check_finite=False)
+    assert not np.isfinite(info["eMod"][0])
