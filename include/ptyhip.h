@@ -130,14 +130,21 @@ int pty_get_graphs(pty_ctx* ctx, int32_t* enabled, int32_t* fell_back);
  * column mode) for the column FFTs; cols_wg (256 or 512) threads per column
  * work-group (512 = 32-column blocks, 256^2 frames only). */
 int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg);
+/* ERA's staged modulus step over blocks of `frames` frames (0 = the whole
+ * stack at once), so the frame spectra stay in the Infinity Cache between its
+ * row / column / inverse-row kernels; default 0 (env PTY_CHUNK_FRAMES).
+ * Frames per block are rounded up to at least N/16 and to whole column
+ * chunks. */
+int pty_set_chunk(pty_ctx* ctx, int32_t frames);
 typedef struct pty_tune_result {
-  int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult, cols_wg;  /* chosen */
+  int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult, cols_wg, chunk_frames;  /* chosen */
   int32_t candidates;                                               /* configurations timed */
   float best_ms, default_ms;                                        /* per iteration */
 } pty_tune_result;
 /* Times `trial_iters` iterations of opts for every launch configuration
  * (staged: rows_mult {2,3,4} x cols_mult {default,2,4}, and at 256^2 512-thread
- * column work-groups x cols_mult {default,1,3}; fused per-frame path
+ * column work-groups x cols_mult {default,1,3}; ERA frame blocks of 0 and
+ * 1/4..2x the default; fused per-frame path
  * where the frame shape allows it: 512 / 1024 threads x CUs / 2*CUs slots),
  * keeps the fastest and restores O, P and the background; the next pty_run
  * must pass begin = 1. */

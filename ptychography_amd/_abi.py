@@ -31,6 +31,7 @@ class PtyTuneResult(ctypes.Structure):
     _fields_ = [("fused", ctypes.c_int32), ("frame_threads", ctypes.c_int32),
                 ("frame_slots", ctypes.c_int32), ("rows_mult", ctypes.c_int32),
                 ("cols_mult", ctypes.c_int32), ("cols_wg", ctypes.c_int32),
+                ("chunk_frames", ctypes.c_int32),
                 ("candidates", ctypes.c_int32),
                 ("best_ms", ctypes.c_float), ("default_ms", ctypes.c_float)]
 
@@ -67,6 +68,7 @@ SIGNATURES = {
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_back_projection": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _f32]),
     "pty_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
+    "pty_set_chunk": (ctypes.c_int, [_vp, _i32]),
     "pty_tune": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
     "pty_batch_create": (ctypes.c_int, [_vp, _i32, _vp]),
     "pty_batch_destroy": (ctypes.c_int, [_vp]),

@@ -109,6 +109,7 @@ struct pty_ctx {
   int rows_mult = 3;         // persistent row kernels: up to rows_mult * CUs work-groups
   int cols_mult = 0;         // persistent column kernels: cols_mult * CUs (0 = per-mode default)
   int cols_wg = 256;         // threads per column work-group (512: wide blocks, 256^2 frames)
+  int chunk_frames = 0;      // ERA modulus step over blocks of this many frames (0 = all at once)
   int cus = 256;             // compute units of the device (persistent grids)
   // probe-reduction launch shape
   int probe_chunk = 1, probe_chunks = 0;
@@ -527,6 +528,31 @@ int check_bound(pty_ctx* c, bool need_bg) {
 // --------------------------------------------------------------------------
 // one ERA iteration (era.py:152-215 and :223-291)
 // --------------------------------------------------------------------------
+// The staged modulus step (row FFTs -> column FFTs + modulus -> inverse row
+// FFTs) can run over blocks of frames small enough that the frame spectra stay
+// in the 256 MB Infinity Cache between its three kernels instead of going
+// through HBM.  At most MAX_CHUNKS blocks (the column kernels' error partials
+// are appended per block).
+constexpr int MAX_CHUNKS = 16;
+
+int chunk_frames(const pty_ctx* c) {
+  if (c->chunk_frames <= 0 || c->chunk_frames >= c->N) return c->N;
+  int G = std::max(c->chunk_frames, (c->N + MAX_CHUNKS - 1) / MAX_CHUNKS);
+  G = (G + c->col_chunk - 1) / c->col_chunk * c->col_chunk;   // keeps background sums aligned
+  return std::min(G, c->N);
+}
+
+// frames [n0, n0 + n) seen as a scan of their own by the row / column kernels
+struct ChunkView {
+  pty_ctx* c;
+  Geom saved;
+  ChunkView(pty_ctx* c_, int n0, int n) : c(c_), saved(c_->g) {
+    c->g.off = saved.off + n0;
+    c->g.N = n;
+  }
+  ~ChunkView() { c->g = saved; }
+};
+
 int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   const bool bgm = o.method >= 4;
   const int upd = (o.method - 1) % 3;  // 0 = O, 1 = P, 2 = OP
@@ -556,23 +582,33 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     f.part2 = esup_id ? c->part_esa : nullptr;
     PTY_TRY(frame<FR_ERA1>(c, f, &ncol));
   } else {
-  PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
-  ColArgs a = col_args(c);
-  a.X = c->X;
-  a.amp = c->amp;
-  a.amp_stride = (long)MK;
-  a.alpha = o.alpha;
-  a.part = c->part_emod;
-  a.part2 = esup_id ? c->part_esa : nullptr;
-  if (bgm) {
-    a.bg = c->era_bg_shared ? c->bbar : c->bg;
-    a.bg_stride = c->era_bg_shared ? 0 : (long)MK;
-    a.partb = c->partb;
-    PTY_TRY(cols<COL_ERAB>(c, a, &ncol));
-  } else {
-    PTY_TRY(cols<COL_ERA1>(c, a, &ncol));
-  }
-  PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->X, nullptr, nullptr, 1.0f, nullptr, nrows));
+    (void)nrows;
+    const int G = chunk_frames(c);
+    for (int n0 = 0; n0 < c->N; n0 += G) {
+      const int nf = std::min(G, c->N - n0);
+      ChunkView view(c, n0, nf);
+      float2* Xc = c->X + (size_t)n0 * MK;
+      PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, Xc, nf * c->M));
+      ColArgs a = col_args(c);
+      a.X = Xc;
+      a.nframes = nf;
+      a.amp = c->amp + (size_t)n0 * MK;
+      a.amp_stride = (long)MK;
+      a.alpha = o.alpha;
+      a.part = c->part_emod + ncol;
+      a.part2 = esup_id ? c->part_esa + ncol : nullptr;
+      int nc = 0;
+      if (bgm) {
+        a.bg = c->era_bg_shared ? c->bbar : c->bg + (size_t)n0 * MK;
+        a.bg_stride = c->era_bg_shared ? 0 : (long)MK;
+        a.partb = c->partb + (size_t)(n0 / c->col_chunk) * MK;
+        PTY_TRY(cols<COL_ERAB>(c, a, &nc));
+      } else {
+        PTY_TRY(cols<COL_ERA1>(c, a, &nc));
+      }
+      ncol += nc;
+      PTY_TRY(rows_inv<DST_STORE>(c, Xc, Xc, nullptr, nullptr, 1.0f, nullptr, nf * c->M));
+    }
   }
 
   // consistency projection
@@ -807,6 +843,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
           (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
           (uint64_t)c->rows_mult, (uint64_t)c->cols_mult, (uint64_t)c->cols_wg,
+          (uint64_t)c->chunk_frames,
           (uint64_t)(uintptr_t)c->O, (uint64_t)(uintptr_t)c->P, (uint64_t)(uintptr_t)c->F,
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
@@ -1068,6 +1105,13 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->cus = cus;
     const char* e_cw = getenv("PTY_COLS_WG");
     if (e_cw && atoi(e_cw) == 512) c->cols_wg = 512;
+    // The modulus step runs over the whole stack at once by default: blocks
+    // sized for the 256 MB Infinity Cache (128..512 frames of 256^2) measured
+    // 8..18% slower per ERA iteration at 4096 x 256^2, the persistent row and
+    // column grids being under-filled by one block.  pty_tune tries them.
+    c->chunk_frames = 0;
+    const char* e_ch = getenv("PTY_CHUNK_FRAMES");
+    if (e_ch) c->chunk_frames = std::max(0, atoi(e_ch));
     // measured: at 256^2 the staged kernels are faster than the fused one
     c->use_frame = c->frame_ok && M == 128 && !(e_on && atoi(e_on) == 0);
     if (e_on && atoi(e_on) == 1) c->use_frame = c->frame_ok;
@@ -1099,8 +1143,8 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   if ((r = dev_alloc(&c->pg, (size_t)c->probe_chunks * MK)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->cm, 2)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->scal, 8)) != PTY_OK) return cleanup(r);
-  if ((r = dev_alloc(&c->part_emod, c->cap_col)) != PTY_OK) return cleanup(r);
-  if ((r = dev_alloc(&c->part_esa, c->cap_col)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_emod, (size_t)c->cap_col * MAX_CHUNKS)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_esa, (size_t)c->cap_col * MAX_CHUNKS)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->part_econO, 2 * (size_t)c->cap_econO)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->part_econP, 2 * (size_t)c->cap_econP)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->part_esup, c->cap_esup)) != PTY_OK) return cleanup(r);
@@ -1255,6 +1299,15 @@ int pty_get_graphs(pty_ctx* c, int32_t* enabled, int32_t* fell_back) {
   return PTY_OK;
 }
 
+int pty_set_chunk(pty_ctx* c, int32_t frames) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (frames < 0) return fail(PTY_EINVAL, "frames must be >= 0");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->chunk_frames = frames;
+  drop_graphs(c);
+  return PTY_OK;
+}
+
 int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 4)
@@ -1302,18 +1355,26 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     if (snap.B) PTY_TRY(copy_d2d(c, c->bg, snap.B, NMK * sizeof(float)));
     return PTY_OK;
   };
-  struct Cfg { int fused, nt, slots, rows, cols, cwg; };
+  struct Cfg { int fused, nt, slots, rows, cols, cwg, chunk; };
   std::vector<Cfg> cands;
-  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg});
+  const int ch = c->chunk_frames;
+  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch});
   for (int rows : {2, 3, 4})
-    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256});
+    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256, ch});
   if (c->M == 256)
-    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512});
+    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512, ch});
+  {
+    // frame blocks for the modulus step: whole stack, and 1/4 .. 2x the default block
+    const size_t per = (size_t)c->M * c->K * 12, budget = (size_t)192 << 20;
+    const int base = std::max<int>(1, (int)(budget / per));
+    for (int chunk : {0, base / 4, base / 2, base, 2 * base})
+      if (chunk < c->N && chunk != ch) cands.push_back({0, c->frame_nt, c->frame_slots, 3, 0, 256, chunk});
+  }
   if (c->frame_ok)
     for (int nt : {512, 1024})
       for (int sl : {c->cus, 2 * c->cus})
         if (sl <= c->cap_col && sl <= c->cap_esup)
-          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg});
+          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg, ch});
   auto apply = [&](const Cfg& g) {
     c->use_frame = g.fused != 0;
     c->frame_nt = g.nt;
@@ -1321,6 +1382,7 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     c->rows_mult = g.rows;
     c->cols_mult = g.cols;
     c->cols_wg = g.cwg;
+    c->chunk_frames = g.chunk;
     drop_graphs(c);
   };
   double best = 1e300, first = 0.0;
@@ -1345,8 +1407,8 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
   c->H_valid = c->G_valid = false;
   c->have_metric_state = false;
   const Cfg& g = cands[besti];
-  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, (int32_t)cands.size(),
-                         (float)best, (float)first};
+  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, g.chunk,
+                         (int32_t)cands.size(), (float)best, (float)first};
   return PTY_OK;
 }
 

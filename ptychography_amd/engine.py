@@ -138,6 +138,10 @@ class Engine:
         threads per column work-group (256, or 512 for 256^2 frames)."""
         call("pty_set_launch", self._ctx, int(rows_mult), int(cols_mult), int(cols_wg))
 
+    def set_chunk(self, frames):
+        """ERA modulus step over blocks of `frames` frames (0 = all at once)."""
+        call("pty_set_chunk", self._ctx, int(frames))
+
     def tune(self, algorithm, method, trial_iters=3, OP_iters=(1, 1), Pmod_probe_until=0,
              probe_centering=False, alpha=1e-10):
         """Time every launch configuration on this scan and keep the fastest
