@@ -136,8 +136,17 @@ int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult, int32_t c
  * Frames per block are rounded up to at least N/16 and to whole column
  * chunks. */
 int pty_set_chunk(pty_ctx* ctx, int32_t frames);
+/* ERA object-only steps (methods 1 and 4) after the first: fuse the inverse
+ * row FFT of psi' into the object update (one work-group per object row, the
+ * same per-pixel summation order as the unfused update), so psi' never goes
+ * back to HBM.  Built for K in {64, 128, 192, 256} and V <= 8192; on by
+ * default where supported (env PTY_ROW_GATHER=0 turns it off).  enable != 0
+ * on an unsupported geometry leaves it off; pty_get_row_gather reports both. */
+int pty_set_row_gather(pty_ctx* ctx, int32_t enable);
+int pty_get_row_gather(pty_ctx* ctx, int32_t* supported, int32_t* enabled);
 typedef struct pty_tune_result {
   int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult, cols_wg, chunk_frames;  /* chosen */
+  int32_t row_gather;                                               /* chosen */
   int32_t candidates;                                               /* configurations timed */
   float best_ms, default_ms;                                        /* per iteration */
 } pty_tune_result;
@@ -171,7 +180,7 @@ enum {
   PTY_STAGE_ROWS_FWD = 0,  /* frame formation + forward row FFT */
   PTY_STAGE_COLS = 1,      /* column FFT + modulus step + inverse column FFT */
   PTY_STAGE_ROWS_INV = 2,  /* inverse row FFT (+ DM update) */
-  PTY_STAGE_GATHER = 3,    /* object update (overlap gather) */
+  PTY_STAGE_GATHER = 3,    /* object update (overlap gather; fused: with the inverse row FFT) */
   PTY_STAGE_PROBE = 4,     /* probe update (frame reduction) */
   PTY_STAGE_REDUCE = 5,    /* metric / background reductions */
   PTY_STAGE_MISC = 6,      /* copies, centring, far-field probe constraint */

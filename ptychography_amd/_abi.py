@@ -32,6 +32,7 @@ class PtyTuneResult(ctypes.Structure):
                 ("frame_slots", ctypes.c_int32), ("rows_mult", ctypes.c_int32),
                 ("cols_mult", ctypes.c_int32), ("cols_wg", ctypes.c_int32),
                 ("chunk_frames", ctypes.c_int32),
+                ("row_gather", ctypes.c_int32),
                 ("candidates", ctypes.c_int32),
                 ("best_ms", ctypes.c_float), ("default_ms", ctypes.c_float)]
 
@@ -69,6 +70,8 @@ SIGNATURES = {
     "pty_back_projection": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _f32]),
     "pty_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "pty_set_chunk": (ctypes.c_int, [_vp, _i32]),
+    "pty_set_row_gather": (ctypes.c_int, [_vp, _i32]),
+    "pty_get_row_gather": (ctypes.c_int, [_vp, _vp, _vp]),
     "pty_tune": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
     "pty_batch_create": (ctypes.c_int, [_vp, _i32, _vp]),
     "pty_batch_destroy": (ctypes.c_int, [_vp]),

@@ -11,7 +11,9 @@
 //   rows forward : frame formation  psi = O_win * P  (or 2 O_win P - f, or f)
 //   columns      : forward column FFT -> modulus / error / background update
 //                  -> inverse column FFT (with FP64 error partials)
-//   rows inverse : inverse row FFT, optionally the DM update f += ...
+//   rows inverse : inverse row FFT, optionally the DM update f += ...; for
+//                  ERA object-only steps fused with the object update
+//                  (k_rows_gather)
 // Reductions are deterministic: every work-group writes its own partial, and
 // partials are summed in a fixed order by a single work-group.
 #pragma once
@@ -540,6 +542,154 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
       const int nb = a.ntiles;
       if (a.part_econ) { a.part_econ[tile] = r[0]; a.part_econ[nb + tile] = r[1]; }
       if (a.part_esup) { a.part_esup[tile] = r[2]; a.part_esup[nb + tile] = r[3]; }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inverse row FFT fused with the object update (ERA object-only steps:
+// pmod_1's inverse transform, era.py:560-572, followed by psup_O, :336-369)
+// ---------------------------------------------------------------------------
+struct RowGatherArgs {
+  const float2* Y;         // row spectra after the column stage [N][M][K]
+  const float2* P;         // probe [M][K]
+  const float2* Oold;      // previous object, for the fused eCon (null = skip)
+  float2* Oout;            // new object [U][V] (may alias Oold)
+  const double* H;         // heatmap sum |P|^2 [U][V] (valid)
+  float alpha;
+  const int* __restrict__ row_ptr;    // CSR over object rows
+  const int2* __restrict__ row_ent;   // per object row, ascending n: (n*M + i, x_n << 16 | i)
+  double* part_econ;       // [2][U]  sum |Oold - O|^2, sum |O|^2
+  double* part_esup;       // [2][U]  sum Re(conj(O) num), sum |O|^2 H
+};
+
+// Largest object width the fused kernel's LDS row accumulator takes (and
+// that the packed column offset of a CSR entry holds).
+constexpr int ROWG_VMAX = 8192;
+static_assert(ROWG_VMAX < (1 << 15), "x_n << 16 must stay a positive int");
+
+// One work-group per object row u.  The row's frames (ascending n) are taken
+// LPW at a time, one per line: frame row i = u - y_n is inverse-transformed in
+// registers (the column stage already applied the 1/(M K) of the inverse),
+// multiplied by conj(P[i, :]) and parked in the line's own LDS exchange area;
+// then each object column of the row adds the parked values of the lines
+// covering it, in line order.  Every pixel so sums its frames in
+// ascending order, as k_gather_O does, but psi' is never written to or read
+// back from HBM.  `row` (dynamic LDS, V float2) accumulates the row.
+//
+// Global loads are issued in the order their results are needed (CSR entries
+// two batches ahead, then this batch's probe row, then the next batch's
+// spectra), so no wait on the in-order load counter also waits for the
+// prefetch of the next batch.
+template <int L>
+__global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
+                                                    const float2* __restrict__ tw) {
+  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
+  constexpr int W = LineLds<L>::words;
+  constexpr int OFF_NONE = -(1 << 29);  // column offset of a spare line: matches no column
+  static_assert(W >= L, "a line's exchange area must hold the parked row");
+  static_assert(64 % T == 0, "a line must lie within one wave");
+  __shared__ float2 lds[LPW * W + 1];  // + one zero word read by uncovered slots
+  __shared__ float2 s_tw[L];
+  __shared__ int s_oy[LPW];
+  __shared__ double red[16];
+  extern __shared__ float2 row[];
+  stage_twiddles<L>(s_tw, tw);
+  const int tid = threadIdx.x, line = tid / T, t = tid % T;
+  const int u = blockIdx.x, M = g.M, V = g.V;
+  for (int v = tid; v < V; v += WG) row[v] = make_float2(0.f, 0.f);
+  if (tid == 0) lds[LPW * W] = make_float2(0.f, 0.f);
+  const int b0 = a.row_ptr[u], cnt = a.row_ptr[u + 1] - b0;
+  const int nbatch = (cnt + LPW - 1) / LPW;
+  // spare lines of the last batch re-read a real entry and are masked by OFF_NONE
+  auto entry = [&](int bt) { return a.row_ent[b0 + min(bt * LPW + line, cnt - 1)]; };
+  float2 nx[E];
+  auto load = [&](int2 m) {
+    const size_t rbase = (size_t)m.x * L;
+#pragma unroll
+    for (int e = 0; e < E; ++e) nx[e] = a.Y[rbase + idxA<R2, R1, T>(e, t)];
+  };
+  int2 m_cur = make_int2(0, 0), m_next = make_int2(0, 0);  // .y = x_n << 16 | i
+  if (nbatch > 0) {
+    m_cur = entry(0);
+    m_next = entry(1);
+    load(m_cur);
+  }
+  for (int bt = 0; bt < nbatch; ++bt) {
+    const int i = m_cur.y & 0xffff;
+    const float2* Prow = a.P + (size_t)i * L;
+    float2 pv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) pv[e] = Prow[idxB<R2, R1, T>(e, t)];
+    const int2 m_after = entry(bt + 2);
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = nx[e];
+    if (bt + 1 < nbatch) load(m_next);
+    line_fft<L, R2, R1, T, true>(v, lds + line * W, s_tw, t);
+    // park conj(P) * psi' in this line's area: every value depends on the
+    // pass-2 reads of the same wave, so no barrier is needed before it
+    float2* park = lds + line * W;
+#pragma unroll
+    for (int e = 0; e < E; ++e) park[idxB<R2, R1, T>(e, t)] = cmulc(v[e], pv[e]);
+    if (t == 0) s_oy[line] = (bt * LPW + line < cnt) ? (m_cur.y >> 16) : OFF_NONE;
+    lds_barrier();
+    int oy[LPW];
+    int lo = V, hi = 0;
+#pragma unroll
+    for (int l = 0; l < LPW; ++l) {
+      oy[l] = __builtin_amdgcn_readfirstlane(s_oy[l]);
+      if (oy[l] != OFF_NONE) {
+        lo = min(lo, oy[l]);
+        hi = max(hi, oy[l] + L);
+      }
+    }
+    // branch-free: every line reads its parked value or the zero word, so the
+    // LPW reads are all in flight before the in-order adds (skipping lines
+    // that miss a wave's columns with uniform branches measured slower)
+    for (int base = lo; base < hi; base += WG) {
+      const int vv = base + tid;
+      float2 s = (vv < hi) ? row[vv] : make_float2(0.f, 0.f);
+      float2 x[LPW];
+#pragma unroll
+      for (int l = 0; l < LPW; ++l) {
+        const int c = vv - oy[l];
+        x[l] = lds[(unsigned)c < (unsigned)L ? l * W + c : LPW * W];
+      }
+#pragma unroll
+      for (int l = 0; l < LPW; ++l) s = s + x[l];
+      if (vv < hi) row[vv] = s;
+    }
+    lds_barrier();
+    m_cur = m_next;
+    m_next = m_after;
+  }
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int v = tid; v < V; v += WG) {
+    const size_t pix = (size_t)u * V + v;
+    const float2 acc = row[v];
+    const double h = a.H[pix];
+    const double den = h + (double)a.alpha;
+    const double ox = (double)acc.x / den, oyv = (double)acc.y / den;
+    const float2 on = make_float2((float)ox, (float)oyv);
+    if (a.Oold) {
+      const float2 od = a.Oold[pix];
+      const double dx = (double)od.x - (double)on.x, dy = (double)od.y - (double)on.y;
+      r[0] += dx * dx + dy * dy;
+      r[1] += (double)on.x * on.x + (double)on.y * on.y;
+    }
+    if (a.part_esup) {
+      r[2] += (double)on.x * acc.x + (double)on.y * acc.y;
+      r[3] += ((double)on.x * on.x + (double)on.y * on.y) * h;
+    }
+    a.Oout[pix] = on;
+  }
+  if (a.part_econ || a.part_esup) {
+    block_sum<4>(r, red);
+    if (tid == 0) {
+      const int nb = g.U;
+      if (a.part_econ) { a.part_econ[u] = r[0]; a.part_econ[nb + u] = r[1]; }
+      if (a.part_esup) { a.part_esup[u] = r[2]; a.part_esup[nb + u] = r[3]; }
     }
   }
 }

@@ -102,6 +102,12 @@ struct pty_ctx {
   int tiles_x = 0, tiles_y = 0, ntiles = 0;
   int* tile_ptr = nullptr;
   int* tile_frames = nullptr;
+  // object rows -> ascending frame lists (CSR), for the fused inverse-row /
+  // object-update kernel of ERA object-only steps
+  int* row_ptr = nullptr;
+  int2* row_ent = nullptr;   // (frame row n*M + i, column offset x_n << 16 | i)
+  bool rowg_ok = false;      // geometry supports the fused kernel
+  bool rowg = false;         // currently selected
   float2* twM = nullptr;
   float2* twK = nullptr;
   // column-stage launch shape
@@ -153,7 +159,8 @@ struct pty_ctx {
   double* part_econO = nullptr;  // 2 * max(ntiles, ECON_BLOCKS)
   double* part_econP = nullptr;  // 2 * max(MK/256, ECON_BLOCKS)
   double* part_esup = nullptr;   // max(ESUP_BLOCKS, row blocks)
-  double* part_esxh = nullptr;   // 2 * ntiles
+  double* part_esxh = nullptr;   // 2 * max(ntiles, U)
+  int cap_gather = 0;            // max(ntiles, U): object-update work-groups
   int cap_col = 0, cap_econO = 0, cap_econP = 0, cap_esup = 0;
 
   // optional per-stage timing with HIP events on the context stream
@@ -403,6 +410,73 @@ int gather(pty_ctx* c, const float2* S, const float2* P, const float2* Oold, flo
   return PTY_OK;
 }
 
+// frame sides the fused inverse-row / object-update kernel is built for
+#define PTY_ROWG_SIZES(X) X(64) X(128) X(192) X(256)
+
+template <int L>
+size_t rowg_static_lds() {
+  constexpr size_t lpw = WG / Plan<L>::T;
+  return sizeof(float2) * (lpw * LineLds<L>::words + L) + 16 * sizeof(double) + lpw * sizeof(int);
+}
+
+bool rowg_supported(int K, int V) {
+  switch (K) {
+#define C(Lv) case Lv: return V <= ROWG_VMAX && rowg_static_lds<Lv>() + (size_t)V * 8 <= (160u << 10);
+    PTY_ROWG_SIZES(C)
+#undef C
+    default: return false;
+  }
+}
+
+// Raise the dynamic-LDS limit of the K-sized instantiation to V float2.
+int rowg_attribute(int K, int V) {
+  const int bytes = V * (int)sizeof(float2);
+  hipError_t e = hipErrorInvalidValue;
+  switch (K) {
+#define C(Lv)                                                                                 \
+  case Lv:                                                                                    \
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_gather<Lv>),                 \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);                \
+    break;
+    PTY_ROWG_SIZES(C)
+#undef C
+    default: break;
+  }
+  if (e != hipSuccess) return fail(PTY_EHIP, std::string("k_rows_gather LDS attribute: ") + hipGetErrorString(e));
+  return PTY_OK;
+}
+
+// Inverse row FFT of Y fused with the object update O <- sum conj(P) psi' / (H + alpha)
+// (H valid); one work-group per object row, partials per row.
+int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, int* nparts) {
+  StageTimer tm(c, PTY_STAGE_GATHER);
+  RowGatherArgs a{};
+  a.Y = Y;
+  a.P = c->P;
+  a.Oold = econ ? c->O : nullptr;
+  a.Oout = c->O;
+  a.H = c->H;
+  a.alpha = alpha;
+  a.row_ptr = c->row_ptr;
+  a.row_ent = c->row_ent;
+  a.part_econ = econ ? c->part_econO : nullptr;
+  a.part_esup = esup ? c->part_esxh : nullptr;
+  const size_t dyn = (size_t)c->V * sizeof(float2);
+  switch (c->K) {
+#define C(Lv)                                                                                \
+  case Lv:                                                                                   \
+    hipLaunchKernelGGL(k_rows_gather<Lv>, dim3(c->U), dim3(WG), dyn, c->stream, c->g, a,     \
+                       (const float2*)c->twK);                                               \
+    break;
+    PTY_ROWG_SIZES(C)
+#undef C
+    default: return fail(PTY_EUNSUP, "fused row gather: unsupported K");
+  }
+  PTY_LAUNCHED("k_rows_gather");
+  if (nparts) *nparts = c->U;
+  return PTY_OK;
+}
+
 int probe_update(pty_ctx* c, const float2* S, const float2* O, const float2* Pold, float2* Pout,
                  double* G, int fresh, int storeG, float alpha, bool econ) {
   StageTimer tm(c, PTY_STAGE_PROBE);
@@ -563,6 +637,9 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   const bool fusedO = upd == 0;
   const bool fusedP = upd == 1 && !pmodp;
   const bool esup_id = upd == 0 && !pmodp;
+  const bool staged = !(c->use_frame && !bgm);
+  // object-only step with a valid heatmap: inverse row FFT fused into the update
+  const bool rowg = upd == 0 && staged && c->rowg && c->H_valid;
 
   if (!fusedO && !fusedP) {
     StageTimer tm(c, PTY_STAGE_MISC);
@@ -572,7 +649,7 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
 
   // modulus projection: psi' = IFFT(Pmod(FFT(O_n P)))
   int ncol = 0;
-  if (c->use_frame && !bgm) {
+  if (!staged) {
     FrameArgs f = frame_args(c);
     f.O = c->O;
     f.P = c->P;
@@ -607,12 +684,15 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
         PTY_TRY(cols<COL_ERA1>(c, a, &nc));
       }
       ncol += nc;
-      PTY_TRY(rows_inv<DST_STORE>(c, Xc, Xc, nullptr, nullptr, 1.0f, nullptr, nf * c->M));
+      if (!rowg) PTY_TRY(rows_inv<DST_STORE>(c, Xc, Xc, nullptr, nullptr, 1.0f, nullptr, nf * c->M));
     }
   }
 
   // consistency projection
-  if (upd == 0) {
+  int ngather = c->ntiles;
+  if (rowg) {
+    PTY_TRY(rows_gather(c, c->X, o.alpha, true, esup_id, &ngather));
+  } else if (upd == 0) {
     PTY_TRY(gather(c, c->X, c->P, c->O, c->O, c->H, !c->H_valid, 1, o.alpha, true, esup_id));
     c->H_valid = true;
   } else if (upd == 1) {
@@ -646,8 +726,8 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   rg.n[S_EMOD] = ncol;
   if (fusedO) {
     rg.p[S_ECON_ON] = c->part_econO;
-    rg.p[S_ECON_OD] = c->part_econO + c->ntiles;
-    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = c->ntiles;
+    rg.p[S_ECON_OD] = c->part_econO + ngather;
+    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ngather;
   } else if (fusedP) {
     const int nb = (int)((MK + WG - 1) / WG);
     rg.p[S_ECON_PN] = c->part_econP;
@@ -673,8 +753,8 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     rg.p[S_ESUP_A] = c->part_esa;
     rg.n[S_ESUP_A] = ncol;
     rg.p[S_ESUP_X] = c->part_esxh;
-    rg.p[S_ESUP_H] = c->part_esxh + c->ntiles;
-    rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = c->ntiles;
+    rg.p[S_ESUP_H] = c->part_esxh + ngather;
+    rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = ngather;
   } else {
     esup_mode = ESUP_EXPLICIT;
     StageTimer tm(c, PTY_STAGE_REDUCE);
@@ -843,7 +923,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
           (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
           (uint64_t)c->rows_mult, (uint64_t)c->cols_mult, (uint64_t)c->cols_wg,
-          (uint64_t)c->chunk_frames,
+          (uint64_t)c->chunk_frames, (uint64_t)c->rowg,
           (uint64_t)(uintptr_t)c->O, (uint64_t)(uintptr_t)c->P, (uint64_t)(uintptr_t)c->F,
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
@@ -1079,6 +1159,21 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
       for (int tx = off[n].y / TILE_W; tx <= (off[n].y + K - 1) / TILE_W; ++tx)
         frames[fill[ty * c->tiles_x + tx]++] = n;
 
+  // object rows -> ascending frame lists, when the fused kernel applies
+  c->rowg_ok = rowg_supported(K, V) && (size_t)N * M <= ((size_t)1 << 26);
+  std::vector<int> rcnt;
+  std::vector<int2> rents;
+  if (c->rowg_ok) {
+    rcnt.assign(U + 1, 0);
+    for (int n = 0; n < N; ++n)
+      for (int i = 0; i < M; ++i) rcnt[off[n].x + i + 1]++;
+    for (int u = 0; u < U; ++u) rcnt[u + 1] += rcnt[u];
+    rents.resize((size_t)N * M);
+    std::vector<int> rfill(rcnt.begin(), rcnt.end() - 1);
+    for (int n = 0; n < N; ++n)
+      for (int i = 0; i < M; ++i) rents[rfill[off[n].x + i]++] = make_int2(n * M + i, off[n].y << 16 | i);
+  }
+
   // launch shapes
   c->col_blocks = (K + (WG / threads_per_line(M)) - 1) / (WG / threads_per_line(M));
   c->col_chunk = std::max(1, std::min(16, (int)((long)N * c->col_blocks / 2048)));
@@ -1117,7 +1212,8 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     if (e_on && atoi(e_on) == 1) c->use_frame = c->frame_ok;
   }
   c->cap_col = std::max(4 * c->cus, c->frame_slots);
-  c->cap_econO = std::max(c->ntiles, ECON_BLOCKS);
+  c->cap_gather = std::max(c->ntiles, U);
+  c->cap_econO = std::max(c->cap_gather, ECON_BLOCKS);
   c->cap_econP = std::max(pblocks, ECON_BLOCKS);
   c->cap_esup = std::max(std::max(ESUP_BLOCKS, row_blocks(c, N * M)), c->frame_slots);
 
@@ -1148,7 +1244,14 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   if ((r = dev_alloc(&c->part_econO, 2 * (size_t)c->cap_econO)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->part_econP, 2 * (size_t)c->cap_econP)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->part_esup, c->cap_esup)) != PTY_OK) return cleanup(r);
-  if ((r = dev_alloc(&c->part_esxh, 2 * (size_t)c->ntiles)) != PTY_OK) return cleanup(r);
+  if ((r = dev_alloc(&c->part_esxh, 2 * (size_t)c->cap_gather)) != PTY_OK) return cleanup(r);
+  if (c->rowg_ok) {
+    if ((r = dev_alloc(&c->row_ptr, (size_t)U + 1)) != PTY_OK) return cleanup(r);
+    if ((r = dev_alloc(&c->row_ent, rents.size())) != PTY_OK) return cleanup(r);
+    if ((r = rowg_attribute(K, V)) != PTY_OK) return cleanup(r);
+    const char* e_rg = getenv("PTY_ROW_GATHER");
+    c->rowg = !(e_rg && atoi(e_rg) == 0);
+  }
 
   std::vector<float2> tw;
   hipError_t e = hipSuccess;
@@ -1161,6 +1264,10 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     e = hipMemcpy(c->tile_ptr, cnt.data(), (c->ntiles + 1) * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(c->tile_frames, frames.data(), frames.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->rowg_ok)
+    e = hipMemcpy(c->row_ptr, rcnt.data(), (U + 1) * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->rowg_ok)
+    e = hipMemcpy(c->row_ent, rents.data(), rents.size() * sizeof(int2), hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(fail(PTY_EHIP, std::string("upload: ") + hipGetErrorString(e)));
   c->g.N = N; c->g.M = M; c->g.K = K; c->g.U = U; c->g.V = V;
   c->g.off = c->off;
@@ -1173,6 +1280,7 @@ int pty_destroy(pty_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   else (void)hipDeviceSynchronize();
   dev_free(c->off); dev_free(c->tile_ptr); dev_free(c->tile_frames);
+  dev_free(c->row_ptr); dev_free(c->row_ent);
   dev_free(c->twM); dev_free(c->twK);
   dev_free(c->X); dev_free(c->F); dev_free(c->Xp);
   dev_free(c->Obak); dev_free(c->Pbak); dev_free(c->Os); dev_free(c->Oroll); dev_free(c->Ps);
@@ -1308,6 +1416,21 @@ int pty_set_chunk(pty_ctx* c, int32_t frames) {
   return PTY_OK;
 }
 
+int pty_set_row_gather(pty_ctx* c, int32_t enable) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->rowg = enable && c->rowg_ok;
+  drop_graphs(c);
+  return PTY_OK;
+}
+
+int pty_get_row_gather(pty_ctx* c, int32_t* supported, int32_t* enabled) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (supported) *supported = c->rowg_ok ? 1 : 0;
+  if (enabled) *enabled = c->rowg ? 1 : 0;
+  return PTY_OK;
+}
+
 int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 4)
@@ -1355,26 +1478,28 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     if (snap.B) PTY_TRY(copy_d2d(c, c->bg, snap.B, NMK * sizeof(float)));
     return PTY_OK;
   };
-  struct Cfg { int fused, nt, slots, rows, cols, cwg, chunk; };
+  struct Cfg { int fused, nt, slots, rows, cols, cwg, chunk, rowg; };
   std::vector<Cfg> cands;
-  const int ch = c->chunk_frames;
-  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch});
+  const int ch = c->chunk_frames, rg = c->rowg;
+  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg});
+  if (c->rowg_ok)
+    cands.push_back({0, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, !rg});
   for (int rows : {2, 3, 4})
-    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256, ch});
+    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256, ch, rg});
   if (c->M == 256)
-    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512, ch});
+    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512, ch, rg});
   {
     // frame blocks for the modulus step: whole stack, and 1/4 .. 2x the default block
     const size_t per = (size_t)c->M * c->K * 12, budget = (size_t)192 << 20;
     const int base = std::max<int>(1, (int)(budget / per));
     for (int chunk : {0, base / 4, base / 2, base, 2 * base})
-      if (chunk < c->N && chunk != ch) cands.push_back({0, c->frame_nt, c->frame_slots, 3, 0, 256, chunk});
+      if (chunk < c->N && chunk != ch) cands.push_back({0, c->frame_nt, c->frame_slots, 3, 0, 256, chunk, rg});
   }
   if (c->frame_ok)
     for (int nt : {512, 1024})
       for (int sl : {c->cus, 2 * c->cus})
         if (sl <= c->cap_col && sl <= c->cap_esup)
-          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg, ch});
+          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg});
   auto apply = [&](const Cfg& g) {
     c->use_frame = g.fused != 0;
     c->frame_nt = g.nt;
@@ -1383,6 +1508,7 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     c->cols_mult = g.cols;
     c->cols_wg = g.cwg;
     c->chunk_frames = g.chunk;
+    c->rowg = g.rowg != 0;
     drop_graphs(c);
   };
   double best = 1e300, first = 0.0;
@@ -1407,7 +1533,7 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
   c->H_valid = c->G_valid = false;
   c->have_metric_state = false;
   const Cfg& g = cands[besti];
-  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, g.chunk,
+  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, g.chunk, g.rowg,
                          (int32_t)cands.size(), (float)best, (float)first};
   return PTY_OK;
 }

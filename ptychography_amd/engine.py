@@ -142,6 +142,16 @@ class Engine:
         """ERA modulus step over blocks of `frames` frames (0 = all at once)."""
         call("pty_set_chunk", self._ctx, int(frames))
 
+    def set_row_gather(self, enable=True):
+        """Fuse the inverse row FFT into ERA's object update (where supported)."""
+        call("pty_set_row_gather", self._ctx, int(bool(enable)))
+
+    def row_gather_status(self):
+        """(geometry supports the fused row gather, currently enabled)."""
+        sup, en = ctypes.c_int32(), ctypes.c_int32()
+        call("pty_get_row_gather", self._ctx, ctypes.byref(sup), ctypes.byref(en))
+        return bool(sup.value), bool(en.value)
+
     def tune(self, algorithm, method, trial_iters=3, OP_iters=(1, 1), Pmod_probe_until=0,
              probe_centering=False, alpha=1e-10):
         """Time every launch configuration on this scan and keep the fastest

@@ -1,0 +1,82 @@
+"""ERA object-only steps with the inverse row FFT fused into the object update
+(k_rows_gather) against the unfused kernels (k_rows_inv + k_gather_O).
+
+The fused kernel sums each object pixel's frames in the same ascending order
+as the unfused gather, but it parks conj(P) psi' in LDS before adding it, while
+the unfused gather lets the compiler contract the product into the running sum
+(two FMAs): the two round differently.  Bar: object iterates within rel L2
+1e-5 (the ERA parity tolerance against the oracle), eMod within 1e-5 relative,
+eCon / eSup (FP64 partials per object row instead of per tile) within 1e-4.
+Oracle parity of the fused path is covered by the 64^2 ERA tests in
+test_gpu_solvers.py, which take it by default.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from ptychography_amd.engine import Engine
+from tests.scans import broadband_start, rel_l2, small_scan
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(s, dev, method, rowg, iters=(1, 6, 5), graphs=False, bg=None, obj_pad=0):
+    N, M, K = s.I.shape
+    U, V = s.O.shape
+    eng = Engine(N, M, K, U + obj_pad, V + 2 * obj_pad, s.R.astype(np.int32), device=dev)
+    eng.set_path(False)
+    eng.set_row_gather(rowg)
+    sup, en = eng.row_gather_status()
+    assert sup and en == rowg
+    eng.set_graphs(graphs)
+    amp, I_norm = eng.prepare_amp(torch.from_numpy(s.I.astype(np.float32)), None)
+    eng.set_data(amp, None, I_norm)
+    O_t = torch.from_numpy(broadband_start((U + obj_pad, V + 2 * obj_pad)).astype(np.complex64)).to(dev)
+    P_t = torch.from_numpy(s.P.astype(np.complex64)).to(dev)
+    eng.set_state(O_t, P_t)
+    if bg is not None:
+        eng.set_background(torch.from_numpy(bg.astype(np.float32)), per_frame=False)
+    rows, it, Os = [], 0, []
+    for ci, n in enumerate(iters):
+        rows.append(eng.run(0, method, n, iter0=it, begin=(ci == 0)))
+        it += n
+        Os.append(O_t.cpu().numpy())
+    eng.close()
+    return np.concatenate(rows), Os
+
+
+def _compare(a, b):
+    ma, Oa = a
+    mb, Ob = b
+    errs = [rel_l2(x, y) for x, y in zip(Oa, Ob)]
+    print("object rel L2 fused vs unfused:", " ".join(f"{e:.1e}" for e in errs))
+    assert max(errs) <= 1e-5
+    np.testing.assert_allclose(ma[:, 0], mb[:, 0], rtol=1e-5)   # eMod
+    np.testing.assert_allclose(ma[:, 1:], mb[:, 1:], rtol=1e-4, atol=1e-300)
+
+
+@pytest.mark.parametrize("frame,grid,step,jitter", [
+    (64, (5, 5), 12, 1), (128, (3, 4), 40, 3), (192, (2, 3), 48, 2), (256, (3, 3), 40, 3)])
+def test_fused_matches_unfused_method1(frame, grid, step, jitter, gpu_device):
+    print(f"row gather method 1 frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=grid, step=step, jitter=jitter)
+    _compare(_run(s, gpu_device, 1, True), _run(s, gpu_device, 1, False))
+
+
+def test_fused_matches_unfused_background(gpu_device):
+    print("row gather method 4 frame 64", flush=True)
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=1, background_frac=0.02)
+    bg = np.random.default_rng(4).random((64, 64)) * 0.5
+    _compare(_run(s, gpu_device, 4, True, bg=bg), _run(s, gpu_device, 4, False, bg=bg))
+
+
+def test_fused_uncovered_rows_and_graphs(gpu_device):
+    # object larger than the scanned area: rows / columns no frame covers
+    # (the object there becomes 0 / (0 + alpha) = 0 in both paths); graphs on
+    print("row gather padded object + graphs", flush=True)
+    s = small_scan(frame=64, grid=(4, 4), step=16, jitter=2)
+    a = _run(s, gpu_device, 1, True, graphs=True, obj_pad=40)
+    b = _run(s, gpu_device, 1, False, graphs=False, obj_pad=40)
+    _compare(a, b)
+    assert np.all(a[1][-1][-40:, :] == 0) and np.all(a[1][-1][:, -80:] == 0)

@@ -148,13 +148,14 @@ def test_era_op_schedule_centering_pmodp(gpu_device):
     _check_metrics(out_g[-1], out_r[-1], single[0][-1])
 
 
-@pytest.mark.parametrize("method", [4, 6])
-def test_era_background_methods(method, gpu_device):
-    print(f"ERA method {method} (background)", flush=True)
-    s = small_scan(frame=32, grid=(5, 5), step=6, jitter=1, background_frac=0.02)
+@pytest.mark.parametrize("method,frame", [(4, 32), (6, 32), (4, 64)])
+def test_era_background_methods(method, frame, gpu_device):
+    # 64^2 frames: object-only steps run the fused inverse-row / object-update kernel
+    print(f"ERA method {method} (background), frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=(5, 5), step=frame * 3 // 16, jitter=1, background_frac=0.02)
     P0 = s.P if method == 4 else flat_start(s)
     rng = np.random.default_rng(21)
-    B0 = rng.random((32, 32)) * 0.5
+    B0 = rng.random((frame, frame)) * 0.5
     out_g, out_r, tg, tr, single = _run_both(pty.ERA, oracle.ERA, (s.I, s.R, P0, _O0(s)),
                                      dict(method=method, background=B0), 12)
     region = _illuminated(s.P, s.R, s.O.shape)
