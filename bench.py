@@ -55,17 +55,27 @@ STAGE_BYTES_PX_STAGED = {
     "rows_inv": 16,   # read 8, write psi' 8
     "gather": 8,      # read psi' 8 (+ object pixel terms below)
 }
+# staged path with the inverse row FFT fused into the object update (the default
+# for ERA method 1 after the first iteration; timed under "gather"): the row
+# spectra are read once and psi' never goes back to HBM
+STAGE_BYTES_PX_ROWG = {
+    "rows_fwd": 8,
+    "cols": 20,
+    "gather": 8,      # read the row spectra 8 (+ object pixel terms below)
+}
 # fused per-frame kernel (timed under the "cols" stage): amplitude 4 in, psi' 8 out
 STAGE_BYTES_PX_FUSED = {"cols": 12, "gather": 8}
+STAGE_KERNEL = {"rows_fwd": "k_rows_fwd", "cols": "k_cols", "rows_inv": "k_rows_inv"}
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
 # scaled x2 / x1 (what the counters read for a 1 GiB copy, --copy-calibration);
-# summary in profiles/era_4096x256_pmc_traffic.md.  Key: (stage, fused, frames, side).
+# summary in profiles/era_4096x256_pmc_traffic.md.  Key: (kernel, frames, side).
 MEASURED_TRAFFIC = {
-    ("rows_fwd", False, 4096, 256): 3.313e9,
-    ("cols", False, 4096, 256): 5.508e9,
-    ("rows_inv", False, 4096, 256): 4.295e9,
-    ("gather", False, 4096, 256): 2.587e9,
+    ("k_rows_fwd", 4096, 256): 3.311e9,
+    ("k_cols", 4096, 256): 5.441e9,
+    ("k_rows_gather", 4096, 256): 2.374e9,
+    ("k_rows_inv", 4096, 256): 4.295e9,
+    ("k_gather_O", 4096, 256): 2.582e9,
 }
 
 
@@ -342,11 +352,15 @@ def main():
 
     its = args.steps / dt
     fused = prof["rows_fwd"][1] == 0
-    STAGE_BYTES_PX = STAGE_BYTES_PX_FUSED if fused else STAGE_BYTES_PX_STAGED
+    rowg = not fused and prof["rows_inv"][1] == 0
+    STAGE_BYTES_PX = (STAGE_BYTES_PX_FUSED if fused else
+                      STAGE_BYTES_PX_ROWG if rowg else STAGE_BYTES_PX_STAGED)
     stage_avg = {k: (ms / n if n else 0.0) for k, (ms, n) in prof.items()}
     dom = max((k for k in prof if k in STAGE_BYTES_PX), key=lambda k: prof[k][0])
     dom_bytes = STAGE_BYTES_PX[dom] * px + (OBJ_BYTES_PX * U * V if dom == "gather" else 0)
     dom_gbs = dom_bytes / (stage_avg[dom] * 1e-3) / 1e9
+    dom_kernel = ("k_frame (fused rows+columns+modulus)" if fused and dom == "cols" else
+                  STAGE_KERNEL.get(dom, "k_rows_gather" if rowg else "k_gather_O"))
     iter_bytes = ERA1_BYTES_PX * px + OBJ_BYTES_PX * U * V
 
     # ---- DM with background refinement (method 4), same shape ----
@@ -399,12 +413,13 @@ def main():
                    "frame": [M, M], "object": [U, V], "method": 1, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((dom, fused, N, M)),
-                     "kernel": ("k_frame (fused rows+columns+modulus)" if fused and dom == "cols" else dom),
+                     "traffic": None if fused else MEASURED_TRAFFIC.get((dom_kernel, N, M)),
+                     "kernel": dom_kernel,
                      "kernel_ms": stage_avg[dom],
                      "algorithmic_bytes_per_launch": dom_bytes},
         "iteration_roofline_frac": iter_bytes * its / (HBM_PEAK_GBS * 1e9),
         "stage_ms_per_launch": stage_avg,
+        "object_update": "k_rows_gather (inverse row FFT fused)" if rowg else "k_gather_O",
         "final_eMod": float(metrics[-1, 0]),
         "gen_s": t_gen,
         "cpu_baseline": cpu,
