@@ -127,8 +127,9 @@ int pty_get_graphs(pty_ctx* ctx, int32_t* enabled, int32_t* fell_back);
  * its loop is NumPy on the CPU) ---- */
 /* Persistent staged kernels use up to rows_mult * CUs (1..4, default 3)
  * work-groups for the row FFTs and cols_mult * CUs (1..4; 0 = the default per
- * column mode) for the column FFTs; cols_wg (256 or 512) threads per column
- * work-group (512 = 32-column blocks, 256^2 frames only). */
+ * column mode) for the column FFTs; cols_wg threads per column work-group:
+ * 256, 512 (32-column blocks, 256^2 frames only) or 0 = automatic (512 for
+ * the Pmod_1 column steps at 256^2, else 256; the default). */
 int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg);
 /* ERA's staged modulus step over blocks of `frames` frames (0 = the whole
  * stack at once), so the frame spectra stay in the Infinity Cache between its

@@ -238,6 +238,7 @@ struct ColArgs {
   const float* amp;
   long amp_stride;       // elements between frames of amp (0 = one shared frame)
   const uint8_t* mask;   // [M][K], 1 = good, ifftshifted; null = all good
+  const uint32_t* maskbits;  // [T][K]: bit e = mask of element e of column-thread t (Pmod_1 modes)
   float alpha;
   float scale;           // applied to the spectrum before the inverse (1/(M K))
   float* bg;             // background b (per frame or shared)
@@ -260,16 +261,25 @@ template <int MODE> struct ColTraits {
   static constexpr bool bsum = MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD;
   static constexpr bool err = MODE == COL_ERA1 || MODE == COL_ERR1 || MODE == COL_ERR7 ||
                               MODE == COL_ERAB;
-  // waves per SIMD the register budget is tuned for (modes that hoist the
-  // amplitude / background loads above the FFT need more registers)
-  static constexpr int wpe = (MODE == COL_ERR1 || MODE == COL_ERR7 || MODE == COL_ERAB ||
-                              MODE == COL_DMB_UPD || MODE == COL_DMB_LEAD) ? 2 : 3;
-  // Pmod_1 modes load this frame's spectra, amplitudes and mask at the top of
-  // the frame step (amplitudes and mask in flight across the forward FFT)
-  // instead of prefetching the next frame's spectra into registers: measured
-  // 1.49 vs 1.55 ms at 4096 x 256^2 (the register prefetch alone bought
-  // nothing, and its 32 VGPRs pay for the 32 the hoisted loads need).
-  static constexpr bool hoist = MODE == COL_ERA1 || MODE == COL_DM1;
+  // modes that keep amplitude / background values across the FFT need more
+  // registers: tuned for 2 waves per SIMD
+  static constexpr bool heavy = MODE == COL_ERR1 || MODE == COL_ERR7 || MODE == COL_ERAB ||
+                                MODE == COL_DMB_UPD || MODE == COL_DMB_LEAD;
+};
+
+// Launch shape of k_cols<L, MODE>.  Pmod_1 modes with at most 16 points per
+// thread load each frame's spectra and amplitudes at the top of the frame step
+// (amplitudes in flight across the forward FFT) instead of prefetching the
+// next frame's spectra into registers, and take their mask packed one word per
+// column (k_pack_mask): that fits 128 VGPRs, 4 waves per SIMD.  Measured at
+// 4096 x 256^2: 1.55 ms (prefetch, byte mask, 3 waves) -> 1.47 (hoisted
+// loads, 3 waves) -> 1.28 (packed mask, 4 waves) -> 1.18 ms with 512-thread
+// work-groups (32-column blocks).  With 24 points per thread (96, 192) the
+// hoisted amplitudes would cost 2 waves per SIMD; those keep the prefetch.
+template <int L, int MODE> struct ColShape {
+  static constexpr int E = L / Plan<L>::T;
+  static constexpr bool hoist = (MODE == COL_ERA1 || MODE == COL_DM1) && E <= 16;
+  static constexpr int wpe = ColTraits<MODE>::heavy ? 2 : hoist ? 4 : 3;  // waves per SIMD
 };
 
 // Modulus-step arithmetic: the hardware square root and reciprocal (1 ulp;
@@ -288,10 +298,30 @@ __device__ __forceinline__ constexpr int urowB(int e) { return (e / Rb) * T + (e
 // CWG threads per work-group: CPW = CWG / T columns per block, so a wave's
 // load reads (64 / T) rows of CPW * 8 contiguous bytes (CWG = 512 at L = 256:
 // 256-byte row segments).
+// Pack the [M][K] mask for the column stage: word (t, c) holds, in bit e, the
+// mask of the pixel that column-thread t keeps in register slot e of column c
+// (row t + urowB(e)), so a Pmod_1 column step reads one word per column
+// instead of E bytes per frame.
+template <int L>
+__global__ __launch_bounds__(WG) void k_pack_mask(const uint8_t* __restrict__ mask, int K,
+                                                  uint32_t* __restrict__ bits) {
+  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T;
+  static_assert(E <= 32, "one bit per register slot");
+  const int idx = blockIdx.x * WG + threadIdx.x;
+  if (idx >= T * K) return;
+  const int t = idx / K, c = idx % K;
+  uint32_t b = 0;
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (mask[(size_t)(t + urowB<R1, R2, T>(e)) * K + c]) b |= 1u << e;
+  bits[idx] = b;
+}
+
 template <int L, int MODE, int CWG = WG>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
-                          amdgpu_waves_per_eu(ColTraits<MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
+                          amdgpu_waves_per_eu(ColShape<L, MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
   using TR = ColTraits<MODE>;
+  constexpr bool HOIST = ColShape<L, MODE>::hoist;
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
@@ -318,7 +348,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     }
   };
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  if constexpr (!TR::hoist)
+  if constexpr (!HOIST)
     if (wg < nitems) load(wg, (wg / ncb) * a.chunk);
   for (int item = wg; item < nitems; item += gridDim.x) {
     const int cb = item % ncb, chunk = item / ncb;
@@ -327,6 +357,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     const uint32_t toff = (uint32_t)(t * K + c);
     const int n0 = chunk * a.chunk;
     const int n1 = min(a.nframes, n0 + a.chunk);
+    const uint32_t mbits = (HOIST && cv && a.maskbits) ? a.maskbits[t * K + c] : 0xffffffffu;
     double bacc[TR::bsum ? E : 1];
 #pragma unroll
     for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0.0;
@@ -336,20 +367,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       const int nu = __builtin_amdgcn_readfirstlane(n);
       float2* Xn = a.X + (size_t)nu * L * K;
       float2 v[E];
-      float amh[TR::hoist ? E : 1], mh[TR::hoist ? E : 1];
-      if constexpr (TR::hoist) {
+      float amh[HOIST ? E : 1];
+      if constexpr (HOIST) {
         load(item, nu);
         const float* ampn = a.amp + (size_t)nu * a.amp_stride;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const size_t uo = (size_t)urowB<R1, R2, T>(e) * K;
-          amh[e] = cv ? (ampn + uo)[toff] : 0.f;
-          mh[e] = (cv && a.mask) ? (float)(a.mask + uo)[toff] : 1.f;
-        }
+        for (int e = 0; e < E; ++e) amh[e] = cv ? (ampn + (size_t)urowB<R1, R2, T>(e) * K)[toff] : 0.f;
       }
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = nx[e];
-      if constexpr (!TR::hoist) {
+      if constexpr (!HOIST) {
         // prefetch: next frame of this item, else the first frame of the next item
         if (nu + 1 < n1) {
           load(item, nu + 1);
@@ -376,8 +403,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
               a.outI[(size_t)nu * L * K + (size_t)rs * K + cs] = z2;
             } else {
               float m, am;
-              if constexpr (TR::hoist) {
-                m = mh[e];
+              if constexpr (HOIST) {
+                m = ((mbits >> e) & 1u) ? 1.f : 0.f;
                 am = amh[e];
               } else {
                 m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;

@@ -114,7 +114,8 @@ struct pty_ctx {
   int col_blocks = 0, col_chunk = 1, col_chunks = 0;
   int rows_mult = 3;         // persistent row kernels: up to rows_mult * CUs work-groups
   int cols_mult = 0;         // persistent column kernels: cols_mult * CUs (0 = per-mode default)
-  int cols_wg = 256;         // threads per column work-group (512: wide blocks, 256^2 frames)
+  int cols_wg = 0;           // threads per column work-group: 256, 512 (32-column blocks,
+                             // 256^2 frames) or 0 = 512 for the Pmod_1 modes at 256^2, else 256
   int chunk_frames = 0;      // ERA modulus step over blocks of this many frames (0 = all at once)
   int cus = 256;             // compute units of the device (persistent grids)
   // probe-reduction launch shape
@@ -123,6 +124,7 @@ struct pty_ctx {
   // bound (caller-owned)
   const float* amp = nullptr;
   const uint8_t* mask = nullptr;
+  uint32_t* maskbits = nullptr;  // T * K packed mask words for the column stage (owned)
   double I_norm = 0.0;
   float2* O = nullptr;
   float2* P = nullptr;
@@ -302,6 +304,7 @@ int row_blocks(pty_ctx* c, int nrows) {
 ColArgs col_args(pty_ctx* c) {
   ColArgs a{};
   a.mask = c->mask;
+  a.maskbits = c->mask ? c->maskbits : nullptr;
   a.scale = 1.0f / (float)c->MK();
   a.tw = c->twM;
   a.nframes = c->N;
@@ -317,10 +320,23 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   // frame chunks only matter where per-chunk background sums are accumulated
   if (!ColTraits<MODE>::bsum) a.chunk = 1;
   const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
-  const bool wide = c->cols_wg == 512 && c->M == 256;
+  int wpe = 3;
+  bool hoist = false;
+  switch (c->M) {
+#define C(Lv)                              \
+  case Lv:                                 \
+    wpe = ColShape<Lv, MODE>::wpe;         \
+    hoist = ColShape<Lv, MODE>::hoist;     \
+    break;
+    PTY_SIZES(C)
+#undef C
+    default: return fail(PTY_EUNSUP, "unsupported M");
+  }
+  const bool wide = c->M == 256 && (c->cols_wg == 512 || (c->cols_wg == 0 && hoist));
   const int cpw = (wide ? 512 : WG) / threads_per_line(c->M);
   const long items = (long)((c->K + cpw - 1) / cpw) * chunks;
-  const int mult = c->cols_mult ? c->cols_mult : (wide ? 2 : ColTraits<MODE>::wpe);
+  // default grid: as many work-groups per CU as the register budget keeps resident
+  const int mult = c->cols_mult ? c->cols_mult : (wide ? std::max(1, wpe / 2) : wpe);
   const int grid = persistent_grid(items, (long)mult * c->cus);
   if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
   if (wide) {
@@ -1199,7 +1215,7 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->frame_slots = e_sl ? std::max(1, atoi(e_sl)) : cus;
     c->cus = cus;
     const char* e_cw = getenv("PTY_COLS_WG");
-    if (e_cw && atoi(e_cw) == 512) c->cols_wg = 512;
+    if (e_cw && (atoi(e_cw) == 256 || atoi(e_cw) == 512)) c->cols_wg = atoi(e_cw);
     // The modulus step runs over the whole stack at once by default: blocks
     // sized for the 256 MB Infinity Cache (128..512 frames of 256^2) measured
     // 8..18% slower per ERA iteration at 4096 x 256^2, the persistent row and
@@ -1285,7 +1301,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->X); dev_free(c->F); dev_free(c->Xp);
   dev_free(c->Obak); dev_free(c->Pbak); dev_free(c->Os); dev_free(c->Oroll); dev_free(c->Ps);
   dev_free(c->H); dev_free(c->G); dev_free(c->bg); dev_free(c->bbar); dev_free(c->bbar_m);
-  dev_free(c->pamp); dev_free(c->partb); dev_free(c->pnum); dev_free(c->pg);
+  dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->pnum); dev_free(c->pg);
   dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
@@ -1320,6 +1336,22 @@ int pty_set_data(pty_ctx* c, const float* amp, const uint8_t* mask, double I_nor
   c->amp = amp;
   c->mask = mask;
   c->I_norm = I_norm;
+  if (mask) {
+    const int T = threads_per_line(c->M);
+    if (!c->maskbits) PTY_TRY(dev_alloc(&c->maskbits, (size_t)T * c->K));
+    const int grid = (T * c->K + WG - 1) / WG;
+    switch (c->M) {
+#define C(Lv)                                                                                  \
+  case Lv:                                                                                     \
+    hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c->stream, mask, c->K,        \
+                       c->maskbits);                                                           \
+    break;
+      PTY_SIZES(C)
+#undef C
+      default: return fail(PTY_EUNSUP, "unsupported M");
+    }
+    PTY_LAUNCHED("k_pack_mask");
+  }
   return PTY_OK;
 }
 
@@ -1435,7 +1467,8 @@ int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t col
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 4)
     return fail(PTY_EINVAL, "rows_mult must be 1..4 and cols_mult 0..4");
-  if (cols_wg != 256 && cols_wg != 512) return fail(PTY_EINVAL, "cols_wg must be 256 or 512");
+  if (cols_wg != 0 && cols_wg != 256 && cols_wg != 512)
+    return fail(PTY_EINVAL, "cols_wg must be 0 (automatic), 256 or 512");
   PTY_HIP(hipStreamSynchronize(c->stream));
   c->rows_mult = rows_mult;
   c->cols_mult = cols_mult;

@@ -133,9 +133,10 @@ class Engine:
         call("pty_run", self._ctx, ctypes.byref(o), int(iter0), int(iters), int(bool(begin)), ptr)
         return out
 
-    def set_launch(self, rows_mult=3, cols_mult=0, cols_wg=256):
-        """Persistent grid sizes of the staged row / column kernels (x CUs) and
-        threads per column work-group (256, or 512 for 256^2 frames)."""
+    def set_launch(self, rows_mult=3, cols_mult=0, cols_wg=0):
+        """Persistent grid sizes of the staged row / column kernels (x CUs; 0 =
+        per-kernel default) and threads per column work-group (256, 512 for
+        256^2 frames, or 0 = automatic)."""
         call("pty_set_launch", self._ctx, int(rows_mult), int(cols_mult), int(cols_wg))
 
     def set_chunk(self, frames):
