@@ -610,10 +610,11 @@ struct RowGatherArgs {
   float2* Oout;            // new object [U][V] (may alias Oold)
   const double* H;         // heatmap sum |P|^2 [U][V] (valid)
   float alpha;
-  const int* __restrict__ row_ptr;    // CSR over object rows
-  const int2* __restrict__ row_ent;   // per object row, ascending n: (n*M + i, x_n << 16 | i)
-  double* part_econ;       // [2][U]  sum |Oold - O|^2, sum |O|^2
-  double* part_esup;       // [2][U]  sum Re(conj(O) num), sum |O|^2 H
+  int segs, segw;          // column segments per object row and their width
+  const int* __restrict__ row_ptr;    // CSR over (object row, segment)
+  const int2* __restrict__ row_ent;   // per segment, ascending n: (n*M + i, x_n << 16 | i)
+  double* part_econ;       // [2][U*segs]  sum |Oold - O|^2, sum |O|^2
+  double* part_esup;       // [2][U*segs]  sum Re(conj(O) num), sum |O|^2 H
 };
 
 // Largest object width the fused kernel's LDS row accumulator takes (and
@@ -621,21 +622,24 @@ struct RowGatherArgs {
 constexpr int ROWG_VMAX = 8192;
 static_assert(ROWG_VMAX < (1 << 15), "x_n << 16 must stay a positive int");
 
-// One work-group per object row u.  The row's frames (ascending n) are taken
-// LPW at a time, one per line: frame row i = u - y_n is inverse-transformed in
-// registers (the column stage already applied the 1/(M K) of the inverse),
-// multiplied by conj(P[i, :]) and parked in the line's own LDS exchange area;
-// then each object column of the row adds the parked values of the lines
-// covering it, in line order.  Every pixel so sums its frames in
+// One work-group per (object row u, column segment [v0, v0 + segw)).  The
+// frames whose window covers the segment's part of row u (ascending n) are
+// taken LPW at a time, one per line: frame row i = u - y_n is inverse-
+// transformed in registers (the column stage already applied the 1/(M K) of
+// the inverse), multiplied by conj(P[i, :]) and parked in the line's own LDS
+// exchange area; then each column of the segment adds the parked values of
+// the lines covering it, in line order.  Every pixel so sums its frames in
 // ascending order, as k_gather_O does, but psi' is never written to or read
-// back from HBM.  `row` (dynamic LDS, V float2) accumulates the row.
+// back from HBM.  `row` (dynamic LDS, segw float2) accumulates the segment;
+// rows are split into segments so that three work-groups fit a CU's LDS (a
+// frame row that straddles two segments is transformed by both).
 //
 // Global loads are issued in the order their results are needed (CSR entries
 // two batches ahead, then this batch's probe row, then the next batch's
 // spectra), so no wait on the in-order load counter also waits for the
 // prefetch of the next batch.
 template <int L>
-__global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
+__global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
                                                     const float2* __restrict__ tw) {
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
@@ -649,10 +653,11 @@ __global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
   extern __shared__ float2 row[];
   stage_twiddles<L>(s_tw, tw);
   const int tid = threadIdx.x, line = tid / T, t = tid % T;
-  const int u = blockIdx.x, M = g.M, V = g.V;
-  for (int v = tid; v < V; v += WG) row[v] = make_float2(0.f, 0.f);
+  const int item = blockIdx.x, u = item / a.segs, M = g.M, V = g.V;
+  const int v0 = (item % a.segs) * a.segw, v1 = min(V, v0 + a.segw);
+  for (int v = tid; v < v1 - v0; v += WG) row[v] = make_float2(0.f, 0.f);
   if (tid == 0) lds[LPW * W] = make_float2(0.f, 0.f);
-  const int b0 = a.row_ptr[u], cnt = a.row_ptr[u + 1] - b0;
+  const int b0 = a.row_ptr[item], cnt = a.row_ptr[item + 1] - b0;
   const int nbatch = (cnt + LPW - 1) / LPW;
   // spare lines of the last batch re-read a real entry and are masked by OFF_NONE
   auto entry = [&](int bt) { return a.row_ent[b0 + min(bt * LPW + line, cnt - 1)]; };
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
     if (t == 0) s_oy[line] = (bt * LPW + line < cnt) ? (m_cur.y >> 16) : OFF_NONE;
     lds_barrier();
     int oy[LPW];
-    int lo = V, hi = 0;
+    int lo = v1, hi = v0;
 #pragma unroll
     for (int l = 0; l < LPW; ++l) {
       oy[l] = __builtin_amdgcn_readfirstlane(s_oy[l]);
@@ -697,12 +702,14 @@ __global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
         hi = max(hi, oy[l] + L);
       }
     }
+    lo = max(lo, v0);
+    hi = min(hi, v1);
     // branch-free: every line reads its parked value or the zero word, so the
     // LPW reads are all in flight before the in-order adds (skipping lines
     // that miss a wave's columns with uniform branches measured slower)
     for (int base = lo; base < hi; base += WG) {
       const int vv = base + tid;
-      float2 s = (vv < hi) ? row[vv] : make_float2(0.f, 0.f);
+      float2 s = (vv < hi) ? row[vv - v0] : make_float2(0.f, 0.f);
       float2 x[LPW];
 #pragma unroll
       for (int l = 0; l < LPW; ++l) {
@@ -711,16 +718,16 @@ __global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
       }
 #pragma unroll
       for (int l = 0; l < LPW; ++l) s = s + x[l];
-      if (vv < hi) row[vv] = s;
+      if (vv < hi) row[vv - v0] = s;
     }
     lds_barrier();
     m_cur = m_next;
     m_next = m_after;
   }
   double r[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int v = tid; v < V; v += WG) {
+  for (int v = v0 + tid; v < v1; v += WG) {
     const size_t pix = (size_t)u * V + v;
-    const float2 acc = row[v];
+    const float2 acc = row[v - v0];
     const double h = a.H[pix];
     const double den = h + (double)a.alpha;
     const double ox = (double)acc.x / den, oyv = (double)acc.y / den;
@@ -740,9 +747,9 @@ __global__ __launch_bounds__(WG, 2) void k_rows_gather(Geom g, RowGatherArgs a,
   if (a.part_econ || a.part_esup) {
     block_sum<4>(r, red);
     if (tid == 0) {
-      const int nb = g.U;
-      if (a.part_econ) { a.part_econ[u] = r[0]; a.part_econ[nb + u] = r[1]; }
-      if (a.part_esup) { a.part_esup[u] = r[2]; a.part_esup[nb + u] = r[3]; }
+      const int nb = gridDim.x;
+      if (a.part_econ) { a.part_econ[item] = r[0]; a.part_econ[nb + item] = r[1]; }
+      if (a.part_esup) { a.part_esup[item] = r[2]; a.part_esup[nb + item] = r[3]; }
     }
   }
 }

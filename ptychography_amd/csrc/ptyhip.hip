@@ -108,6 +108,7 @@ struct pty_ctx {
   int2* row_ent = nullptr;   // (frame row n*M + i, column offset x_n << 16 | i)
   bool rowg_ok = false;      // geometry supports the fused kernel
   bool rowg = false;         // currently selected
+  int rowg_segs = 1, rowg_segw = 0;  // column segments per object row, their width
   float2* twM = nullptr;
   float2* twK = nullptr;
   // column-stage launch shape
@@ -432,21 +433,34 @@ int gather(pty_ctx* c, const float2* S, const float2* P, const float2* Oold, flo
 template <int L>
 size_t rowg_static_lds() {
   constexpr size_t lpw = WG / Plan<L>::T;
-  return sizeof(float2) * (lpw * LineLds<L>::words + L) + 16 * sizeof(double) + lpw * sizeof(int);
+  return sizeof(float2) * (lpw * LineLds<L>::words + 1 + L) + 16 * sizeof(double) + lpw * sizeof(int);
 }
 
-bool rowg_supported(int K, int V) {
+// Column segments per object row for the fused kernel: the fewest that let
+// three work-groups (else two, else one) share a CU's 160 KB of LDS, each
+// segment at least 512 columns wide; 0 = geometry not supported.
+int rowg_segments(int K, int V) {
+  size_t st = 0;
   switch (K) {
-#define C(Lv) case Lv: return V <= ROWG_VMAX && rowg_static_lds<Lv>() + (size_t)V * 8 <= (160u << 10);
+#define C(Lv) case Lv: st = rowg_static_lds<Lv>(); break;
     PTY_ROWG_SIZES(C)
 #undef C
-    default: return false;
+    default: return 0;
   }
+  if (V > ROWG_VMAX) return 0;
+  for (int per_cu = 3; per_cu >= 1; --per_cu) {
+    const size_t budget = (size_t)(160u << 10) / per_cu;
+    if (budget < st + 512 * sizeof(float2)) continue;
+    const size_t w = (budget - st) / sizeof(float2);
+    const int segs = (int)((V + w - 1) / w);
+    if (segs <= 4) return segs;
+  }
+  return 0;
 }
 
-// Raise the dynamic-LDS limit of the K-sized instantiation to V float2.
-int rowg_attribute(int K, int V) {
-  const int bytes = V * (int)sizeof(float2);
+// Raise the dynamic-LDS limit of the K-sized instantiation to `width` float2.
+int rowg_attribute(int K, int width) {
+  const int bytes = width * (int)sizeof(float2);
   hipError_t e = hipErrorInvalidValue;
   switch (K) {
 #define C(Lv)                                                                                 \
@@ -473,15 +487,18 @@ int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, 
   a.Oout = c->O;
   a.H = c->H;
   a.alpha = alpha;
+  a.segs = c->rowg_segs;
+  a.segw = c->rowg_segw;
   a.row_ptr = c->row_ptr;
   a.row_ent = c->row_ent;
   a.part_econ = econ ? c->part_econO : nullptr;
   a.part_esup = esup ? c->part_esxh : nullptr;
-  const size_t dyn = (size_t)c->V * sizeof(float2);
+  const size_t dyn = (size_t)c->rowg_segw * sizeof(float2);
+  const int grid = c->U * c->rowg_segs;
   switch (c->K) {
 #define C(Lv)                                                                                \
   case Lv:                                                                                   \
-    hipLaunchKernelGGL(k_rows_gather<Lv>, dim3(c->U), dim3(WG), dyn, c->stream, c->g, a,     \
+    hipLaunchKernelGGL(k_rows_gather<Lv>, dim3(grid), dim3(WG), dyn, c->stream, c->g, a,     \
                        (const float2*)c->twK);                                               \
     break;
     PTY_ROWG_SIZES(C)
@@ -489,7 +506,7 @@ int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, 
     default: return fail(PTY_EUNSUP, "fused row gather: unsupported K");
   }
   PTY_LAUNCHED("k_rows_gather");
-  if (nparts) *nparts = c->U;
+  if (nparts) *nparts = grid;
   return PTY_OK;
 }
 
@@ -1176,18 +1193,27 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
         frames[fill[ty * c->tiles_x + tx]++] = n;
 
   // object rows -> ascending frame lists, when the fused kernel applies
-  c->rowg_ok = rowg_supported(K, V) && (size_t)N * M <= ((size_t)1 << 26);
+  const int segs = rowg_segments(K, V);
+  c->rowg_ok = segs > 0 && (size_t)N * M <= ((size_t)1 << 26);
   std::vector<int> rcnt;
   std::vector<int2> rents;
   if (c->rowg_ok) {
-    rcnt.assign(U + 1, 0);
+    c->rowg_segs = segs;
+    c->rowg_segw = (V + segs - 1) / segs;
+    const int sw = c->rowg_segw, nitem = U * segs;
+    auto seg0 = [&](int n) { return off[n].y / sw; };
+    auto seg1 = [&](int n) { return (off[n].y + K - 1) / sw; };
+    rcnt.assign(nitem + 1, 0);
     for (int n = 0; n < N; ++n)
-      for (int i = 0; i < M; ++i) rcnt[off[n].x + i + 1]++;
-    for (int u = 0; u < U; ++u) rcnt[u + 1] += rcnt[u];
-    rents.resize((size_t)N * M);
+      for (int i = 0; i < M; ++i)
+        for (int sg = seg0(n); sg <= seg1(n); ++sg) rcnt[(off[n].x + i) * segs + sg + 1]++;
+    for (int k = 0; k < nitem; ++k) rcnt[k + 1] += rcnt[k];
+    rents.resize(rcnt[nitem]);
     std::vector<int> rfill(rcnt.begin(), rcnt.end() - 1);
     for (int n = 0; n < N; ++n)
-      for (int i = 0; i < M; ++i) rents[rfill[off[n].x + i]++] = make_int2(n * M + i, off[n].y << 16 | i);
+      for (int i = 0; i < M; ++i)
+        for (int sg = seg0(n); sg <= seg1(n); ++sg)
+          rents[rfill[(off[n].x + i) * segs + sg]++] = make_int2(n * M + i, off[n].y << 16 | i);
   }
 
   // launch shapes
@@ -1228,7 +1254,7 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     if (e_on && atoi(e_on) == 1) c->use_frame = c->frame_ok;
   }
   c->cap_col = std::max(4 * c->cus, c->frame_slots);
-  c->cap_gather = std::max(c->ntiles, U);
+  c->cap_gather = std::max(c->ntiles, U * c->rowg_segs);
   c->cap_econO = std::max(c->cap_gather, ECON_BLOCKS);
   c->cap_econP = std::max(pblocks, ECON_BLOCKS);
   c->cap_esup = std::max(std::max(ESUP_BLOCKS, row_blocks(c, N * M)), c->frame_slots);
@@ -1262,9 +1288,9 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   if ((r = dev_alloc(&c->part_esup, c->cap_esup)) != PTY_OK) return cleanup(r);
   if ((r = dev_alloc(&c->part_esxh, 2 * (size_t)c->cap_gather)) != PTY_OK) return cleanup(r);
   if (c->rowg_ok) {
-    if ((r = dev_alloc(&c->row_ptr, (size_t)U + 1)) != PTY_OK) return cleanup(r);
+    if ((r = dev_alloc(&c->row_ptr, rcnt.size())) != PTY_OK) return cleanup(r);
     if ((r = dev_alloc(&c->row_ent, rents.size())) != PTY_OK) return cleanup(r);
-    if ((r = rowg_attribute(K, V)) != PTY_OK) return cleanup(r);
+    if ((r = rowg_attribute(K, c->rowg_segw)) != PTY_OK) return cleanup(r);
     const char* e_rg = getenv("PTY_ROW_GATHER");
     c->rowg = !(e_rg && atoi(e_rg) == 0);
   }
@@ -1281,7 +1307,7 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   if (e == hipSuccess)
     e = hipMemcpy(c->tile_frames, frames.data(), frames.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess && c->rowg_ok)
-    e = hipMemcpy(c->row_ptr, rcnt.data(), (U + 1) * sizeof(int), hipMemcpyHostToDevice);
+    e = hipMemcpy(c->row_ptr, rcnt.data(), rcnt.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess && c->rowg_ok)
     e = hipMemcpy(c->row_ent, rents.data(), rents.size() * sizeof(int2), hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(fail(PTY_EHIP, std::string("upload: ") + hipGetErrorString(e)));

@@ -64,6 +64,16 @@ def test_fused_matches_unfused_method1(frame, grid, step, jitter, gpu_device):
     _compare(_run(s, gpu_device, 1, True), _run(s, gpu_device, 1, False))
 
 
+@pytest.mark.parametrize("frame,grid,step", [(256, (2, 16), 130), (64, (2, 80), 55)])
+def test_fused_segmented_rows(frame, grid, step, gpu_device):
+    # objects wide enough that each object row is split into two column
+    # segments (frames straddling the boundary are transformed by both)
+    print(f"row gather segmented rows frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=grid, step=step, jitter=2)
+    assert s.O.shape[1] > {256: 2200, 64: 4400}[frame]
+    _compare(_run(s, gpu_device, 1, True), _run(s, gpu_device, 1, False))
+
+
 def test_fused_matches_unfused_background(gpu_device):
     print("row gather method 4 frame 64", flush=True)
     s = small_scan(frame=64, grid=(5, 5), step=12, jitter=1, background_frac=0.02)
