@@ -1012,6 +1012,20 @@ struct Regions {
 
 enum EsupMode { ESUP_NONE = 0, ESUP_EXPLICIT = 1, ESUP_IDENTITY = 2 };
 
+// Fixed-order strided sum p[tid] + p[tid + nt] + ... over eight independent
+// accumulators (eight loads in flight per thread), combined pairwise.
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, int n) {
+  const int nt = blockDim.x;
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int i = threadIdx.x;
+  for (; i + 7 * nt < n; i += 8 * nt) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] += p[i + j * nt];
+  }
+  for (; i < n; i += nt) a[7] += p[i];
+  return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 // metrics[0..2] (or, with a slot counter, metrics[3*slot .. 3*slot+2] followed
 // by slot += 1, so one captured graph serves every iteration)
 __global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, double MK,
@@ -1019,12 +1033,7 @@ __global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, dou
   __shared__ double red[16 * NSLOT];
   double s[NSLOT];
 #pragma unroll
-  for (int k = 0; k < NSLOT; ++k) {
-    double acc = 0.0;
-    if (rg.p[k])
-      for (int i = threadIdx.x; i < rg.n[k]; i += blockDim.x) acc += rg.p[k][i];
-    s[k] = acc;
-  }
+  for (int k = 0; k < NSLOT; ++k) s[k] = rg.p[k] ? strided_sum(rg.p[k], rg.n[k]) : 0.0;
   block_sum<NSLOT>(s, red);
   if (threadIdx.x == 0) {
     if (slot) {
@@ -1050,8 +1059,7 @@ __global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, dou
 // plain sum of one region into out[0]
 __global__ __launch_bounds__(1024) void k_sum(const double* p, int n, double* out) {
   __shared__ double red[16];
-  double s[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s[0] += p[i];
+  double s[1] = {strided_sum(p, n)};
   block_sum<1>(s, red);
   if (threadIdx.x == 0) out[0] = s[0];
 }
