@@ -125,8 +125,8 @@ int pty_get_graphs(pty_ctx* ctx, int32_t* enabled, int32_t* fell_back);
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference:
  * its loop is NumPy on the CPU) ---- */
-/* Persistent staged kernels use up to rows_mult * CUs (1..4, default 3)
- * work-groups for the row FFTs and cols_mult * CUs (1..4; 0 = the default per
+/* Persistent staged kernels use up to rows_mult * CUs (1..256, default 3)
+ * work-groups for the row FFTs and cols_mult * CUs (1..128; 0 = the default per
  * column mode) for the column FFTs; cols_wg threads per column work-group:
  * 256, 512 (32-column blocks, 256^2 frames only) or 0 = automatic (512 for
  * the Pmod_1 column steps at 256^2, else 256; the default). */

@@ -36,6 +36,20 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float norm2(float2 a) { return a.x * a.x + a.y * a.y; }
 
+// Streaming loads / store (non-temporal hint) for frame stacks that are read
+// or written once per kernel and would only push reused data out of L2.
+__device__ __forceinline__ float2 load_stream(const float2* p) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v w = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(w.x, w.y);
+}
+__device__ __forceinline__ float load_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void store_stream(float2* p, float2 v) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+}
+
 // multiply by -i (forward sign) or +i (inverse sign)
 template <bool INV>
 __device__ __forceinline__ float2 mul_negi(float2 a) {

@@ -1494,8 +1494,8 @@ int pty_get_row_gather(pty_ctx* c, int32_t* supported, int32_t* enabled) {
 
 int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t cols_wg) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
-  if (rows_mult < 1 || rows_mult > 4 || cols_mult < 0 || cols_mult > 128)
-    return fail(PTY_EINVAL, "rows_mult must be 1..4 and cols_mult 0..128");
+  if (rows_mult < 1 || rows_mult > 256 || cols_mult < 0 || cols_mult > 128)
+    return fail(PTY_EINVAL, "rows_mult must be 1..256 and cols_mult 0..128");
   if (cols_wg != 0 && cols_wg != 256 && cols_wg != 512)
     return fail(PTY_EINVAL, "cols_wg must be 0 (automatic), 256 or 512");
   PTY_HIP(hipStreamSynchronize(c->stream));
