@@ -280,8 +280,15 @@ template <int MODE> struct ColTraits {
 // hoisted amplitudes would cost 2 waves per SIMD; those keep the prefetch.
 template <int L, int MODE> struct ColShape {
   static constexpr int E = L / Plan<L>::T;
-  static constexpr bool hoist = (MODE == COL_ERA1 || MODE == COL_DM1) && E <= 16;
-  static constexpr int wpe = ColTraits<MODE>::heavy ? 2 : hoist ? 4 : 3;  // waves per SIMD
+  static constexpr bool modulus = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_ERR1 ||
+                                  MODE == COL_ERR7 || MODE == COL_ERAB ||
+                                  MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD;
+  static constexpr bool hoist = modulus && E <= 16;
+  // modes that also hold background values across the FFT fit 3 waves
+  static constexpr bool bgvals = MODE == COL_ERR7 || MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
+                                 MODE == COL_DMB_UPD;
+  static constexpr int wpe = hoist ? (bgvals ? ((MODE == COL_DMB_UPD && L == 128) ? 2 : 3) : 4)
+                                   : ColTraits<MODE>::heavy ? 2 : 3;  // waves per SIMD
 };
 
 // Modulus-step arithmetic: the hardware square root and reciprocal (1 ulp;
@@ -324,6 +331,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                           amdgpu_waves_per_eu(ColShape<L, MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
   using TR = ColTraits<MODE>;
   constexpr bool HOIST = ColShape<L, MODE>::hoist;
+  // hoisted per-frame (or shared) background values; the frame mean bbar is
+  // the same for every frame (an L2 hit) and is read where it is used, which
+  // keeps DMB_UPD and ERR7 within their register budget
+  constexpr bool HBG = HOIST && (MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD);
+  constexpr bool HBB = false;
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
@@ -360,22 +372,34 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     const int n0 = chunk * a.chunk;
     const int n1 = min(a.nframes, n0 + a.chunk);
     const uint32_t mbits = (HOIST && cv && a.maskbits) ? a.maskbits[t * K + c] : 0xffffffffu;
-    double bacc[TR::bsum ? E : 1];
+    // per-chunk background sums (at most 16 frames): FP32 registers in the
+    // hoisted modes (relative rounding <= 16 * 2^-24), written out in FP64
+    using BAcc = std::conditional_t<HOIST, float, double>;
+    BAcc bacc[TR::bsum ? E : 1];
 #pragma unroll
-    for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0.0;
+    for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0;
 
     for (int n = n0; n < n1; ++n) {
       // opaque, wave-uniform frame index: keeps the frame base in SGPRs
       const int nu = __builtin_amdgcn_readfirstlane(n);
       float2* Xn = a.X + (size_t)nu * L * K;
       float2 v[E];
-      float amh[HOIST ? E : 1];
+      float amh[HOIST ? E : 1], bgh[HBG ? E : 1], bbh[HBB ? E : 1];
       if constexpr (HOIST) {
         load(item, nu);
         const float* ampn = a.amp + (size_t)nu * a.amp_stride;
 #pragma unroll
         for (int e = 0; e < E; ++e)
           amh[e] = cv ? load_stream(&(ampn + (size_t)urowB<R1, R2, T>(e) * K)[toff]) : 0.f;
+        if constexpr (HBG) {
+          const float* bgn = a.bg + (size_t)nu * a.bg_stride;
+#pragma unroll
+          for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)urowB<R1, R2, T>(e) * K)[toff] : 0.f;
+        }
+        if constexpr (HBB) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) bbh[e] = cv ? (a.bbar + (size_t)urowB<R1, R2, T>(e) * K)[toff] : 0.f;
+        }
       }
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = nx[e];
@@ -427,12 +451,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 const float d = msqrt(z2) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERR7) {
-                const float b = (a.bbar + uo)[toff];
+                float b;
+                if constexpr (HBB) b = bbh[e];
+                else b = (a.bbar + uo)[toff];
                 const float d = msqrt(z2 + b * b) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
                 float* bp = (bgn + uo) + toff;
-                const float b = *bp;
+                float b;
+                if constexpr (HBG) b = bgh[e];
+                else b = *bp;
                 const float mg = msqrt(z2 + b * b + a.alpha);
                 if constexpr (MODE == COL_ERAB) {
                   const float d = mg - am;
@@ -440,21 +468,27 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 }
                 const float fac = m * am * mrcp(mg);
                 const float bn = b * fac;
-                bacc[e] += (double)bn;
+                bacc[e] += (BAcc)bn;
                 if constexpr (MODE == COL_DMB_LEAD) *bp = bn;
                 const float2 zn = z * fac;
                 if constexpr (MODE == COL_ERAB) acc[1] += (double)norm2(zn);
                 v[e] = zn * a.scale;
               } else if constexpr (MODE == COL_DMB_UPD) {
                 float* bp = (bgn + uo) + toff;
-                const float b = *bp;
-                const float bb = (a.bbar + uo)[toff];
+                float b, bb;
+                if constexpr (HBG) {
+                  b = bgh[e];
+                  bb = (a.bbar + uo)[toff];
+                } else {
+                  b = *bp;
+                  bb = (a.bbar + uo)[toff];
+                }
                 const float b0 = 2.f * bb - b;
                 const float mg = msqrt(z2 + b0 * b0 + a.alpha);
                 const float fac = m * am * mrcp(mg);
                 const float bn = (b - bb) + b0 * fac;
                 *bp = bn;
-                bacc[e] += (double)bn;
+                bacc[e] += (BAcc)bn;
                 v[e] = z * (fac * a.scale);
               }
             }
@@ -487,7 +521,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       if (cv) {
         double* pb = a.partb + (size_t)chunk * L * K;
 #pragma unroll
-        for (int e = 0; e < E; ++e) (pb + (size_t)urowB<R1, R2, T>(e) * K)[toff] = bacc[e];
+        for (int e = 0; e < E; ++e) (pb + (size_t)urowB<R1, R2, T>(e) * K)[toff] = (double)bacc[e];
       }
     }
   }
