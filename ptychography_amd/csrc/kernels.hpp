@@ -108,7 +108,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
     const size_t rbase = (size_t)row * L;
     if constexpr (SRC == SRC_F) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) n0[e] = F[rbase + idxA<R1, R2, T>(e, t)];
+      for (int e = 0; e < E; ++e) n0[e] = load_stream(&F[rbase + idxA<R1, R2, T>(e, t)]);
     } else {
       const int n = row / M, i = row % M;
       const int2 o = g.off[n];
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
       for (int e = 0; e < E; ++e) {
         const int c = idxA<R1, R2, T>(e, t);
         n0[e] = Orow[c];
-        if constexpr (SRC == SRC_DM) n2[e] = F[rbase + c];
+        if constexpr (SRC == SRC_DM) n2[e] = load_stream(&F[rbase + c]);
       }
     }
   };
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
     const int row = min(grp * LPW + line, nrows - 1);
     const size_t rbase = (size_t)row * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nx[e] = Y[rbase + idxA<R2, R1, T>(e, t)];
+    for (int e = 0; e < E; ++e) nx[e] = load_stream(&Y[rbase + idxA<R2, R1, T>(e, t)]);
   };
   double acc[1] = {0.0};
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
           const float2 y = v[e] * scale;
           const float2 d = y - ex;
           acc[0] += (double)norm2(d);
-          Out[rbase + c] = Out[rbase + c] + d;
+          store_stream(&Out[rbase + c], load_stream(&Out[rbase + c]) + d);
         }
       }
     }
