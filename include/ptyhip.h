@@ -108,9 +108,9 @@ int pty_background(pty_ctx* ctx, float* b_dev);
  * frame through an Infinity-Cache-resident scratch slot; square 128 / 256
  * frames, methods without background) and the staged row / column kernels.
  * threads: 0 = keep, 512 or 1024 per work-group; slots: 0 = keep, else the
- * number of work-groups (and scratch slots).  Default: fused for 128 x 128
- * frames (staged for 256 x 256, where it measured faster), one slot per
- * compute unit. */
+ * number of work-groups (and scratch slots).  Default: staged (measured
+ * faster at 128 x 128 and 256 x 256; env PTY_FRAME=1 selects the fused
+ * kernel), one slot per compute unit. */
 int pty_set_path(pty_ctx* ctx, int32_t fused, int32_t threads, int32_t slots);
 
 /* enable != 0: pty_run captures each distinct kind of iteration (its branch

@@ -29,8 +29,9 @@ def require_gpu(device=None):
     return dev
 
 
-# Path override for every new Engine: None = library default (fused per-frame
-# kernel where the frame shape supports it), False = staged kernels, True = fused.
+# Path override for every new Engine: None = library default (the staged
+# kernels), False = staged kernels, True = the fused per-frame kernel where the
+# frame shape supports it.
 FUSED = None
 
 

@@ -231,8 +231,8 @@ def test_r_shifted_in_place_and_errors(gpu_device):
 
 @pytest.mark.parametrize("fused", [True, False])
 def test_era_and_dm_128_frames_both_paths(fused, gpu_device):
-    """128^2 frames run through the fused per-frame kernel by default; the staged
-    kernels must give the same answers."""
+    """128^2 frames through the fused per-frame kernel (opt-in) and through the
+    staged kernels (the default): both hold the oracle parity bar."""
     print(f"ERA/DM at 128^2, fused={fused}", flush=True)
     from ptychography_amd import engine
     s = small_scan(frame=128, grid=(4, 4), step=24, jitter=2)
