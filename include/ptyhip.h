@@ -153,8 +153,9 @@ typedef struct pty_tune_result {
 } pty_tune_result;
 /* Times `trial_iters` iterations of opts for every launch configuration
  * (staged: rows_mult {2,3,4} x cols_mult {default,2,4}, and at 256^2 512-thread
- * column work-groups x cols_mult {default,1,3}; ERA frame blocks of 0 and
- * 1/4..2x the default; fused per-frame path
+ * column work-groups x cols_mult {default,1,3}; the fused inverse-row object
+ * update on / off; ERA frame blocks of 0 and 1/4..2x the default; fused
+ * per-frame path
  * where the frame shape allows it: 512 / 1024 threads x CUs / 2*CUs slots),
  * keeps the fastest and restores O, P and the background; the next pty_run
  * must pass begin = 1. */
