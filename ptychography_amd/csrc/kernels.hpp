@@ -335,7 +335,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
   // the same for every frame (an L2 hit) and is read where it is used, which
   // keeps DMB_UPD and ERR7 within their register budget
   constexpr bool HBG = HOIST && (MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD);
-  constexpr bool HBB = false;
   constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
@@ -384,7 +383,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       const int nu = __builtin_amdgcn_readfirstlane(n);
       float2* Xn = a.X + (size_t)nu * L * K;
       float2 v[E];
-      float amh[HOIST ? E : 1], bgh[HBG ? E : 1], bbh[HBB ? E : 1];
+      float amh[HOIST ? E : 1], bgh[HBG ? E : 1];
       if constexpr (HOIST) {
         load(item, nu);
         const float* ampn = a.amp + (size_t)nu * a.amp_stride;
@@ -395,10 +394,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           const float* bgn = a.bg + (size_t)nu * a.bg_stride;
 #pragma unroll
           for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)urowB<R1, R2, T>(e) * K)[toff] : 0.f;
-        }
-        if constexpr (HBB) {
-#pragma unroll
-          for (int e = 0; e < E; ++e) bbh[e] = cv ? (a.bbar + (size_t)urowB<R1, R2, T>(e) * K)[toff] : 0.f;
         }
       }
 #pragma unroll
@@ -451,9 +446,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 const float d = msqrt(z2) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERR7) {
-                float b;
-                if constexpr (HBB) b = bbh[e];
-                else b = (a.bbar + uo)[toff];
+                const float b = (a.bbar + uo)[toff];
                 const float d = msqrt(z2 + b * b) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
@@ -475,14 +468,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 v[e] = zn * a.scale;
               } else if constexpr (MODE == COL_DMB_UPD) {
                 float* bp = (bgn + uo) + toff;
-                float b, bb;
-                if constexpr (HBG) {
-                  b = bgh[e];
-                  bb = (a.bbar + uo)[toff];
-                } else {
-                  b = *bp;
-                  bb = (a.bbar + uo)[toff];
-                }
+                float b;
+                if constexpr (HBG) b = bgh[e];
+                else b = *bp;
+                const float bb = (a.bbar + uo)[toff];
                 const float b0 = 2.f * bb - b;
                 const float mg = msqrt(z2 + b0 * b0 + a.alpha);
                 const float fac = m * am * mrcp(mg);
@@ -889,16 +878,22 @@ __global__ __launch_bounds__(WG) void k_make_exits(Geom g, const float2* O, cons
 // partials of sum |psi'_n - O_n P|^2 (support error on an explicit pass)
 __global__ __launch_bounds__(WG) void k_esup(Geom g, const float2* S, const float2* O,
                                              const float2* P, double* part) {
+  // frame rows (n, i) in a grid-stride loop, WG / K rows at a time, threads
+  // across the row: the row / column split is one 32-bit division per row
   __shared__ double red[4];
-  const size_t MK = (size_t)g.M * g.K, total = MK * g.N;
+  const int K = g.K, M = g.M;
+  const int rpi = K <= WG ? WG / K : 1;
+  const int sub = threadIdx.x / K, j = threadIdx.x % K;
+  const int nrows = g.N * M;
   double r[1] = {0.0};
-  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
-       idx += (size_t)gridDim.x * WG) {
-    const int n = (int)(idx / MK);
-    const int p = (int)(idx % MK), i = p / g.K, j = p % g.K;
-    const int2 o = g.off[n];
-    const float2 d = S[idx] - cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[p]);
-    r[0] += (double)norm2(d);
+  if (sub < rpi) {
+    for (int row = blockIdx.x * rpi + sub; row < nrows; row += gridDim.x * rpi) {
+      const int n = row / M, i = row - n * M;
+      const int2 o = g.off[n];
+      const float2 d = S[(size_t)row * K + j] -
+                       cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[(size_t)i * K + j]);
+      r[0] += (double)norm2(d);
+    }
   }
   block_sum<1>(r, red);
   if (threadIdx.x == 0) part[blockIdx.x] = r[0];
