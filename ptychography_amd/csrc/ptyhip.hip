@@ -672,12 +672,17 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
   const bool fusedO = upd == 0;
   const bool fusedP = upd == 1 && !pmodp;
+  // object + probe step that writes O and P exactly once (one O/P pass, no
+  // centring, no far-field probe constraint): the updates read the previous
+  // O / P before overwriting them and emit the eCon partials themselves,
+  // instead of backup copies and two comparison passes
+  const bool fusedOP = op_step && o.op_n == 1 && !o.probe_centering && !pmodp;
   const bool esup_id = upd == 0 && !pmodp;
   const bool staged = !(c->use_frame && !bgm);
   // object-only step with a valid heatmap: inverse row FFT fused into the update
   const bool rowg = upd == 0 && staged && c->rowg && c->H_valid;
 
-  if (!fusedO && !fusedP) {
+  if (!fusedO && !fusedP && !fusedOP) {
     StageTimer tm(c, PTY_STAGE_MISC);
     if (upd != 1) PTY_TRY(copy_d2d(c, c->Obak, c->O, UV * sizeof(float2)));
     if (upd != 0) PTY_TRY(copy_d2d(c, c->Pbak, c->P, MK * sizeof(float2)));
@@ -734,6 +739,10 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   } else if (upd == 1) {
     PTY_TRY(probe_update(c, c->X, c->O, c->P, c->P, c->G, !c->G_valid, 1, o.alpha, fusedP));
     c->G_valid = true;
+  } else if (fusedOP) {
+    PTY_TRY(gather(c, c->X, c->P, c->O, c->O, c->H, 1, 1, o.alpha, true, false));
+    PTY_TRY(probe_update(c, c->X, c->O, c->P, c->P, c->G, 1, 1, o.alpha, true));
+    c->H_valid = c->G_valid = true;
   } else if (op_step) {
     for (int r = 0; r < o.op_n; ++r) {
       PTY_TRY(gather(c, c->X, c->P, nullptr, c->O, c->H, 1, 1, o.alpha, false, false));
@@ -766,6 +775,14 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ngather;
   } else if (fusedP) {
     const int nb = (int)((MK + WG - 1) / WG);
+    rg.p[S_ECON_PN] = c->part_econP;
+    rg.p[S_ECON_PD] = c->part_econP + nb;
+    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = nb;
+  } else if (fusedOP) {
+    const int nb = (int)((MK + WG - 1) / WG);
+    rg.p[S_ECON_ON] = c->part_econO;
+    rg.p[S_ECON_OD] = c->part_econO + c->ntiles;
+    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = c->ntiles;
     rg.p[S_ECON_PN] = c->part_econP;
     rg.p[S_ECON_PD] = c->part_econP + nb;
     rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = nb;
