@@ -439,6 +439,10 @@ size_t rowg_static_lds() {
   return sizeof(float2) * (lpw * LineLds<L>::words + 1 + L) + 16 * sizeof(double) + lpw * sizeof(int);
 }
 
+// Dynamic LDS (the row-segment accumulator) never exceeds this: rowg_segments
+// refuses layouts that would need more.
+constexpr size_t ROWG_DYN_MAX = 64u << 10;
+
 // Column segments per object row for the fused kernel: the fewest that let
 // three work-groups (else two, else one) share a CU's 160 KB of LDS, each
 // segment at least 512 columns wide; 0 = geometry not supported.
@@ -454,22 +458,24 @@ int rowg_segments(int K, int V) {
   for (int per_cu = 3; per_cu >= 1; --per_cu) {
     const size_t budget = (size_t)(160u << 10) / per_cu;
     if (budget < st + 512 * sizeof(float2)) continue;
-    const size_t w = (budget - st) / sizeof(float2);
+    const size_t w = std::min((budget - st) / sizeof(float2), ROWG_DYN_MAX / sizeof(float2));
     const int segs = (int)((V + w - 1) / w);
     if (segs <= 4) return segs;
   }
   return 0;
 }
 
-// Raise the dynamic-LDS limit of the K-sized instantiation to `width` float2.
-int rowg_attribute(int K, int width) {
-  const int bytes = width * (int)sizeof(float2);
+// Set the dynamic-LDS limit of the K-sized instantiation.  The attribute
+// belongs to the kernel, not to a context: one constant that covers every
+// segment width keeps contexts with different objects from lowering each
+// other's limit.
+int rowg_attribute(int K) {
   hipError_t e = hipErrorInvalidValue;
   switch (K) {
 #define C(Lv)                                                                                 \
   case Lv:                                                                                    \
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_gather<Lv>),                 \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);                \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ROWG_DYN_MAX);    \
     break;
     PTY_ROWG_SIZES(C)
 #undef C
@@ -1311,7 +1317,7 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   if (c->rowg_ok) {
     if ((r = dev_alloc(&c->row_ptr, rcnt.size())) != PTY_OK) return cleanup(r);
     if ((r = dev_alloc(&c->row_ent, rents.size())) != PTY_OK) return cleanup(r);
-    if ((r = rowg_attribute(K, c->rowg_segw)) != PTY_OK) return cleanup(r);
+    if ((r = rowg_attribute(K)) != PTY_OK) return cleanup(r);
     const char* e_rg = getenv("PTY_ROW_GATHER");
     c->rowg = !(e_rg && atoi(e_rg) == 0);
   }

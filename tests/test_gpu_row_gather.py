@@ -21,10 +21,11 @@ from tests.scans import broadband_start, rel_l2, small_scan
 pytestmark = pytest.mark.gpu
 
 
-def _run(s, dev, method, rowg, iters=(1, 6, 5), graphs=False, bg=None, obj_pad=0):
+def _run(s, dev, method, rowg, iters=(1, 6, 5), graphs=False, bg=None, obj_pad=0, before_run=None):
     N, M, K = s.I.shape
     U, V = s.O.shape
     eng = Engine(N, M, K, U + obj_pad, V + 2 * obj_pad, s.R.astype(np.int32), device=dev)
+    other = before_run() if before_run else None
     eng.set_path(False)
     eng.set_row_gather(rowg)
     sup, en = eng.row_gather_status()
@@ -43,6 +44,8 @@ def _run(s, dev, method, rowg, iters=(1, 6, 5), graphs=False, bg=None, obj_pad=0
         it += n
         Os.append(O_t.cpu().numpy())
     eng.close()
+    if other is not None:
+        other.close()
     return np.concatenate(rows), Os
 
 
@@ -72,6 +75,20 @@ def test_fused_segmented_rows(frame, grid, step, gpu_device):
     s = small_scan(frame=frame, grid=grid, step=step, jitter=2)
     assert s.O.shape[1] > {256: 2200, 64: 4400}[frame]
     _compare(_run(s, gpu_device, 1, True), _run(s, gpu_device, 1, False))
+
+
+def test_contexts_with_different_segment_widths(gpu_device):
+    # a context for a narrower object of the same frame size, created after
+    # this one, must not change what this one's fused kernel may use
+    print("row gather: two contexts, different segment widths", flush=True)
+    s = small_scan(frame=256, grid=(2, 16), step=130, jitter=2)      # two 1105-wide segments
+    narrow = small_scan(frame=256, grid=(2, 2), step=40, jitter=1)   # one 296-wide segment
+
+    def make_narrow():
+        N, M, K = narrow.I.shape
+        return Engine(N, M, K, *narrow.O.shape, narrow.R.astype(np.int32), device=gpu_device)
+
+    _compare(_run(s, gpu_device, 1, True, before_run=make_narrow), _run(s, gpu_device, 1, False))
 
 
 def test_fused_matches_unfused_background(gpu_device):
