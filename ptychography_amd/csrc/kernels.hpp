@@ -572,21 +572,38 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
       s_o[k] = g.off[n];
     }
     __syncthreads();
-    for (int b = 0; b < cnt; ++b) {
-      const int n = s_n[b];
-      const int2 o = s_o[b];
-      const int j = v - o.y;
-      if ((unsigned)j >= (unsigned)K) continue;
-      const float2* Sn = a.S + (size_t)n * M * K + j;
-      const float2* Pj = a.P + j;
+    // four frames per step, predicated rather than skipped, so their loads
+    // are in flight together; the sums still take the frames in order
+    constexpr int FB = 4;
+    for (int b0f = 0; b0f < cnt; b0f += FB) {
+      float2 ps[FB][PY], ss[FB][PY];
+      bool ok[FB][PY];
 #pragma unroll
-      for (int k = 0; k < PY; ++k) {
-        const int i = u0 + ty + k * (WG / TILE_W) - o.x;
-        if ((unsigned)i < (unsigned)M) {
-          const float2 p = Pj[(size_t)i * K];
-          const float2 s = Sn[(size_t)i * K];
-          acc[k] = acc[k] + cmulc(s, p);
-          if (a.fresh) hacc[k] += (double)norm2(p);
+      for (int f = 0; f < FB; ++f) {
+        const int b = min(b0f + f, cnt - 1);
+        const int n = s_n[b];
+        const int2 o = s_o[b];
+        const int j = v - o.y;
+        const bool jin = b0f + f < cnt && (unsigned)j < (unsigned)K;
+        const float2* Sn = a.S + (size_t)n * M * K + (jin ? j : 0);
+        const float2* Pj = a.P + (jin ? j : 0);
+#pragma unroll
+        for (int k = 0; k < PY; ++k) {
+          const int i = u0 + ty + k * (WG / TILE_W) - o.x;
+          ok[f][k] = jin && (unsigned)i < (unsigned)M;
+          const size_t ro = ok[f][k] ? (size_t)i * K : 0;
+          ps[f][k] = ok[f][k] ? Pj[ro] : make_float2(0.f, 0.f);
+          ss[f][k] = ok[f][k] ? Sn[ro] : make_float2(0.f, 0.f);
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < FB; ++f) {
+#pragma unroll
+        for (int k = 0; k < PY; ++k) {
+          if (ok[f][k]) {
+            acc[k] = acc[k] + cmulc(ss[f][k], ps[f][k]);
+            if (a.fresh) hacc[k] += (double)norm2(ps[f][k]);
+          }
         }
       }
     }
