@@ -10,9 +10,13 @@ through properties that do not need the oracle (too slow at this size):
   the distance eMod between the two sets cannot grow from one iteration to
   the next (Fienup's error-reduction property), whatever the noise;
 * determinism: two runs from the same start are bit-identical;
-* the fused inverse-row / object-update kernel against the unfused kernels.
+* the fused inverse-row / object-update kernel against the unfused kernels;
+* DM: from the true object and probe on noise-free data the DM iterate f is
+  consistent and satisfies the modulus constraint, so f, O and P stay put;
+  DM with background refinement (method 4) on the noisy data is
+  bit-identical between two runs and its metrics stay finite.
 
-Bars: eMod at the fixed point <= 2e-5 and object change <= 1e-5 (rel L2 on
+Bars: eMod at the fixed point <= 2e-5 (ERA and DM) and object change <= 1e-5 (rel L2 on
 pixels whose heatmap is >= 1e-3 of its maximum); eMod non-increasing within
 a relative 1e-6 per step; fused vs unfused object rel L2 <= 1e-5 and eMod
 within 1e-5 relative after 8 iterations.
@@ -142,3 +146,42 @@ def test_c3_frame_fft_against_torch(gpu_device):
     eng.close()
     del x, y, z, ref
     torch.cuda.empty_cache()
+
+
+def test_c3_dm_fixed_point(gpu_device):
+    print("full size: DM fixed point", flush=True)
+    spec = sim.RECIPES["c3"].with_(background_frac=0.0)
+    scan = sim.make_scan_gpu(spec, gpu_device, poisson=False)
+    eng, keep = _engine(scan, gpu_device)
+    O_t, P_t = _state(eng, gpu_device, scan["O"], scan["P"])
+    m = eng.run(1, 3, 3, begin=True)
+    print("DM eMod at the true object and probe:", m[:, 0])
+    assert np.all(m[:, 0] <= 2e-5)
+    H = eng.heatmap()
+    lit = (H >= 1e-3 * H.max()).cpu().numpy() if H is not None else np.ones(scan["O"].shape, bool)
+    eo = rel_l2(O_t.cpu().numpy()[lit], scan["O"][lit])
+    ep = rel_l2(P_t.cpu().numpy(), scan["P"])
+    print(f"object change {eo:.2e}, probe change {ep:.2e}")
+    assert eo <= 1e-5 and ep <= 1e-5
+    eng.close()
+    del keep, O_t, P_t, scan
+    torch.cuda.empty_cache()
+
+
+def test_c3_dm_background_deterministic(c3_scan, gpu_device):
+    print("full size: DM method 4 bit-identical reruns", flush=True)
+    eng, keep = _engine(c3_scan, gpu_device)
+    b0 = torch.from_numpy(np.sqrt(np.fft.ifftshift(c3_scan["B"])).astype(np.float32))
+    O0 = np.ones(c3_scan["O"].shape, np.complex64)
+    runs = []
+    for rep in range(2):
+        O_t, P_t = _state(eng, gpu_device, O0, c3_scan["P"])
+        eng.set_background(b0, per_frame=False)
+        m = eng.run(1, 4, 6, begin=True)
+        runs.append((m, O_t.cpu().numpy(), eng.background().cpu().numpy()))
+    print("DM-bg eMod:", " ".join(f"{x:.4e}" for x in runs[0][0][:, 0]))
+    assert np.all(np.isfinite(runs[0][0]))
+    for x, y in zip(runs[0], runs[1]):
+        assert np.array_equal(x, y)
+    eng.close()
+    del keep
