@@ -171,16 +171,19 @@ __device__ __forceinline__ void dft(float2* v) {
   }
 }
 
-// Line-FFT plan for one length: L = R1 * R2, T = threads per line.
+// Line-FFT plan for one length: L = R1 * R2 (* R3 for three passes), T =
+// threads per line.  96 and 192 take three passes so that a thread holds 12
+// points instead of 24 (the two-pass radix-24 / radix-12 x 8 plans needed
+// ~250 VGPRs: 2 waves per SIMD).
 template <int L> struct Plan;
-template <> struct Plan<16>  { static constexpr int R1 = 4,  R2 = 4,  T = 4; };
-template <> struct Plan<32>  { static constexpr int R1 = 8,  R2 = 4,  T = 4; };
-template <> struct Plan<48>  { static constexpr int R1 = 12, R2 = 4,  T = 4; };
-template <> struct Plan<64>  { static constexpr int R1 = 8,  R2 = 8,  T = 8; };
-template <> struct Plan<96>  { static constexpr int R1 = 12, R2 = 8,  T = 4; };
-template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  T = 8; };
-template <> struct Plan<192> { static constexpr int R1 = 24, R2 = 8,  T = 8; };
-template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, T = 16; };
+template <> struct Plan<16>  { static constexpr int R1 = 4,  R2 = 4,  R3 = 1, T = 4; };
+template <> struct Plan<32>  { static constexpr int R1 = 8,  R2 = 4,  R3 = 1, T = 4; };
+template <> struct Plan<48>  { static constexpr int R1 = 12, R2 = 4,  R3 = 1, T = 4; };
+template <> struct Plan<64>  { static constexpr int R1 = 8,  R2 = 8,  R3 = 1, T = 8; };
+template <> struct Plan<96>  { static constexpr int R1 = 6,  R2 = 4,  R3 = 4, T = 8; };
+template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  R3 = 1, T = 8; };
+template <> struct Plan<192> { static constexpr int R1 = 12, R2 = 4,  R3 = 4, T = 16; };
+template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, R3 = 1, T = 16; };
 
 // Element index held in register slot e by thread t, for layoutA / layoutB.
 template <int Ra, int Rb, int T>
@@ -188,11 +191,12 @@ __device__ __forceinline__ int idxA(int e, int t) { return (t + (e / Ra) * T) + 
 template <int Ra, int Rb, int T>
 __device__ __forceinline__ int idxB(int e, int t) { return (t + (e / Rb) * T) + (e % Rb) * Ra; }
 
-// LDS words (float2) one line needs for its exchange buffer (both orders).
+// LDS words (float2) one line needs for its exchange buffer (both directions).
 template <int L>
 struct LineLds {
-  static constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2;
-  static constexpr int a = R2 * (R1 + 1), b = R1 * (R2 + 1);
+  static constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, R3 = Plan<L>::R3;
+  static constexpr int a = R3 > 1 ? R1 * (R2 * R3 + 1) : R2 * (R1 + 1);
+  static constexpr int b = R3 > 1 ? R3 * (R1 * R2 + 1) : R1 * (R2 + 1);
   static constexpr int words = (a > b ? a : b) | 1;  // odd stride between lines
 };
 
@@ -230,5 +234,119 @@ __device__ __forceinline__ void line_fft(float2* v, float2* mid, const float2* _
   });
   static_assert(NQ1 * Ra == E && NQ2 * Rb == E, "layout size mismatch");
 }
+
+// Three-pass transform of a line of L = Ra * Rb * Rc points held in registers
+// (E = L / T per thread).  With N2 = Rb * Rc and N3 = Ra * Rb:
+//   pass 1: radix-Ra DFTs over x[n2 + N2 r] for n2 = t + q T, times W_L^(n2 k1)
+//   pass 2: for each (k1, j) = (g % Ra, g / Ra), g = t + q T: radix-Rb DFTs over
+//           y[k1][j + Rc m], times W_L^(Ra j kb)          (the N2-point DFT of
+//   pass 3: for each b = k1 + Ra kb = t + q T: radix-Rc     row k1, split Rb x Rc)
+//           DFTs over j, giving X[b + N3 kc]
+// In: slot e = q Ra + r <-> element t + q T + N2 r; out: slot e = q Rc + kc <->
+// element t + q T + N3 kc.  The inverse is the same network with the radices
+// reversed (input in the forward's output layout, output in its input layout).
+// Three LDS barriers; the caller puts one between two calls that reuse `mid`.
+template <int L, int Ra, int Rb, int Rc, int T, bool INV>
+__device__ __forceinline__ void line_fft3(float2* v, float2* mid, const float2* __restrict__ tw,
+                                          int t) {
+  static_assert(Ra * Rb * Rc == L, "bad factorisation");
+  constexpr int N2 = Rb * Rc, N3 = Ra * Rb;
+  static_assert(N2 % T == 0 && (Ra * Rc) % T == 0 && N3 % T == 0, "T must divide every pass");
+  constexpr int E = L / T, NQ1 = N2 / T, NQ2 = (Ra * Rc) / T, NQ3 = N3 / T;
+  static_assert(NQ1 * Ra == E && NQ2 * Rb == E && NQ3 * Rc == E, "layout size mismatch");
+  auto twid = [&](float2 a, int k) {
+    float2 w = tw[k];
+    if constexpr (INV) w.y = -w.y;
+    return cmul(a, w);
+  };
+  static_for<0, NQ1>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    dft<Ra, INV>(v + q * Ra);
+    const int n2 = t + q * T;
+#pragma unroll
+    for (int k1 = 0; k1 < Ra; ++k1)
+      mid[k1 * (N2 + 1) + n2] = k1 == 0 ? v[q * Ra] : twid(v[q * Ra + k1], n2 * k1);
+  });
+  lds_barrier();
+  static_for<0, NQ2>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const int g = t + q * T, k1 = g % Ra, j = g / Ra;
+#pragma unroll
+    for (int m = 0; m < Rb; ++m) v[q * Rb + m] = mid[k1 * (N2 + 1) + j + Rc * m];
+  });
+  lds_barrier();
+  static_for<0, NQ2>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const int g = t + q * T, k1 = g % Ra, j = g / Ra;
+    dft<Rb, INV>(v + q * Rb);
+#pragma unroll
+    for (int kb = 0; kb < Rb; ++kb)
+      mid[j * (N3 + 1) + k1 + Ra * kb] = kb == 0 ? v[q * Rb] : twid(v[q * Rb + kb], Ra * j * kb);
+  });
+  lds_barrier();
+  static_for<0, NQ3>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    const int b = t + q * T;
+#pragma unroll
+    for (int j = 0; j < Rc; ++j) v[q * Rc + j] = mid[j * (N3 + 1) + b];
+    dft<Rc, INV>(v + q * Rc);
+  });
+}
+
+// Plan-level transforms and register layouts used by the kernels.  Layout A
+// (forward input, inverse output): slot e of thread t holds element
+// t + (e / R1) T + (e % R1) (L / R1); layout B (forward output, inverse input):
+// the same with the last radix.  Both are "thread index + per-slot constant".
+template <int L>
+__device__ __forceinline__ void fft_fwd(float2* v, float2* mid, const float2* __restrict__ tw, int t) {
+  using Pl = Plan<L>;
+  if constexpr (Pl::R3 > 1) line_fft3<L, Pl::R1, Pl::R2, Pl::R3, Pl::T, false>(v, mid, tw, t);
+  else line_fft<L, Pl::R1, Pl::R2, Pl::T, false>(v, mid, tw, t);
+}
+template <int L>
+__device__ __forceinline__ void fft_inv(float2* v, float2* mid, const float2* __restrict__ tw, int t) {
+  using Pl = Plan<L>;
+  if constexpr (Pl::R3 > 1) line_fft3<L, Pl::R3, Pl::R2, Pl::R1, Pl::T, true>(v, mid, tw, t);
+  else line_fft<L, Pl::R2, Pl::R1, Pl::T, true>(v, mid, tw, t);
+}
+template <int L> struct Lay {
+  static constexpr int T = Plan<L>::T, RA = Plan<L>::R1;
+  static constexpr int RB = Plan<L>::R3 > 1 ? Plan<L>::R3 : Plan<L>::R2;
+};
+template <int L>
+__host__ __device__ __forceinline__ constexpr int rowA(int e) {
+  return (e / Lay<L>::RA) * Lay<L>::T + (e % Lay<L>::RA) * (L / Lay<L>::RA);
+}
+template <int L>
+__host__ __device__ __forceinline__ constexpr int rowB(int e) {
+  return (e / Lay<L>::RB) * Lay<L>::T + (e % Lay<L>::RB) * (L / Lay<L>::RB);
+}
+template <int L>
+__host__ __device__ __forceinline__ constexpr int layA(int e, int t) { return t + rowA<L>(e); }
+template <int L>
+__host__ __device__ __forceinline__ constexpr int layB(int e, int t) { return t + rowB<L>(e); }
+
+// Compile-time checks: for two-pass plans the layouts are the two-pass
+// Stockham ones (idxA / idxB of R1, R2), and every layout maps the T x E
+// (thread, slot) pairs one-to-one onto 0 .. L-1.
+template <int L>
+constexpr bool layouts_ok() {
+  constexpr int T = Plan<L>::T, E = L / T, R1 = Plan<L>::R1, R2 = Plan<L>::R2;
+  bool seenA[L] = {}, seenB[L] = {};
+  for (int t = 0; t < T; ++t)
+    for (int e = 0; e < E; ++e) {
+      const int a = layA<L>(e, t), b = layB<L>(e, t);
+      if (a < 0 || a >= L || b < 0 || b >= L || seenA[a] || seenB[b]) return false;
+      seenA[a] = seenB[b] = true;
+      if (Plan<L>::R3 == 1) {
+        if (a != (t + (e / R1) * T) + (e % R1) * R2) return false;
+        if (b != (t + (e / R2) * T) + (e % R2) * R1) return false;
+      }
+    }
+  return true;
+}
+static_assert(layouts_ok<16>() && layouts_ok<32>() && layouts_ok<48>() && layouts_ok<64>() &&
+                  layouts_ok<96>() && layouts_ok<128>() && layouts_ok<192>() && layouts_ok<256>(),
+              "line layouts");
 
 }  // namespace pty

@@ -91,7 +91,7 @@ template <int L, int SRC>
 __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, const float2* P,
                                                  const float2* F, float2* X,
                                                  const float2* __restrict__ tw, int nrows) {
-  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
+  constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
   constexpr int E2 = (SRC == SRC_DM) ? E : 1;  // f values (DM)
   __shared__ float2 lds[LPW * W];
@@ -108,14 +108,14 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
     const size_t rbase = (size_t)row * L;
     if constexpr (SRC == SRC_F) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) n0[e] = load_stream(&F[rbase + idxA<R1, R2, T>(e, t)]);
+      for (int e = 0; e < E; ++e) n0[e] = load_stream(&F[rbase + layA<L>(e, t)]);
     } else {
       const int n = row / M, i = row % M;
       const int2 o = g.off[n];
       const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const int c = idxA<R1, R2, T>(e, t);
+        const int c = layA<L>(e, t);
         n0[e] = Orow[c];
         if constexpr (SRC == SRC_DM) n2[e] = load_stream(&F[rbase + c]);
       }
@@ -133,18 +133,18 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
       const float2* Prow = P + (size_t)i * L;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        float2 psi = cmul(n0[e], Prow[idxA<R1, R2, T>(e, t)]);
+        float2 psi = cmul(n0[e], Prow[layA<L>(e, t)]);
         if constexpr (SRC == SRC_DM) psi = make_float2(2.f * psi.x - n2[e].x, 2.f * psi.y - n2[e].y);
         v[e] = psi;
       }
     }
     if (grp + (int)gridDim.x < ngroups) load(grp + gridDim.x);
-    line_fft<L, R1, R2, T, false>(v, lds + line * W, s_tw, t);
+    fft_fwd<L>(v, lds + line * W, s_tw, t);
     const int row = grp * LPW + line;
     if (row < nrows) {
       const size_t rbase = (size_t)row * L;
 #pragma unroll
-      for (int e = 0; e < E; ++e) store_stream(&X[rbase + idxB<R1, R2, T>(e, t)], v[e]);
+      for (int e = 0; e < E; ++e) store_stream(&X[rbase + layB<L>(e, t)], v[e]);
     }
     lds_barrier();
   }
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
                                                  const float2* O, const float2* P, float scale,
                                                  double* partial, const float2* __restrict__ tw,
                                                  int nrows) {
-  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
+  constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[LPW * W];
   __shared__ float2 s_tw[L];
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
     const int row = min(grp * LPW + line, nrows - 1);
     const size_t rbase = (size_t)row * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nx[e] = load_stream(&Y[rbase + idxA<R2, R1, T>(e, t)]);
+    for (int e = 0; e < E; ++e) nx[e] = load_stream(&Y[rbase + layB<L>(e, t)]);
   };
   double acc[1] = {0.0};
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -187,13 +187,13 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (grp + (int)gridDim.x < ngroups) load(grp + gridDim.x);
-    line_fft<L, R2, R1, T, true>(v, lds + line * W, s_tw, t);
+    fft_inv<L>(v, lds + line * W, s_tw, t);
     const int row = grp * LPW + line;
     if (row < nrows) {
       const size_t rbase = (size_t)row * L;
       if constexpr (DST == DST_STORE) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) store_stream(&Out[rbase + idxB<R2, R1, T>(e, t)], v[e] * scale);
+        for (int e = 0; e < E; ++e) store_stream(&Out[rbase + layA<L>(e, t)], v[e] * scale);
       } else {
         const int M = g.M, n = row / M, i = row % M;
         const int2 o = g.off[n];
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
         const float2* Prow = P + (size_t)i * L;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const int c = idxB<R2, R1, T>(e, t);
+          const int c = layA<L>(e, t);
           const float2 ex = cmul(Orow[c], Prow[c]);
           const float2 y = v[e] * scale;
           const float2 d = y - ex;
@@ -314,7 +314,7 @@ __device__ __forceinline__ constexpr int urowB(int e) { return (e / Rb) * T + (e
 template <int L>
 __global__ __launch_bounds__(WG) void k_pack_mask(const uint8_t* __restrict__ mask, int K,
                                                   uint32_t* __restrict__ bits) {
-  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T;
+  constexpr int T = Plan<L>::T, E = L / T;
   static_assert(E <= 32, "one bit per register slot");
   const int idx = blockIdx.x * WG + threadIdx.x;
   if (idx >= T * K) return;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(WG) void k_pack_mask(const uint8_t* __restrict__ ma
   uint32_t b = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    if (mask[(size_t)(t + urowB<R1, R2, T>(e)) * K + c]) b |= 1u << e;
+    if (mask[(size_t)(t + rowB<L>(e)) * K + c]) b |= 1u << e;
   bits[idx] = b;
 }
 
@@ -335,7 +335,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
   // the same for every frame (an L2 hit) and is read where it is used, which
   // keeps DMB_UPD and ERR7 within their register budget
   constexpr bool HBG = HOIST && (MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD);
-  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, CPW = CWG / T;
+  constexpr int T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
   __shared__ float2 s_tw[L];
@@ -356,7 +356,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     const float2* Xn = a.X + (size_t)n * L * K;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int u = TR::fwd ? urowA<R1, R2, T>(e) : urowB<R1, R2, T>(e);
+      const int u = TR::fwd ? rowA<L>(e) : rowB<L>(e);
       nx[e] = load_stream(&(Xn + (size_t)u * K)[toff]);
     }
   };
@@ -389,11 +389,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         const float* ampn = a.amp + (size_t)nu * a.amp_stride;
 #pragma unroll
         for (int e = 0; e < E; ++e)
-          amh[e] = cv ? load_stream(&(ampn + (size_t)urowB<R1, R2, T>(e) * K)[toff]) : 0.f;
+          amh[e] = cv ? load_stream(&(ampn + (size_t)rowB<L>(e) * K)[toff]) : 0.f;
         if constexpr (HBG) {
           const float* bgn = a.bg + (size_t)nu * a.bg_stride;
 #pragma unroll
-          for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)urowB<R1, R2, T>(e) * K)[toff] : 0.f;
+          for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)rowB<L>(e) * K)[toff] : 0.f;
         }
       }
 #pragma unroll
@@ -407,7 +407,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           load(ni, (ni / ncb) * a.chunk);
         }
       }
-      if constexpr (TR::fwd) line_fft<L, R1, R2, T, false>(v, mid, s_tw, t);
+      if constexpr (TR::fwd) fft_fwd<L>(v, mid, s_tw, t);
 
       if constexpr (MODE != COL_FWD && MODE != COL_INV) {
         if (cv) {
@@ -415,7 +415,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           float* bgn = a.bg ? a.bg + (size_t)nu * a.bg_stride : nullptr;
 #pragma unroll
           for (int e = 0; e < E; ++e) {
-            const int u = urowB<R1, R2, T>(e);
+            const int u = rowB<L>(e);
             const size_t uo = (size_t)u * K;
             const float2 z = v[e];
             const float z2 = norm2(z);
@@ -488,7 +488,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       if constexpr (TR::store_fwd) {
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (Xn + (size_t)urowB<R1, R2, T>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) (Xn + (size_t)rowB<L>(e) * K)[toff] = v[e];
         }
       }
       if constexpr (TR::inv) {
@@ -497,10 +497,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
 #pragma unroll
           for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
         }
-        line_fft<L, R2, R1, T, true>(v, mid, s_tw, t);
+        fft_inv<L>(v, mid, s_tw, t);
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (Xn + (size_t)urowA<R1, R2, T>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) (Xn + (size_t)rowA<L>(e) * K)[toff] = v[e];
         }
       }
       lds_barrier();
@@ -510,7 +510,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       if (cv) {
         double* pb = a.partb + (size_t)chunk * L * K;
 #pragma unroll
-        for (int e = 0; e < E; ++e) (pb + (size_t)urowB<R1, R2, T>(e) * K)[toff] = (double)bacc[e];
+        for (int e = 0; e < E; ++e) (pb + (size_t)rowB<L>(e) * K)[toff] = (double)bacc[e];
       }
     }
   }
@@ -684,7 +684,7 @@ static_assert(ROWG_VMAX < (1 << 15), "x_n << 16 must stay a positive int");
 template <int L>
 __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
                                                     const float2* __restrict__ tw) {
-  constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, T = Plan<L>::T, E = L / T, LPW = WG / T;
+  constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
   constexpr int OFF_NONE = -(1 << 29);  // column offset of a spare line: matches no column
   static_assert(W >= L, "a line's exchange area must hold the parked row");
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
   auto load = [&](int2 m) {
     const size_t rbase = (size_t)m.x * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nx[e] = a.Y[rbase + idxA<R2, R1, T>(e, t)];
+    for (int e = 0; e < E; ++e) nx[e] = a.Y[rbase + layB<L>(e, t)];
   };
   int2 m_cur = make_int2(0, 0), m_next = make_int2(0, 0);  // .y = x_n << 16 | i
   if (nbatch > 0) {
@@ -721,18 +721,18 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
     const float2* Prow = a.P + (size_t)i * L;
     float2 pv[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) pv[e] = Prow[idxB<R2, R1, T>(e, t)];
+    for (int e = 0; e < E; ++e) pv[e] = Prow[layA<L>(e, t)];
     const int2 m_after = entry(bt + 2);
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (bt + 1 < nbatch) load(m_next);
-    line_fft<L, R2, R1, T, true>(v, lds + line * W, s_tw, t);
+    fft_inv<L>(v, lds + line * W, s_tw, t);
     // park conj(P) * psi' in this line's area: every value depends on the
     // pass-2 reads of the same wave, so no barrier is needed before it
     float2* park = lds + line * W;
 #pragma unroll
-    for (int e = 0; e < E; ++e) park[idxB<R2, R1, T>(e, t)] = cmulc(v[e], pv[e]);
+    for (int e = 0; e < E; ++e) park[layA<L>(e, t)] = cmulc(v[e], pv[e]);
     if (t == 0) s_oy[line] = (bt * LPW + line < cnt) ? (m_cur.y >> 16) : OFF_NONE;
     lds_barrier();
     int oy[LPW];

@@ -431,7 +431,7 @@ int gather(pty_ctx* c, const float2* S, const float2* P, const float2* Oold, flo
 }
 
 // frame sides the fused inverse-row / object-update kernel is built for
-#define PTY_ROWG_SIZES(X) X(64) X(128) X(192) X(256)
+#define PTY_ROWG_SIZES(X) X(64) X(96) X(128) X(192) X(256)
 
 template <int L>
 size_t rowg_static_lds() {

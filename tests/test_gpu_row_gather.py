@@ -60,7 +60,8 @@ def _compare(a, b):
 
 
 @pytest.mark.parametrize("frame,grid,step,jitter", [
-    (64, (5, 5), 12, 1), (128, (3, 4), 40, 3), (192, (2, 3), 48, 2), (256, (3, 3), 40, 3)])
+    (64, (5, 5), 12, 1), (96, (4, 4), 20, 2), (128, (3, 4), 40, 3), (192, (2, 3), 48, 2),
+    (256, (3, 3), 40, 3)])
 def test_fused_matches_unfused_method1(frame, grid, step, jitter, gpu_device):
     print(f"row gather method 1 frame {frame}", flush=True)
     s = small_scan(frame=frame, grid=grid, step=step, jitter=jitter)
