@@ -839,13 +839,17 @@ struct ProbeFinalArgs {
   const float2* Pold; // for the fused eCon (null = skip)
   float2* Pout;
   double* part_econ;  // [2][nblocks]
+  double* part_esup;  // [2][nblocks]  sum Re(conj(P) pnum), sum |P|^2 G (null = skip)
 };
 
+// With part_esup, also the two probe-side terms of the eSup identity
+// sum |psi' - P O|^2 = sum |psi'|^2 - 2 Re sum conj(P O) psi' + sum |P O|^2:
+// pnum and G hold sum_n conj(O_n) psi'_n and sum_n |O_n|^2 of the final O.
 __global__ __launch_bounds__(WG) void k_probe_final(Geom g, ProbeFinalArgs a) {
-  __shared__ double red[8];
+  __shared__ double red[16];
   const int MK = g.M * g.K;
   const int p = blockIdx.x * WG + threadIdx.x;
-  double r[2] = {0.0, 0.0};
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
   if (p < MK) {
     double nx = 0.0, ny = 0.0, gg = 0.0;
     for (int c = 0; c < a.nchunks; ++c) {
@@ -864,13 +868,23 @@ __global__ __launch_bounds__(WG) void k_probe_final(Geom g, ProbeFinalArgs a) {
       r[0] = dx * dx + dy * dy;
       r[1] = (double)pn.x * pn.x + (double)pn.y * pn.y;
     }
+    if (a.part_esup) {
+      r[2] = (double)pn.x * nx + (double)pn.y * ny;
+      r[3] = ((double)pn.x * pn.x + (double)pn.y * pn.y) * gg;
+    }
     a.Pout[p] = pn;
   }
-  if (a.part_econ) {
-    block_sum<2>(r, red);
+  if (a.part_econ || a.part_esup) {
+    block_sum<4>(r, red);
     if (threadIdx.x == 0) {
-      a.part_econ[blockIdx.x] = r[0];
-      a.part_econ[gridDim.x + blockIdx.x] = r[1];
+      if (a.part_econ) {
+        a.part_econ[blockIdx.x] = r[0];
+        a.part_econ[gridDim.x + blockIdx.x] = r[1];
+      }
+      if (a.part_esup) {
+        a.part_esup[blockIdx.x] = r[2];
+        a.part_esup[gridDim.x + blockIdx.x] = r[3];
+      }
     }
   }
 }
@@ -895,15 +909,16 @@ __global__ __launch_bounds__(WG) void k_make_exits(Geom g, const float2* O, cons
 // partials of sum |psi'_n - O_n P|^2 (support error on an explicit pass)
 __global__ __launch_bounds__(WG) void k_esup(Geom g, const float2* S, const float2* O,
                                              const float2* P, double* part) {
-  // frame rows (n, i) in a grid-stride loop, WG / K rows at a time, threads
-  // across the row: the row / column split is one 32-bit division per row
+  // frame rows (n, i) in a grid-stride loop, blockDim.x / K rows at a time
+  // (the host launches a multiple of K threads), threads across the row: the
+  // row / column split is one 32-bit division per row
   __shared__ double red[4];
   const int K = g.K, M = g.M;
-  const int rpi = K <= WG ? WG / K : 1;
+  const int rpi = (int)blockDim.x / K;
   const int sub = threadIdx.x / K, j = threadIdx.x % K;
   const int nrows = g.N * M;
   double r[1] = {0.0};
-  if (sub < rpi) {
+  {
     for (int row = blockIdx.x * rpi + sub; row < nrows; row += gridDim.x * rpi) {
       const int n = row / M, i = row - n * M;
       const int2 o = g.off[n];

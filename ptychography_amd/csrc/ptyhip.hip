@@ -520,7 +520,7 @@ int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, 
 }
 
 int probe_update(pty_ctx* c, const float2* S, const float2* O, const float2* Pold, float2* Pout,
-                 double* G, int fresh, int storeG, float alpha, bool econ) {
+                 double* G, int fresh, int storeG, float alpha, bool econ, bool esup = false) {
   StageTimer tm(c, PTY_STAGE_PROBE);
   const int MK = (int)c->MK();
   ProbeArgs a{};
@@ -544,6 +544,7 @@ int probe_update(pty_ctx* c, const float2* S, const float2* O, const float2* Pol
   f.Pold = econ ? Pold : nullptr;
   f.Pout = Pout;
   f.part_econ = econ ? c->part_econP : nullptr;
+  f.part_esup = esup ? c->part_esup : nullptr;
   hipLaunchKernelGGL(k_probe_final, dim3((MK + WG - 1) / WG), dim3(WG), 0, c->stream, c->g, f);
   PTY_LAUNCHED("k_probe_final");
   return PTY_OK;
@@ -683,7 +684,11 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   // O / P before overwriting them and emit the eCon partials themselves,
   // instead of backup copies and two comparison passes
   const bool fusedOP = op_step && o.op_n == 1 && !o.probe_centering && !pmodp;
-  const bool esup_id = upd == 0 && !pmodp;
+  // eSup from the identity: object-only steps take its terms from the object
+  // update, object + probe steps whose O and P are final after the probe
+  // update (no centring, no far-field constraint) from the probe update
+  const bool esup_probe = op_step && !o.probe_centering && !pmodp;
+  const bool esup_id = (upd == 0 && !pmodp) || esup_probe;
   const bool staged = !(c->use_frame && !bgm);
   // object-only step with a valid heatmap: inverse row FFT fused into the update
   const bool rowg = upd == 0 && staged && c->rowg && c->H_valid;
@@ -747,12 +752,13 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     c->G_valid = true;
   } else if (fusedOP) {
     PTY_TRY(gather(c, c->X, c->P, c->O, c->O, c->H, 1, 1, o.alpha, true, false));
-    PTY_TRY(probe_update(c, c->X, c->O, c->P, c->P, c->G, 1, 1, o.alpha, true));
+    PTY_TRY(probe_update(c, c->X, c->O, c->P, c->P, c->G, 1, 1, o.alpha, true, esup_probe));
     c->H_valid = c->G_valid = true;
   } else if (op_step) {
     for (int r = 0; r < o.op_n; ++r) {
       PTY_TRY(gather(c, c->X, c->P, nullptr, c->O, c->H, 1, 1, o.alpha, false, false));
-      PTY_TRY(probe_update(c, c->X, c->O, nullptr, c->P, c->G, 1, 1, o.alpha, false));
+      PTY_TRY(probe_update(c, c->X, c->O, nullptr, c->P, c->G, 1, 1, o.alpha, false,
+                           esup_probe && r == o.op_n - 1));
       c->H_valid = c->G_valid = true;
     }
     if (o.probe_centering) {
@@ -811,13 +817,22 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     esup_mode = ESUP_IDENTITY;
     rg.p[S_ESUP_A] = c->part_esa;
     rg.n[S_ESUP_A] = ncol;
-    rg.p[S_ESUP_X] = c->part_esxh;
-    rg.p[S_ESUP_H] = c->part_esxh + ngather;
-    rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = ngather;
+    if (esup_probe) {
+      const int nb = (int)((MK + WG - 1) / WG);
+      rg.p[S_ESUP_X] = c->part_esup;
+      rg.p[S_ESUP_H] = c->part_esup + nb;
+      rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = nb;
+    } else {
+      rg.p[S_ESUP_X] = c->part_esxh;
+      rg.p[S_ESUP_H] = c->part_esxh + ngather;
+      rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = ngather;
+    }
   } else {
     esup_mode = ESUP_EXPLICIT;
     StageTimer tm(c, PTY_STAGE_REDUCE);
-    hipLaunchKernelGGL(k_esup, dim3(ESUP_BLOCKS), dim3(WG), 0, c->stream, c->g, (const float2*)c->X,
+    // a whole number of frame rows per work-group, every thread busy
+    const int esup_threads = (WG / c->K) * c->K;
+    hipLaunchKernelGGL(k_esup, dim3(ESUP_BLOCKS), dim3(esup_threads), 0, c->stream, c->g, (const float2*)c->X,
                        (const float2*)c->O, (const float2*)c->P, c->part_esup);
     PTY_LAUNCHED("k_esup");
     rg.p[S_ESUP_E] = c->part_esup;
