@@ -816,14 +816,26 @@ __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
   const int i = p / g.K, j = p % g.K;
   const int n0 = blockIdx.y * a.chunk, n1 = min(g.N, n0 + a.chunk);
   double nx = 0.0, ny = 0.0, gg = 0.0;
-  for (int n = n0; n < n1; ++n) {
-    const int2 o = g.off[n];
-    const float2 ov = a.O[(size_t)(o.x + i) * g.V + o.y + j];
-    const float2 s = a.S[(size_t)n * MK + p];
-    const float2 q = cmulc(s, ov);
-    nx += (double)q.x;
-    ny += (double)q.y;
-    if (a.fresh) gg += (double)norm2(ov);
+  // four frames' loads in flight per step; the sums take the frames in order
+  constexpr int FB = 4;
+  for (int nb = n0; nb < n1; nb += FB) {
+    float2 ov[FB], sv[FB];
+#pragma unroll
+    for (int f = 0; f < FB; ++f) {
+      const int n = min(nb + f, n1 - 1);
+      const int2 o = g.off[n];
+      ov[f] = a.O[(size_t)(o.x + i) * g.V + o.y + j];
+      sv[f] = a.S[(size_t)n * MK + p];
+    }
+#pragma unroll
+    for (int f = 0; f < FB; ++f) {
+      if (nb + f < n1) {
+        const float2 q = cmulc(sv[f], ov[f]);
+        nx += (double)q.x;
+        ny += (double)q.y;
+        if (a.fresh) gg += (double)norm2(ov[f]);
+      }
+    }
   }
   a.pnum[(size_t)blockIdx.y * MK + p] = make_double2(nx, ny);
   if (a.fresh) a.pg[(size_t)blockIdx.y * MK + p] = gg;
@@ -852,11 +864,24 @@ __global__ __launch_bounds__(WG) void k_probe_final(Geom g, ProbeFinalArgs a) {
   double r[4] = {0.0, 0.0, 0.0, 0.0};
   if (p < MK) {
     double nx = 0.0, ny = 0.0, gg = 0.0;
-    for (int c = 0; c < a.nchunks; ++c) {
-      const double2 q = a.pnum[(size_t)c * MK + p];
-      nx += q.x;
-      ny += q.y;
-      if (a.fresh) gg += a.pg[(size_t)c * MK + p];
+    constexpr int CB = 4;  // chunks' loads in flight per step, summed in order
+    for (int cb = 0; cb < a.nchunks; cb += CB) {
+      double2 q[CB];
+      double pgv[CB];
+#pragma unroll
+      for (int f = 0; f < CB; ++f) {
+        const int c = min(cb + f, a.nchunks - 1);
+        q[f] = a.pnum[(size_t)c * MK + p];
+        pgv[f] = a.fresh ? a.pg[(size_t)c * MK + p] : 0.0;
+      }
+#pragma unroll
+      for (int f = 0; f < CB; ++f) {
+        if (cb + f < a.nchunks) {
+          nx += q[f].x;
+          ny += q[f].y;
+          if (a.fresh) gg += pgv[f];
+        }
+      }
     }
     if (!a.fresh) gg = a.G[p];
     else if (a.storeG) a.G[p] = gg;
