@@ -359,17 +359,17 @@ def overlap_pairs(off, M, K, frac=0.75):
     N = len(off)
     ty, tx = int(frac * M), int(frac * K)
     order = np.argsort(off[:, 0], kind="stable")
-    ys = off[order, 0]
+    ys, xs = off[order, 0], off[order, 1]
     lo = np.arange(1, N + 1)                             # later in row order only
     hi = np.searchsorted(ys, ys + ty, side="left")      # first y_j >= y_i + ty
     counts = np.maximum(hi - lo, 0)
-    a = np.repeat(np.arange(N), counts)
-    b = np.arange(counts.sum()) - np.repeat(np.cumsum(counts) - counts, counts) + np.repeat(lo, counts)
-    keep = np.abs(off[order[b], 1] - off[order[a], 1]) < tx
+    a = np.repeat(np.arange(N), counts)                 # candidate (a, b), both in row order
+    b = np.arange(len(a)) - np.repeat(np.cumsum(counts) - counts - lo, counts)
+    keep = np.abs(xs[b] - xs[a]) < tx
     i, j = order[a[keep]], order[b[keep]]
-    pairs = np.stack([np.minimum(i, j), np.maximum(i, j)], 1)
-    pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
-    return pairs.astype(np.int32)
+    lo_, hi_ = np.minimum(i, j), np.maximum(i, j)
+    srt = np.argsort(lo_ * N + hi_)                     # unique keys: (i, j) lexicographic
+    return np.stack([lo_[srt], hi_[srt]], 1).astype(np.int32)
 
 
 def pair_csr(pairs, N):
@@ -379,8 +379,10 @@ def pair_csr(pairs, N):
     rows = np.concatenate([pairs[:, 0], pairs[:, 1]]).astype(np.int64)
     cols = np.concatenate([pairs[:, 1], pairs[:, 0]]).astype(np.int64)
     qs = np.concatenate([q, -q - 1])
-    order = np.lexsort((cols, rows))
-    rowptr = np.searchsorted(rows[order], np.arange(N + 1)).astype(np.int32)
+    order = np.argsort(rows * N + cols)                 # unique keys: row-major, columns ascending
+    rowptr = np.zeros(N + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=N), out=rowptr[1:])
+    rowptr = rowptr.astype(np.int32)
     ent = np.stack([cols[order], qs[order]], 1).astype(np.int32)
     return rowptr, ent
 
@@ -408,7 +410,7 @@ def sync_phases(patches, off, weight=None, iters=300, frac=0.75):
     return z
 
 
-def learned_start(net, eng, I_t, P_t, O_plain_t, photons=None, alpha=1e-10, sync_iters=300):
+def learned_start(net, eng, I_t, P_t, O_plain_t, photons=None, alpha=1e-10, sync_iters=150):
     """Learned O0 on an engine whose data are set, gated by the first-iteration eMod.
 
     Patches are predicted from the frames, phase-aligned by ``sync_phases`` and
@@ -451,7 +453,7 @@ def learned_start(net, eng, I_t, P_t, O_plain_t, photons=None, alpha=1e-10, sync
 
 
 def initial_object(net, I, R, P, U, V, photons=None, mask=None, alpha=1e-10, device=None,
-                   O_plain=None, sync_iters=300):
+                   O_plain=None, sync_iters=150):
     """Stand-alone learned O0 (complex64 numpy) and the gate's info; the plain
     start is ``O_plain`` (default: a flat object of ones)."""
     dev = require_gpu(device)
