@@ -665,6 +665,45 @@ struct RowGatherArgs {
 constexpr int ROWG_VMAX = 8192;
 static_assert(ROWG_VMAX < (1 << 15), "x_n << 16 must stay a positive int");
 
+// s + the parked values of lines [lf, lf + N) at column vv, added in line
+// order (a line that does not cover vv gives the zero word at LPW * W).  All N
+// reads are issued before the first add.  oyv holds line l's column offset in
+// lane l.
+template <int L, int W, int LPW, int N>
+__device__ __forceinline__ float2 add_lines(float2 s, const float2* lds, int oyv, int lf, int vv) {
+  float2 x[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int l = lf + k;
+    const int c = vv - __builtin_amdgcn_readlane(oyv, l);
+    x[k] = lds[(unsigned)c < (unsigned)L ? l * W + c : LPW * W];
+  }
+  // keep the adds after the last read: interleaved, they drain the LDS queue
+  // once per group of four reads
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < N; ++k) s = s + x[k];
+  return s;
+}
+
+// add_lines over a wave-uniform line range [lf, lf + n): n is rounded up to
+// the next unrolled group size (1..12 -- with the row value, 13 LDS reads in
+// flight stay within the 15-deep LDS counter -- then doubling, capped at LPW)
+// and the group shifted to stay within [0, LPW); the extra lines miss the
+// wave's columns and add +0.0, which leaves every partial sum unchanged (a
+// sum that starts at +0.0 never becomes -0.0).  A 64-column window meets the
+// lines whose offsets fall in a 319-column range: 8 or 9 at a 40-pixel step.
+template <int L, int W, int LPW, int N = 1>
+__device__ __forceinline__ float2 add_line_range(float2 s, const float2* lds, int oyv, int lf, int n,
+                                                 int vv) {
+  if constexpr (N >= LPW) {
+    return add_lines<L, W, LPW, LPW>(s, lds, oyv, 0, vv);
+  } else {
+    if (n <= N) return add_lines<L, W, LPW, N>(s, lds, oyv, min(lf, LPW - N), vv);
+    return add_line_range<L, W, LPW, (N < 12 ? N + 1 : 2 * N)>(s, lds, oyv, lf, n, vv);
+  }
+}
+
 // One work-group per (object row u, column segment [v0, v0 + segw)).  The
 // frames whose window covers the segment's part of row u (ascending n) are
 // taken LPW at a time, one per line: frame row i = u - y_n is inverse-
@@ -689,6 +728,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
   constexpr int OFF_NONE = -(1 << 29);  // column offset of a spare line: matches no column
   static_assert(W >= L, "a line's exchange area must hold the parked row");
   static_assert(64 % T == 0, "a line must lie within one wave");
+  static_assert((LPW & (LPW - 1)) == 0 && LPW <= 64, "line offsets are spread one per lane");
   __shared__ float2 lds[LPW * W + 1];  // + one zero word read by uncovered slots
   __shared__ float2 s_tw[L];
   __shared__ int s_oy[LPW];
@@ -735,32 +775,36 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
     for (int e = 0; e < E; ++e) park[layA<L>(e, t)] = cmulc(v[e], pv[e]);
     if (t == 0) s_oy[line] = (bt * LPW + line < cnt) ? (m_cur.y >> 16) : OFF_NONE;
     lds_barrier();
-    int oy[LPW];
+    // lane l < LPW: line l's column offset; other lanes hold OFF_NONE
+    const int oyv = ((tid & 63) < LPW) ? s_oy[tid & (LPW - 1)] : OFF_NONE;
     int lo = v1, hi = v0;
 #pragma unroll
     for (int l = 0; l < LPW; ++l) {
-      oy[l] = __builtin_amdgcn_readfirstlane(s_oy[l]);
-      if (oy[l] != OFF_NONE) {
-        lo = min(lo, oy[l]);
-        hi = max(hi, oy[l] + L);
+      const int o = __builtin_amdgcn_readlane(oyv, l);
+      if (o != OFF_NONE) {
+        lo = min(lo, o);
+        hi = max(hi, o + L);
       }
     }
     lo = max(lo, v0);
     hi = min(hi, v1);
-    // branch-free: every line reads its parked value or the zero word, so the
-    // LPW reads are all in flight before the in-order adds (skipping lines
-    // that miss a wave's columns with uniform branches measured slower)
+    // Each wave reads only the lines that meet its 64 columns: with frames a
+    // scan step apart a line covers a few of the batch's windows, and reading
+    // every line's parked value or the zero word cost twice the LDS reads.
+    // Within a window the range's reads are all in flight before the adds.
+    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
     for (int base = lo; base < hi; base += WG) {
+      const int w0 = base + wbase, w1 = min(w0 + 64, hi);
+      if (w0 >= hi) continue;
+      // line meets [w0, w1) <=> oyv in [w0 - L + 1, w1), one unsigned compare
+      // (OFF_NONE, far below, wraps to a large unsigned and never meets)
+      const uint64_t meet = __builtin_amdgcn_ballot_w64((unsigned)(oyv - (w0 - L + 1)) <
+                                                        (unsigned)(w1 - w0 + L - 1));
+      if (meet == 0) continue;
+      const int lf = __builtin_ctzll(meet), ll = 63 - __builtin_clzll(meet);
       const int vv = base + tid;
       float2 s = (vv < hi) ? row[vv - v0] : make_float2(0.f, 0.f);
-      float2 x[LPW];
-#pragma unroll
-      for (int l = 0; l < LPW; ++l) {
-        const int c = vv - oy[l];
-        x[l] = lds[(unsigned)c < (unsigned)L ? l * W + c : LPW * W];
-      }
-#pragma unroll
-      for (int l = 0; l < LPW; ++l) s = s + x[l];
+      s = add_line_range<L, W, LPW>(s, lds, oyv, lf, ll - lf + 1, vv);
       if (vv < hi) row[vv - v0] = s;
     }
     lds_barrier();
