@@ -658,6 +658,8 @@ struct RowGatherArgs {
   const int2* __restrict__ row_ent;   // per segment, ascending n: (n*M + i, x_n << 16 | i)
   double* part_econ;       // [2][U*segs]  sum |Oold - O|^2, sum |O|^2
   double* part_esup;       // [2][U*segs]  sum Re(conj(O) num), sum |O|^2 H
+  float2* Fout;            // when set: the inverse-transformed frame rows [N][M][K]
+                           // (each row written once, by the segment holding its first column)
 };
 
 // Largest object width the fused kernel's LDS row accumulator takes (and
@@ -768,6 +770,14 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (bt + 1 < nbatch) load(m_next);
     fft_inv<L>(v, lds + line * W, s_tw, t);
+    if (a.Fout) {
+      const int xn = m_cur.y >> 16;
+      if (bt * LPW + line < cnt && xn >= v0 && xn < v0 + a.segw) {
+        const size_t rbase = (size_t)m_cur.x * L;
+#pragma unroll
+        for (int e = 0; e < E; ++e) store_stream(&a.Fout[rbase + layA<L>(e, t)], v[e]);
+      }
+    }
     // park conj(P) * psi' in this line's area: every value depends on the
     // pass-2 reads of the same wave, so no barrier is needed before it
     float2* park = lds + line * W;

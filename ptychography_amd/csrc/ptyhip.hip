@@ -486,8 +486,10 @@ int rowg_attribute(int K) {
 }
 
 // Inverse row FFT of Y fused with the object update O <- sum conj(P) psi' / (H + alpha)
-// (H valid); one work-group per object row, partials per row.
-int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, int* nparts) {
+// (H valid); one work-group per object row segment, partials per segment.  Fout, when
+// given, also receives the inverse-transformed rows (what rows_inv<DST_STORE> writes).
+int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, int* nparts,
+                float2* Fout = nullptr) {
   StageTimer tm(c, PTY_STAGE_GATHER);
   RowGatherArgs a{};
   a.Y = Y;
@@ -502,6 +504,7 @@ int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, 
   a.row_ent = c->row_ent;
   a.part_econ = econ ? c->part_econO : nullptr;
   a.part_esup = esup ? c->part_esxh : nullptr;
+  a.Fout = Fout;
   const size_t dyn = (size_t)c->rowg_segw * sizeof(float2);
   const int grid = c->U * c->rowg_segs;
   switch (c->K) {
@@ -879,6 +882,9 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   const int nrows = c->N * c->M;
   const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
   const int nchunks = (c->N + c->col_chunk - 1) / c->col_chunk;
+  // object-only projections of the freshly modulus-projected f (dm_project's
+  // gather of f with P and the cached heatmap) fuse into its inverse row FFT
+  const bool fused_proj = bgm && c->rowg && c->H_valid && (upd == 0 || (upd == 2 && !op_step));
 
   if (bgm) {
     // leading Pmod_7 of (f, b) (dm.py:171)
@@ -892,11 +898,16 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     a.bg_stride = (long)MK;
     a.partb = c->partb;
     PTY_TRY(cols<COL_DMB_LEAD>(c, a));
-    PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->F, nullptr, nullptr, 1.0f, nullptr, nrows));
+    if (fused_proj) {
+      // the inverse row FFT writes f and projects it onto the object in one pass
+      PTY_TRY(rows_gather(c, c->X, o.alpha, false, false, nullptr, c->F));
+    } else {
+      PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->F, nullptr, nullptr, 1.0f, nullptr, nrows));
+    }
     PTY_TRY(bg_mean(c, nchunks, c->bbar));
   }
 
-  PTY_TRY(dm_project(c, o, it, upd, op_step, c->O, c->P, false));
+  if (!fused_proj) PTY_TRY(dm_project(c, o, it, upd, op_step, c->O, c->P, false));
   if (pmodp) PTY_TRY(pmod_probe(c, c->P, o.alpha));
 
   // f <- f - Ps f + Pm(2 Ps f - f)    (dm.py:91-97, :216-224)

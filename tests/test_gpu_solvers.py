@@ -201,15 +201,17 @@ def test_dm_centering_schedule(gpu_device):
     _check_metrics(out_g[-1], out_r[-1], single[0][-1])
 
 
-@pytest.mark.parametrize("method", [4, 6])
-def test_dm_background_methods(method, gpu_device):
-    print(f"DM method {method} (background)", flush=True)
-    s = small_scan(frame=32, grid=(5, 5), step=6, jitter=1, background_frac=0.02,
+@pytest.mark.parametrize("method,frame,op", [(4, 32, 1), (6, 32, 1), (4, 64, 1), (6, 64, (1, 2))])
+def test_dm_background_methods(method, frame, op, gpu_device):
+    # at 64^2 the object-only projections after the leading Pmod_7 run fused
+    # into its inverse row FFT (method 6 on its non-OP iterations)
+    print(f"DM method {method} (background), frame {frame}, OP_iters {op}", flush=True)
+    s = small_scan(frame=frame, grid=(5, 5), step=frame * 3 // 16, jitter=1, background_frac=0.02,
                    beamstop_radius=1.5)
     P0 = s.P if method == 4 else flat_start(s)
-    B0 = np.random.default_rng(4).random((32, 32)) * 0.5
+    B0 = np.random.default_rng(4).random((frame, frame)) * 0.5
     out_g, out_r, tg, tr, single = _run_both(pty.DM, oracle.DM, (s.I, s.R, P0, _O0(s)),
-                                     dict(method=method, background=B0, mask=s.mask), 10)
+                                     dict(method=method, background=B0, mask=s.mask, OP_iters=op), 10)
     region = _illuminated(s.P, s.R, s.O.shape)
     _check_traj(tg, tr, region, "O" if method == 4 else "OP", single)
     _check_metrics(out_g[-1], out_r[-1], single[0][-1])
