@@ -34,6 +34,12 @@ own formula; the tests live in ``tests/test_oracle_*.py``):
                                    inputs (direct DFT, explicit loops).
 * ``DM`` (methods 1-6)          -- exact-data fixed point, an independent
                                    pure-loop DM on tiny inputs.
+* ``hio.HIO``                   -- Fienup's classic input-output rule, to which
+                                   HIO reduces when Psup is a 0/1 support
+                                   projection (binary probe, frames that do not
+                                   overlap; one frame and a binary object for
+                                   'probe'); the exact-data fixed point; the
+                                   first eMod equals ERA's.
 * ``numpy.fft`` itself           -- parity unpinned against the reference's own
                                    (unpinned) numpy version; pinned here by the
                                    direct DFT and closed forms.
@@ -43,8 +49,9 @@ from .era import (ERA, preamble, pmod_1, Pmod_1, pmod_7, Pmod_7, Pmod_P, psup_O,
                   psup_P, make_P_heatmap, make_O_heatmap, make_exits, multiroll,
                   center_of_mass, probe_centre_shift)
 from .dm import DM, model_error_1
+from .hio import HIO
 
 __all__ = ["ERA", "DM", "preamble", "pmod_1", "Pmod_1", "pmod_7", "Pmod_7",
            "Pmod_P", "psup_O", "psup_P", "make_P_heatmap", "make_O_heatmap",
            "make_exits", "multiroll", "center_of_mass", "probe_centre_shift",
-           "model_error_1"]
+           "model_error_1", "HIO"]
