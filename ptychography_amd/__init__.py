@@ -9,6 +9,7 @@ from .era import ERA
 from .dm import DM
 from .batch import ERA_many, DM_many
 from .back_projection import Back_projection
+from .hio import HIO
 from . import sim
 
-__all__ = ["ERA", "DM", "ERA_many", "DM_many", "Back_projection", "sim"]
+__all__ = ["ERA", "DM", "ERA_many", "DM_many", "Back_projection", "HIO", "sim"]

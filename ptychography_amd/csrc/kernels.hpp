@@ -985,6 +985,34 @@ __global__ __launch_bounds__(WG) void k_make_exits(Geom g, const float2* O, cons
   }
 }
 
+// f_n += scale * O_win,n * P  (HIO's Psup term, utility_Ptych.py:16)
+__global__ __launch_bounds__(WG) void k_add_exits(Geom g, const float2* O, const float2* P,
+                                                  float scale, float2* f) {
+  const size_t MK = (size_t)g.M * g.K, total = MK * g.N;
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const int n = (int)(idx / MK);
+    const int p = (int)(idx % MK), i = p / g.K, j = p % g.K;
+    const int2 o = g.off[n];
+    const float2 e = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[p]);
+    const float2 v = f[idx];
+    f[idx] = make_float2(v.x + scale * e.x, v.y + scale * e.y);
+  }
+}
+
+// HIO's linear combinations (utility_Ptych.py:16), m = Pmod(f) on entry:
+//   m <- a m - b f   (a = 1 + 1/beta, b = 1/beta: Psup's argument)
+//   f <- f - beta m
+__global__ __launch_bounds__(WG) void k_hio_mix(float2* f, float2* m, size_t total, float a,
+                                                float b, float beta) {
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const float2 fv = f[idx], mv = m[idx];
+    m[idx] = make_float2(a * mv.x - b * fv.x, a * mv.y - b * fv.y);
+    f[idx] = make_float2(fv.x - beta * mv.x, fv.y - beta * mv.y);
+  }
+}
+
 // partials of sum |psi'_n - O_n P|^2 (support error on an explicit pass)
 __global__ __launch_bounds__(WG) void k_esup(Geom g, const float2* S, const float2* O,
                                              const float2* P, double* part) {

@@ -1912,6 +1912,28 @@ int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfl
   return PTY_OK;
 }
 
+int pty_hio_mix(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, float beta) {
+  if (!c || !f || !m) return fail(PTY_EINVAL, "NULL argument");
+  if (!(beta != 0.0f) || !std::isfinite(beta)) return fail(PTY_EINVAL, "beta must be finite and non-zero");
+  const size_t total = (size_t)c->N * c->MK();
+  const double binv = 1.0 / (double)beta;
+  hipLaunchKernelGGL(k_hio_mix, dim3(4096), dim3(WG), 0, c->stream, reinterpret_cast<float2*>(f),
+                     reinterpret_cast<float2*>(m), total, (float)(1.0 + binv), (float)binv, beta);
+  PTY_LAUNCHED("k_hio_mix");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_add_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, float scale, pty_cfloat* f) {
+  if (!c || !O || !P || !f) return fail(PTY_EINVAL, "NULL argument");
+  hipLaunchKernelGGL(k_add_exits, dim3(4096), dim3(WG), 0, c->stream, c->g,
+                     reinterpret_cast<const float2*>(O), reinterpret_cast<const float2*>(P), scale,
+                     reinterpret_cast<float2*>(f));
+  PTY_LAUNCHED("k_add_exits");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
 int pty_pmod1(pty_ctx* c, pty_cfloat* exits, float alpha, double* emod) {
   if (!c || !exits) return fail(PTY_EINVAL, "NULL argument");
   if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");

@@ -234,6 +234,25 @@ class Engine:
         call("pty_psup_O", self._ctx, _ptr(exits), _ptr(P), _ptr(O), _ptr(H), int(fresh), float(alpha))
         return O, H
 
+    def hio_mix(self, f, m, beta):
+        """HIO's linear steps in place on (N, M, K) exits, m = Pmod(f) on entry:
+        m <- (1 + 1/beta) m - f / beta, f <- f - beta m."""
+        for t in (f, m):
+            assert t.dtype == torch.complex64 and t.is_contiguous()
+            assert tuple(t.shape) == (self.N, self.M, self.K)
+        self._sync_torch()
+        call("pty_hio_mix", self._ctx, _ptr(f), _ptr(m), float(beta))
+        return f, m
+
+    def add_exits(self, f, O, P, scale):
+        """f += scale * make_exits(O, P) in place."""
+        assert f.dtype == torch.complex64 and f.is_contiguous()
+        assert tuple(f.shape) == (self.N, self.M, self.K)
+        assert tuple(O.shape) == (self.U, self.V) and tuple(P.shape) == (self.M, self.K)
+        self._sync_torch()
+        call("pty_add_exits", self._ctx, _ptr(O), _ptr(P), float(scale), _ptr(f))
+        return f
+
     def back_projection(self, P, alpha=1e-10, exits=False):
         """Back-projection object from the bound data: (O, H, exits or None)."""
         O = torch.empty((self.U, self.V), dtype=torch.complex64, device=self.device)
