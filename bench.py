@@ -47,6 +47,10 @@ ERA1_BYTES_PX = 20
 OBJ_BYTES_PX = 24
 # DM method 4 (f, b, amp through two modulus steps + two overlap reads + metric amp read)
 DM4_BYTES_PX = 76
+# HIO (update 'sample') on the path's kernels, per frame pixel: row forward f -> m (read 8,
+# write 8), columns 20, row inverse with both linear combinations (read m and f 16, write
+# both 16), the object gather 8, f += beta O P (read 8, write 8)
+HIO_BYTES_PX = 92
 
 # algorithmic bytes per frame pixel of each stage's launch (ERA method 1)
 STAGE_BYTES_PX_STAGED = {
@@ -94,6 +98,8 @@ def parse():
     ap.add_argument("--frame-slots", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-init timing")
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-scan timing")
+    ap.add_argument("--no-widening", action="store_true",
+                    help="skip the HIO and back-projection timings")
     ap.add_argument("--multi-iters", type=int, default=150)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--nn", action="store_true",
@@ -275,6 +281,40 @@ def make_scan_gpu(grid, dev):
     return dict(I=I, R=R, P=P, O=O, B=B, mask=mask, N=N, M=M, U=U, V=V, spec=spec)
 
 
+def widening(scan, dev, iters=6):
+    """HIO (update 'sample', beta 0.9, flat start) and back projection on the same
+    scan.  HIO's time per iteration is the difference of two calls (iters and
+    2 iters), which cancels the set-up and the copy of O and P back to the host;
+    back projection is timed as a whole call (set-up, one pass, O to the host)."""
+    import torch
+    import ptychography_amd as pty
+    N, M, U, V = scan["N"], scan["M"], scan["U"], scan["V"]
+    O0 = np.ones((U, V), np.complex64)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def hio(k):
+        return timed(lambda: pty.HIO(scan["I"], scan["R"].copy(), scan["P"], O0, k, beta=0.9,
+                                     mask=scan["mask"], full_output=False, device=dev))
+
+    hio(1)
+    t1, t2 = hio(iters), hio(2 * iters)
+    per = (t2 - t1) / iters
+    bp = [timed(lambda: pty.Back_projection(scan["I"], scan["R"].copy(), scan["P"], O0,
+                                            mask=scan["mask"], full_output=False, device=dev))
+          for _ in range(2)]
+    px = N * M * M
+    return {"hio_workload": f"hio-sample-beta0.9-{N}x{M}x{M}-object{U}x{V}",
+            "hio_ms_per_iter": 1e3 * per, "hio_iters_per_s": 1.0 / per,
+            "hio_roofline_frac": (HIO_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per / (HBM_PEAK_GBS * 1e9),
+            "back_projection_call_s": bp[-1]}
+
+
 def bind(eng, scan, dev, O0, P0):
     import torch
     mask_c = torch.from_numpy(scan["mask"].astype(np.uint8)).to(dev)
@@ -392,6 +432,10 @@ def main():
     if not args.no_multi:
         multi = multi_scan(dev, args.multi_iters)
 
+    widen = None
+    if not args.no_widening:
+        widen = widening(scan, dev)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scan, args.cpu_frames)
@@ -426,6 +470,7 @@ def main():
         "extra": dm,
         "learned_init": learned,
         "multi_scan": multi,
+        "widening": widen,
     }
     print(json.dumps(out))
 

@@ -153,17 +153,25 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
 // ---------------------------------------------------------------------------
 // row stage, inverse
 // ---------------------------------------------------------------------------
-enum RowDst { DST_STORE = 0, DST_DMUPD = 1 };
+enum RowDst { DST_STORE = 0, DST_DMUPD = 1, DST_HIO = 2 };
+
+// HIO's linear steps (utility_Ptych.py:16) applied where m = Pmod(f) leaves
+// the inverse row FFT: Out <- a m - b f (Psup's argument), F <- f - beta m.
+struct HioMix {
+  float2* F;
+  float a, b, beta;   // 1 + 1/beta, 1/beta, beta
+};
 
 // y = scale * IFFT_row(Y[n, i, :]) (unnormalised inverse, times scale), then
 //   DST_STORE: Out[n, i, c] = y                         (Out may alias Y)
 //   DST_DMUPD: Out[n, i, c] += y - O[..] P[i, c]        (DM: f - Ps f + Pm(..))
 //              partial[block] = sum |y - O P|^2          (DM step size)
+//   DST_HIO  : Out[n, i, c] = a y - b F, F[n, i, c] -= beta y   (HioMix)
 template <int L, int DST>
 __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, float2* Out,
                                                  const float2* O, const float2* P, float scale,
                                                  double* partial, const float2* __restrict__ tw,
-                                                 int nrows) {
+                                                 int nrows, HioMix hm) {
   constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[LPW * W];
@@ -194,6 +202,15 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
       if constexpr (DST == DST_STORE) {
 #pragma unroll
         for (int e = 0; e < E; ++e) store_stream(&Out[rbase + layA<L>(e, t)], v[e] * scale);
+      } else if constexpr (DST == DST_HIO) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int c = layA<L>(e, t);
+          const float2 m = v[e] * scale;
+          const float2 fv = load_stream(&hm.F[rbase + c]);
+          store_stream(&Out[rbase + c], make_float2(hm.a * m.x - hm.b * fv.x, hm.a * m.y - hm.b * fv.y));
+          store_stream(&hm.F[rbase + c], make_float2(fv.x - hm.beta * m.x, fv.y - hm.beta * m.y));
+        }
       } else {
         const int M = g.M, n = row / M, i = row % M;
         const int2 o = g.off[n];
@@ -997,19 +1014,6 @@ __global__ __launch_bounds__(WG) void k_add_exits(Geom g, const float2* O, const
     const float2 e = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[p]);
     const float2 v = f[idx];
     f[idx] = make_float2(v.x + scale * e.x, v.y + scale * e.y);
-  }
-}
-
-// HIO's linear combinations (utility_Ptych.py:16), m = Pmod(f) on entry:
-//   m <- a m - b f   (a = 1 + 1/beta, b = 1/beta: Psup's argument)
-//   f <- f - beta m
-__global__ __launch_bounds__(WG) void k_hio_mix(float2* f, float2* m, size_t total, float a,
-                                                float b, float beta) {
-  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
-       idx += (size_t)gridDim.x * WG) {
-    const float2 fv = f[idx], mv = m[idx];
-    m[idx] = make_float2(a * mv.x - b * fv.x, a * mv.y - b * fv.y);
-    f[idx] = make_float2(fv.x - beta * mv.x, fv.y - beta * mv.y);
   }
 }
 

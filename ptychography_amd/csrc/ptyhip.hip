@@ -278,7 +278,7 @@ int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, floa
 
 template <int DST>
 int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const float2* P,
-             float scale, double* partial, int nrows, int* nparts = nullptr) {
+             float scale, double* partial, int nrows, int* nparts = nullptr, HioMix hm = {}) {
   StageTimer tm(c, PTY_STAGE_ROWS_INV);
   const int T = threads_per_line(c->K), lpw = WG / T;
   dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
@@ -287,7 +287,7 @@ int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const fl
 #define C(Lv)                                                                               \
   case Lv:                                                                                  \
     hipLaunchKernelGGL((k_rows_inv<Lv, DST>), grid, dim3(WG), 0, c->stream, c->g, Y, Out, O, P, \
-                       scale, partial, (const float2*)c->twK, nrows);                       \
+                       scale, partial, (const float2*)c->twK, nrows, hm);                   \
     break;
     PTY_SIZES(C)
 #undef C
@@ -1912,15 +1912,32 @@ int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfl
   return PTY_OK;
 }
 
-int pty_hio_mix(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, float beta) {
+int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, float beta, float alpha,
+                    double* emod) {
   if (!c || !f || !m) return fail(PTY_EINVAL, "NULL argument");
+  if (f == m) return fail(PTY_EINVAL, "f and m must be different buffers");
+  if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
   if (!(beta != 0.0f) || !std::isfinite(beta)) return fail(PTY_EINVAL, "beta must be finite and non-zero");
-  const size_t total = (size_t)c->N * c->MK();
+  float2* F = reinterpret_cast<float2*>(f);
+  float2* Mb = reinterpret_cast<float2*>(m);
+  const int nrows = c->N * c->M;
+  // FFT rows of f into m; columns with the modulus step and eMod partials;
+  // inverse rows with both linear combinations applied on the way out
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, F, Mb, nrows));
+  ColArgs a = col_args(c);
+  a.X = Mb;
+  a.amp = c->amp;
+  a.amp_stride = (long)c->MK();
+  a.alpha = alpha;
+  a.part = c->part_emod;
+  int ncol = 0;
+  PTY_TRY(cols<COL_ERA1>(c, a, &ncol));
   const double binv = 1.0 / (double)beta;
-  hipLaunchKernelGGL(k_hio_mix, dim3(4096), dim3(WG), 0, c->stream, reinterpret_cast<float2*>(f),
-                     reinterpret_cast<float2*>(m), total, (float)(1.0 + binv), (float)binv, beta);
-  PTY_LAUNCHED("k_hio_mix");
-  PTY_HIP(hipStreamSynchronize(c->stream));
+  const HioMix hm{F, (float)(1.0 + binv), (float)binv, beta};
+  PTY_TRY(rows_inv<DST_HIO>(c, Mb, Mb, nullptr, nullptr, 1.0f, nullptr, nrows, nullptr, hm));
+  double s = 0.0;
+  PTY_TRY(sum_region(c, c->part_emod, ncol, &s));
+  if (emod) *emod = s;
   return PTY_OK;
 }
 

@@ -234,15 +234,18 @@ class Engine:
         call("pty_psup_O", self._ctx, _ptr(exits), _ptr(P), _ptr(O), _ptr(H), int(fresh), float(alpha))
         return O, H
 
-    def hio_mix(self, f, m, beta):
-        """HIO's linear steps in place on (N, M, K) exits, m = Pmod(f) on entry:
-        m <- (1 + 1/beta) m - f / beta, f <- f - beta m."""
+    def hio_modulus(self, f, m, beta, alpha=1e-10):
+        """One HIO modulus step on (N, M, K) exits: m = Pmod(f), then in the same
+        pass m <- (1 + 1/beta) m - f / beta and f <- f - beta m.  Returns the
+        un-normalised eMod sum of f."""
         for t in (f, m):
             assert t.dtype == torch.complex64 and t.is_contiguous()
             assert tuple(t.shape) == (self.N, self.M, self.K)
+        assert f.data_ptr() != m.data_ptr()
         self._sync_torch()
-        call("pty_hio_mix", self._ctx, _ptr(f), _ptr(m), float(beta))
-        return f, m
+        e = ctypes.c_double()
+        call("pty_hio_modulus", self._ctx, _ptr(f), _ptr(m), float(beta), float(alpha), ctypes.byref(e))
+        return e.value
 
     def add_exits(self, f, O, P, scale):
         """f += scale * make_exits(O, P) in place."""

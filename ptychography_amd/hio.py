@@ -20,11 +20,12 @@ path's (pmod_1 with the preamble's centred amplitudes and mask, psup_O /
 psup_P with alpha), and eMod is era.py's modulus error of the exits entering
 each iteration (era.py:561-563, 210).
 
-Every step runs on the path's kernels: ``pty_pmod1`` (row / column FFT stages
-with the masked modulus step and eMod partials), ``pty_hio_mix`` (both linear
-combinations in one pass), ``pty_psup_O`` / ``pty_psup_P`` (deterministic
-gathers, heatmap kept while its factor is fixed) and ``pty_add_exits`` (the
-Psup term added to the exits).
+Every step runs on the path's kernels: ``pty_hio_modulus`` (the row / column
+FFT stages with the masked modulus step and eMod partials, the forward rows
+reading f and writing m, both linear combinations applied as m leaves the
+inverse row FFT), ``pty_psup_O`` / ``pty_psup_P`` (deterministic gathers,
+heatmap kept while its factor is fixed) and ``pty_add_exits`` (the Psup term
+added to the exits).
 """
 
 import math
@@ -83,10 +84,8 @@ def HIO(I, R, P, O, iters, beta=1.0, update='sample', mask=1, alpha=1.0e-10, ful
         H = G = None
         eMods = []
         for i in range(int(iters)):
-            m.copy_(f)
-            _, e = eng.pmod1(m, alpha=alpha)
+            e = eng.hio_modulus(f, m, beta, alpha)  # m: Psup's argument; f: f - beta Pmod(f)
             eMods.append(math.sqrt(e / I_norm))
-            eng.hio_mix(f, m, beta)                 # m: Psup's argument; f: f - beta Pmod(f)
             if update == 'sample':
                 O_t, H = eng.psup_O(m, P_t, H, alpha=alpha)
             elif update == 'probe':
