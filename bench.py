@@ -49,8 +49,8 @@ OBJ_BYTES_PX = 24
 DM4_BYTES_PX = 76
 # HIO (update 'sample') on the path's kernels, per frame pixel: row forward f -> m (read 8,
 # write 8), columns 20, row inverse with both linear combinations (read m and f 16, write
-# both 16), the object gather 8, f += beta O P (read 8, write 8)
-HIO_BYTES_PX = 92
+# both 16), the object gather 8 (the pending Psup term comes from the object window)
+HIO_BYTES_PX = 76
 
 # algorithmic bytes per frame pixel of each stage's launch (ERA method 1)
 STAGE_BYTES_PX_STAGED = {

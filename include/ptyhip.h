@@ -227,14 +227,16 @@ int pty_psup_P(pty_ctx* ctx, const pty_cfloat* exits_dev, const pty_cfloat* O_de
 int pty_back_projection(pty_ctx* ctx, const pty_cfloat* P_dev, pty_cfloat* exits_out_dev,
                         pty_cfloat* O_out_dev, double* H_dev, float alpha);
 
-/* One HIO modulus step on N x M x K exits f (utility_Ptych.py:14-17, the update
- * of the legacy Ptychography.HIO driver, Ptychography_2dsample_2dprobe_farfield.py:93-127)
- * with the bound data: m = pmod_1(f) (era.py:550-572) is formed in m_dev and
- * the update's two linear combinations are applied as it leaves the inverse
- * row FFT: m <- (1 + 1/beta) m - f / beta (Psup's argument), f <- f - beta m.
+/* One HIO modulus step (utility_Ptych.py:14-17, the update of the legacy
+ * Ptychography.HIO driver, Ptychography_2dsample_2dprobe_farfield.py:93-127)
+ * with the bound data.  The exits are f = F + w * make_exits(O, P, R), F in
+ * f_dev (w = 0: f = F; w = beta after an iteration leaves its Psup term
+ * pending).  m = pmod_1(f) (era.py:550-572) is formed in m_dev and the update's
+ * two linear combinations are applied as it leaves the inverse row FFT:
+ * m <- (1 + 1/beta) m - f / beta (Psup's argument), F <- f - beta m.
  * *emod_host = the un-normalised eMod sum of f.  f_dev != m_dev, beta != 0. */
-int pty_hio_modulus(pty_ctx* ctx, pty_cfloat* f_dev, pty_cfloat* m_dev, float beta, float alpha,
-                    double* emod_host);
+int pty_hio_modulus(pty_ctx* ctx, pty_cfloat* f_dev, pty_cfloat* m_dev, const pty_cfloat* O_dev,
+                    const pty_cfloat* P_dev, float w, float beta, float alpha, double* emod_host);
 
 /* f += scale * make_exits(O, P, R): the Psup term of the same update
  * (utility_Ptych.py:16 with makeExits, :24-30). */

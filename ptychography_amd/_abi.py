@@ -68,7 +68,8 @@ SIGNATURES = {
     "pty_psup_O": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_back_projection": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _f32]),
-    "pty_hio_modulus": (ctypes.c_int, [_vp, _vp, _vp, _f32, _f32, ctypes.POINTER(_f64)]),
+    "pty_hio_modulus": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32,
+                                       ctypes.POINTER(_f64)]),
     "pty_add_exits": (ctypes.c_int, [_vp, _vp, _vp, _f32, _vp]),
     "pty_set_launch": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "pty_set_chunk": (ctypes.c_int, [_vp, _i32]),

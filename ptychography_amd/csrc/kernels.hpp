@@ -79,7 +79,7 @@ __device__ __forceinline__ void stage_twiddles(float2* s_tw, const float2* __res
 // ---------------------------------------------------------------------------
 // row stage, forward
 // ---------------------------------------------------------------------------
-enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2 };
+enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2, SRC_HIO = 3 };
 
 // X[n, i, :] = FFT_row( source(n, i, :) ) for rows [0, nrows) of the stack
 // (streaming stores: measured 0.62 vs 0.68 ms at 4096 x 256^2, the object
@@ -87,13 +87,16 @@ enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2 };
 //   SRC_OP: O[y_n + i, x_n + c] * P[i, c]
 //   SRC_DM: 2 O[..] P[i, c] - F[n, i, c]
 //   SRC_F : F[n, i, c]                 (X may alias F)
+//   SRC_HIO: F[n, i, c] + w O[..] P[i, c]  (HIO's iterate with its pending Psup term)
 template <int L, int SRC>
 __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, const float2* P,
                                                  const float2* F, float2* X,
-                                                 const float2* __restrict__ tw, int nrows) {
+                                                 const float2* __restrict__ tw, int nrows,
+                                                 float w) {
   constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<L>::words;
-  constexpr int E2 = (SRC == SRC_DM) ? E : 1;  // f values (DM)
+  constexpr bool HAS_F = (SRC == SRC_DM || SRC == SRC_HIO);
+  constexpr int E2 = HAS_F ? E : 1;  // f values (DM, HIO)
   __shared__ float2 lds[LPW * W];
   __shared__ float2 s_tw[L];
   stage_twiddles<L>(s_tw, tw);
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
       for (int e = 0; e < E; ++e) {
         const int c = layA<L>(e, t);
         n0[e] = Orow[c];
-        if constexpr (SRC == SRC_DM) n2[e] = load_stream(&F[rbase + c]);
+        if constexpr (HAS_F) n2[e] = load_stream(&F[rbase + c]);
       }
     }
   };
@@ -135,6 +138,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
       for (int e = 0; e < E; ++e) {
         float2 psi = cmul(n0[e], Prow[layA<L>(e, t)]);
         if constexpr (SRC == SRC_DM) psi = make_float2(2.f * psi.x - n2[e].x, 2.f * psi.y - n2[e].y);
+        if constexpr (SRC == SRC_HIO) psi = make_float2(n2[e].x + w * psi.x, n2[e].y + w * psi.y);
         v[e] = psi;
       }
     }
@@ -156,9 +160,13 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd(Geom g, const float2* O, con
 enum RowDst { DST_STORE = 0, DST_DMUPD = 1, DST_HIO = 2 };
 
 // HIO's linear steps (utility_Ptych.py:16) applied where m = Pmod(f) leaves
-// the inverse row FFT: Out <- a m - b f (Psup's argument), F <- f - beta m.
+// the inverse row FFT, with f = F + w O P (F holds the iterate without its
+// pending Psup term): Out <- a m - b f (Psup's argument), F <- f - beta m.
 struct HioMix {
   float2* F;
+  const float2* O;
+  const float2* P;
+  float w;            // coefficient of the pending O P term (0: none)
   float a, b, beta;   // 1 + 1/beta, 1/beta, beta
 };
 
@@ -166,7 +174,7 @@ struct HioMix {
 //   DST_STORE: Out[n, i, c] = y                         (Out may alias Y)
 //   DST_DMUPD: Out[n, i, c] += y - O[..] P[i, c]        (DM: f - Ps f + Pm(..))
 //              partial[block] = sum |y - O P|^2          (DM step size)
-//   DST_HIO  : Out[n, i, c] = a y - b F, F[n, i, c] -= beta y   (HioMix)
+//   DST_HIO  : f = F + w O[..] P[i, c]; Out = a y - b f, F = f - beta y (HioMix)
 template <int L, int DST>
 __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, float2* Out,
                                                  const float2* O, const float2* P, float scale,
@@ -203,11 +211,19 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
 #pragma unroll
         for (int e = 0; e < E; ++e) store_stream(&Out[rbase + layA<L>(e, t)], v[e] * scale);
       } else if constexpr (DST == DST_HIO) {
+        const int M = g.M, n = row / M, i = row % M;
+        const int2 o = g.off[n];
+        const float2* Orow = hm.O + (size_t)(o.x + i) * g.V + o.y;
+        const float2* Prow = hm.P + (size_t)i * L;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int c = layA<L>(e, t);
           const float2 m = v[e] * scale;
-          const float2 fv = load_stream(&hm.F[rbase + c]);
+          float2 fv = load_stream(&hm.F[rbase + c]);
+          if (hm.w != 0.f) {
+            const float2 ex = cmul(Orow[c], Prow[c]);
+            fv = make_float2(fv.x + hm.w * ex.x, fv.y + hm.w * ex.y);
+          }
           store_stream(&Out[rbase + c], make_float2(hm.a * m.x - hm.b * fv.x, hm.a * m.y - hm.b * fv.y));
           store_stream(&hm.F[rbase + c], make_float2(fv.x - hm.beta * m.x, fv.y - hm.beta * m.y));
         }

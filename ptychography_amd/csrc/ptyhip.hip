@@ -258,7 +258,8 @@ int persistent_grid(long items, long cap) {
 }
 
 template <int SRC>
-int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows) {
+int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows,
+             float w = 0.f) {
   StageTimer tm(c, PTY_STAGE_ROWS_FWD);
   const int T = threads_per_line(c->K), lpw = WG / T;
   dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
@@ -266,7 +267,7 @@ int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, floa
 #define C(Lv)                                                                              \
   case Lv:                                                                                 \
     hipLaunchKernelGGL((k_rows_fwd<Lv, SRC>), grid, dim3(WG), 0, c->stream, c->g, O, P, F, X, \
-                       (const float2*)c->twK, nrows);                                      \
+                       (const float2*)c->twK, nrows, w);                                   \
     break;
     PTY_SIZES(C)
 #undef C
@@ -1912,18 +1913,21 @@ int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfl
   return PTY_OK;
 }
 
-int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, float beta, float alpha,
-                    double* emod) {
-  if (!c || !f || !m) return fail(PTY_EINVAL, "NULL argument");
+int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, const pty_cfloat* O,
+                    const pty_cfloat* P, float w, float beta, float alpha, double* emod) {
+  if (!c || !f || !m || !O || !P) return fail(PTY_EINVAL, "NULL argument");
+  if (!std::isfinite(w)) return fail(PTY_EINVAL, "w must be finite");
   if (f == m) return fail(PTY_EINVAL, "f and m must be different buffers");
   if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
   if (!(beta != 0.0f) || !std::isfinite(beta)) return fail(PTY_EINVAL, "beta must be finite and non-zero");
   float2* F = reinterpret_cast<float2*>(f);
   float2* Mb = reinterpret_cast<float2*>(m);
+  const float2* Ob = reinterpret_cast<const float2*>(O);
+  const float2* Pb = reinterpret_cast<const float2*>(P);
   const int nrows = c->N * c->M;
-  // FFT rows of f into m; columns with the modulus step and eMod partials;
-  // inverse rows with both linear combinations applied on the way out
-  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, F, Mb, nrows));
+  // FFT rows of f = F + w O P into m; columns with the modulus step and eMod
+  // partials; inverse rows with both linear combinations applied on the way out
+  PTY_TRY(rows_fwd<SRC_HIO>(c, Ob, Pb, F, Mb, nrows, w));
   ColArgs a = col_args(c);
   a.X = Mb;
   a.amp = c->amp;
@@ -1933,7 +1937,7 @@ int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, float beta, float 
   int ncol = 0;
   PTY_TRY(cols<COL_ERA1>(c, a, &ncol));
   const double binv = 1.0 / (double)beta;
-  const HioMix hm{F, (float)(1.0 + binv), (float)binv, beta};
+  const HioMix hm{F, Ob, Pb, w, (float)(1.0 + binv), (float)binv, beta};
   PTY_TRY(rows_inv<DST_HIO>(c, Mb, Mb, nullptr, nullptr, 1.0f, nullptr, nrows, nullptr, hm));
   double s = 0.0;
   PTY_TRY(sum_region(c, c->part_emod, ncol, &s));

@@ -234,17 +234,19 @@ class Engine:
         call("pty_psup_O", self._ctx, _ptr(exits), _ptr(P), _ptr(O), _ptr(H), int(fresh), float(alpha))
         return O, H
 
-    def hio_modulus(self, f, m, beta, alpha=1e-10):
-        """One HIO modulus step on (N, M, K) exits: m = Pmod(f), then in the same
-        pass m <- (1 + 1/beta) m - f / beta and f <- f - beta m.  Returns the
-        un-normalised eMod sum of f."""
-        for t in (f, m):
+    def hio_modulus(self, F, m, O, P, w, beta, alpha=1e-10):
+        """One HIO modulus step on (N, M, K) exits f = F + w make_exits(O, P):
+        m = Pmod(f), then in the same pass m <- (1 + 1/beta) m - f / beta and
+        F <- f - beta m.  Returns the un-normalised eMod sum of f."""
+        for t in (F, m):
             assert t.dtype == torch.complex64 and t.is_contiguous()
             assert tuple(t.shape) == (self.N, self.M, self.K)
-        assert f.data_ptr() != m.data_ptr()
+        assert F.data_ptr() != m.data_ptr()
+        assert tuple(O.shape) == (self.U, self.V) and tuple(P.shape) == (self.M, self.K)
         self._sync_torch()
         e = ctypes.c_double()
-        call("pty_hio_modulus", self._ctx, _ptr(f), _ptr(m), float(beta), float(alpha), ctypes.byref(e))
+        call("pty_hio_modulus", self._ctx, _ptr(F), _ptr(m), _ptr(O), _ptr(P), float(w), float(beta),
+             float(alpha), ctypes.byref(e))
         return e.value
 
     def add_exits(self, f, O, P, scale):

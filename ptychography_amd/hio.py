@@ -21,11 +21,12 @@ psup_P with alpha), and eMod is era.py's modulus error of the exits entering
 each iteration (era.py:561-563, 210).
 
 Every step runs on the path's kernels: ``pty_hio_modulus`` (the row / column
-FFT stages with the masked modulus step and eMod partials, the forward rows
-reading f and writing m, both linear combinations applied as m leaves the
-inverse row FFT), ``pty_psup_O`` / ``pty_psup_P`` (deterministic gathers,
-heatmap kept while its factor is fixed) and ``pty_add_exits`` (the Psup term
-added to the exits).
+FFT stages with the masked modulus step and eMod partials; the iterate is kept
+without its Psup term, which both row stages add back on the fly from the
+object and probe, and the update's two linear combinations are applied as
+Pmod(f) leaves the inverse row FFT) and ``pty_psup_O`` / ``pty_psup_P``
+(deterministic gathers, heatmap kept while its factor is fixed);
+``pty_add_exits`` completes the exits once for ``info['exits']``.
 """
 
 import math
@@ -79,12 +80,16 @@ def HIO(I, R, P, O, iters, beta=1.0, update='sample', mask=1, alpha=1.0e-10, ful
         eng.set_data(amp, mask_s, I_norm)
         O_t = torch.from_numpy(np.ascontiguousarray(O.astype(np.complex64))).to(dev)
         P_t = torch.from_numpy(np.ascontiguousarray(P.astype(np.complex64))).to(dev)
-        f = eng.make_exits(O_t, P_t)
-        m = torch.empty_like(f)
+        # F holds the iterate without its pending Psup term: f = F + w O P, where w
+        # is 0 at the start (F = O P) and beta after every iteration
+        F = eng.make_exits(O_t, P_t)
+        m = torch.empty_like(F)
         H = G = None
+        w = 0.0
         eMods = []
         for i in range(int(iters)):
-            e = eng.hio_modulus(f, m, beta, alpha)  # m: Psup's argument; f: f - beta Pmod(f)
+            # m: Psup's argument; F: f - beta Pmod(f)
+            e = eng.hio_modulus(F, m, O_t, P_t, w, beta, alpha)
             eMods.append(math.sqrt(e / I_norm))
             if update == 'sample':
                 O_t, H = eng.psup_O(m, P_t, H, alpha=alpha)
@@ -93,13 +98,15 @@ def HIO(I, R, P, O, iters, beta=1.0, update='sample', mask=1, alpha=1.0e-10, ful
             else:
                 O_t, _ = eng.psup_O(m, P_t, None, alpha=alpha)
                 P_t, _ = eng.psup_P(eng.make_exits(O_t, P_t), O_t, None, alpha=alpha)
-            eng.add_exits(f, O_t, P_t, beta)
+            w = beta
             if callback is not None:
                 callback(i, O_t.cpu().numpy().astype(c_dtype), P_t.cpu().numpy().astype(c_dtype))
         O_out = O_t.cpu().numpy().astype(c_dtype)
         P_out = P_t.cpu().numpy().astype(c_dtype)
         if not full_output:
             return O_out, P_out
-        return O_out, P_out, {'eMod': eMods, 'exits': f.cpu().numpy().astype(c_dtype)}
+        if w != 0.0:
+            eng.add_exits(F, O_t, P_t, w)
+        return O_out, P_out, {'eMod': eMods, 'exits': F.cpu().numpy().astype(c_dtype)}
     finally:
         eng.close()
