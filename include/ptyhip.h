@@ -138,11 +138,14 @@ int pty_set_launch(pty_ctx* ctx, int32_t rows_mult, int32_t cols_mult, int32_t c
  * chunks. */
 int pty_set_chunk(pty_ctx* ctx, int32_t frames);
 /* ERA object-only steps (methods 1 and 4) after the first: fuse the inverse
- * row FFT of psi' into the object update (one work-group per object row, the
- * same per-pixel summation order as the unfused update), so psi' never goes
- * back to HBM.  Built for K in {64, 96, 128, 192, 256} and V <= 8192; on by
- * default where supported (env PTY_ROW_GATHER=0 turns it off).  enable != 0
- * on an unsupported geometry leaves it off; pty_get_row_gather reports both. */
+ * row FFT of psi' into the object update (one work-group per object row
+ * segment, the same per-pixel summation order as the unfused update), so psi'
+ * never goes back to HBM.  DM with background (methods 4, and 6 on its
+ * object-only iterations): the leading modulus step's inverse row FFT writes
+ * f and projects it onto the object in the same pass.  Built for K in
+ * {64, 96, 128, 192, 256} and V <= 8192; on by default where supported (env
+ * PTY_ROW_GATHER=0 turns it off).  enable != 0 on an unsupported geometry
+ * leaves it off; pty_get_row_gather reports both. */
 int pty_set_row_gather(pty_ctx* ctx, int32_t enable);
 int pty_get_row_gather(pty_ctx* ctx, int32_t* supported, int32_t* enabled);
 typedef struct pty_tune_result {

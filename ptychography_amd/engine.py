@@ -145,7 +145,8 @@ class Engine:
         call("pty_set_chunk", self._ctx, int(frames))
 
     def set_row_gather(self, enable=True):
-        """Fuse the inverse row FFT into ERA's object update (where supported)."""
+        """Fuse the inverse row FFT into ERA's object update and into DM-with-background's
+        leading projection (where supported)."""
         call("pty_set_row_gather", self._ctx, int(bool(enable)))
 
     def row_gather_status(self):
