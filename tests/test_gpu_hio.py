@@ -78,3 +78,40 @@ def test_hio_argument_errors(gpu_device):
         pty.HIO(s.I, s.R.copy(), s.P, None, 1, update="object")
     with pytest.raises(ValueError):
         pty.HIO(s.I, s.R.copy(), s.P, None, 1, beta=0.0)
+
+
+def test_hio_entry_points_reject_bad_arguments(gpu_device):
+    # the C entry points check their arguments before any device work
+    import ctypes
+    from ptychography_amd._abi import PtyError, call
+    from ptychography_amd.engine import Engine, _ptr
+    print("HIO entry points: argument checks", flush=True)
+    s = small_scan(frame=32, grid=(2, 2), step=8)
+    N, M, K = s.I.shape
+    eng = Engine(N, M, K, *s.O.shape, s.R.astype(np.int32), device=gpu_device)
+    try:
+        amp, I_norm = eng.prepare_amp(torch.from_numpy(s.I.astype(np.float32)), None)
+        eng.set_data(amp, None, I_norm)
+        O = torch.from_numpy(s.O.astype(np.complex64)).to(gpu_device)
+        P = torch.from_numpy(s.P.astype(np.complex64)).to(gpu_device)
+        F = eng.make_exits(O, P)
+        m = torch.empty_like(F)
+        e = ctypes.c_double()
+        bad = [(F, m, O, P, 0.0, 0.0),            # beta = 0
+               (F, m, O, P, 0.0, float("nan")),   # beta not finite
+               (F, F, O, P, 0.0, 1.0),            # f aliases m
+               (F, m, None, P, 0.0, 1.0),         # no object
+               (F, m, O, P, float("inf"), 1.0)]   # w not finite
+        for f_, m_, O_, P_, w, beta in bad:
+            with pytest.raises(PtyError) as err:
+                call("pty_hio_modulus", eng._ctx, _ptr(f_), _ptr(m_), _ptr(O_), _ptr(P_), w, beta,
+                     1e-10, ctypes.byref(e))
+            assert err.value.code == -1
+        with pytest.raises(PtyError):
+            call("pty_add_exits", eng._ctx, _ptr(O), None, 1.0, _ptr(F))
+        # a good call after the rejected ones still works
+        call("pty_hio_modulus", eng._ctx, _ptr(F), _ptr(m), _ptr(O), _ptr(P), 0.0, 1.0, 1e-10,
+             ctypes.byref(e))
+        assert np.isfinite(e.value)
+    finally:
+        eng.close()
