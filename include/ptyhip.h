@@ -231,7 +231,7 @@ int pty_back_projection(pty_ctx* ctx, const pty_cfloat* P_dev, pty_cfloat* exits
                         pty_cfloat* O_out_dev, double* H_dev, float alpha);
 
 /* One HIO modulus step (utility_Ptych.py:14-17, the update of the legacy
- * Ptychography.HIO driver, Ptychography_2dsample_2dprobe_farfield.py:93-127)
+ * Ptychography.HIO method, Ptychography_2dsample_2dprobe_farfield.py:93-127)
  * with the bound data.  The exits are f = F + w * make_exits(O, P, R), F in
  * f_dev (w = 0: f = F; w = beta after an iteration leaves its Psup term
  * pending).  m = pmod_1(f) (era.py:550-572) is formed in m_dev and the update's

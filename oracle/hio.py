@@ -1,7 +1,7 @@
 """Oracle: hybrid input-output (HIO) on the ERA path's projections (TEST
 INFRASTRUCTURE ONLY).
 
-Restates the legacy driver ``Ptychography.HIO(iters, beta, update)`` of
+Restates the legacy method ``Ptychography.HIO(iters, beta, update)`` of
 ``ptychography/Ptychography_2dsample_2dprobe_farfield.py:93-127`` with the
 update of ``ptychography/utility_Ptych.py:14-17``::
 
@@ -37,7 +37,7 @@ UPDATES = ('sample', 'probe', 'both')
 
 def HIO(I, R, P, O, iters, beta=1.0, update='sample', mask=1, alpha=1.0e-10,
         full_output=True, dtype=None, callback=None):
-    """Legacy HIO driver (see the module note); returns (O, P, info) or (O, P)."""
+    """Legacy HIO method (see the module note); returns (O, P, info) or (O, P)."""
     if update not in UPDATES:
         raise ValueError(f"update must be one of {UPDATES}")
     if beta == 0:

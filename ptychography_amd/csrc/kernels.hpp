@@ -90,7 +90,8 @@ enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2, SRC_HIO = 3 };
 //   SRC_HIO: F[n, i, c] + w O[..] P[i, c]  (HIO's iterate with its pending Psup term)
 // The sources that also read f (DM, HIO) hold two line groups of inputs in
 // registers: at three work-groups per CU they spilled 88 bytes per lane; at
-// two they do not (DM with background: 9.61 vs 9.65-9.70 ms per iteration).
+// two they do not (DM with background at 4096 x 256^2 frames: 9.61 vs
+// 9.65-9.70 ms per iteration).
 template <int L, int SRC>
 __global__ __launch_bounds__(WG, (SRC == SRC_DM || SRC == SRC_HIO) ? 2 : 3)
 void k_rows_fwd(Geom g, const float2* O, const float2* P,
