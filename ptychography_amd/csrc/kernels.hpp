@@ -208,8 +208,17 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (grp + (int)gridDim.x < ngroups) load(grp + gridDim.x);
-    fft_inv<L>(v, lds + line * W, s_tw, t);
     const int row = grp * LPW + line;
+    // HIO reads its iterate F at every pixel: issue those loads before the
+    // inverse FFT, whose LDS exchanges hide them (DM's update, at its register
+    // budget, measured slower the same way)
+    float2 fpre[DST == DST_HIO ? E : 1];
+    if constexpr (DST == DST_HIO) {
+      const size_t rb = (size_t)min(row, nrows - 1) * L;
+#pragma unroll
+      for (int e = 0; e < E; ++e) fpre[e] = load_stream(&hm.F[rb + layA<L>(e, t)]);
+    }
+    fft_inv<L>(v, lds + line * W, s_tw, t);
     if (row < nrows) {
       const size_t rbase = (size_t)row * L;
       if constexpr (DST == DST_STORE) {
@@ -224,7 +233,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
         for (int e = 0; e < E; ++e) {
           const int c = layA<L>(e, t);
           const float2 m = v[e] * scale;
-          float2 fv = load_stream(&hm.F[rbase + c]);
+          float2 fv = fpre[e];
           if (hm.w != 0.f) {
             const float2 ex = cmul(Orow[c], Prow[c]);
             fv = make_float2(fv.x + hm.w * ex.x, fv.y + hm.w * ex.y);
