@@ -47,10 +47,13 @@ def test_era_graphs_bit_identical(frame, gpu_device):
         assert np.array_equal(x, y)
 
 
-def test_dm_background_graphs_bit_identical(gpu_device):
-    print("graphs DM method 4", flush=True)
-    s = small_scan(frame=32, grid=(4, 4), step=6, jitter=1, background_frac=0.02)
-    bg = np.sqrt(np.random.default_rng(2).random((32, 32)) * 0.5)
+@pytest.mark.parametrize("frame", [32, 64])
+def test_dm_background_graphs_bit_identical(frame, gpu_device):
+    # at 64^2 the leading projection runs fused into its inverse row FFT
+    # (k_rows_gather writing f) inside the replayed graphs
+    print(f"graphs DM method 4 frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=(4, 4), step=frame * 3 // 16, jitter=1, background_frac=0.02)
+    bg = np.sqrt(np.random.default_rng(2).random((frame, frame)) * 0.5)
     a = _run(s, gpu_device, 1, 4, False, s.P, bg=bg)
     b = _run(s, gpu_device, 1, 4, True, s.P, bg=bg)
     for x, y in zip(a, b):
