@@ -89,7 +89,7 @@ constexpr int ESUP_BLOCKS = 1024;
 // host-side state an iteration changes; a graph replay restores what its
 // capture left
 struct HostIterState {
-  bool H_valid, G_valid, era_bg_shared, have_metric_state;
+  bool H_valid, G_valid, era_bg_shared, have_metric_state, os_next;
   float2* F;
   float2* X;
 };
@@ -198,6 +198,9 @@ struct pty_ctx {
 
   // DM bookkeeping
   bool dm_ready = false;
+  // Os holds the next object-only DM projection of f (the last iteration's
+  // metric projection, with the same f, P and heatmap); cleared by every run
+  bool os_next = false;
   bool have_metric_state = false;
   int last_algo = -1;
 
@@ -908,7 +911,15 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     PTY_TRY(bg_mean(c, nchunks, c->bbar));
   }
 
-  if (!fused_proj) PTY_TRY(dm_project(c, o, it, upd, op_step, c->O, c->P, false));
+  if (!bgm && upd == 0 && c->os_next) {
+    // dm.py:85: the projection of this f with P and the heatmap is the one the
+    // previous iteration's metric step computed into Os (f, P and the heatmap
+    // have not changed since)
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->O, c->Os, UV * sizeof(float2)));
+  } else if (!fused_proj) {
+    PTY_TRY(dm_project(c, o, it, upd, op_step, c->O, c->P, false));
+  }
   if (pmodp) PTY_TRY(pmod_probe(c, c->P, o.alpha));
 
   // f <- f - Ps f + Pm(2 Ps f - f)    (dm.py:91-97, :216-224)
@@ -949,6 +960,7 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     PTY_TRY(copy_d2d(c, c->Ps, c->P, MK * sizeof(float2)));
   }
   PTY_TRY(dm_project(c, o, it, upd, op_step, c->Os, c->Ps, true));
+  c->os_next = !bgm && upd == 0;   // Ps == P here; pmod_probe below changes only Ps
   if (pmodp) PTY_TRY(pmod_probe(c, c->Ps, o.alpha));
   if (bgm) PTY_TRY(bg_mean(c, nchunks, c->bbar_m));
   int ncol = 0;
@@ -1007,6 +1019,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(upd == 2 && (it % o.op_every) == 0), (uint64_t)(it < o.pmod_probe_until),
           (uint64_t)o.probe_centering, bits((double)o.alpha), bits(c->I_norm),
           (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
+          (uint64_t)c->os_next,
           (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
           (uint64_t)c->rows_mult, (uint64_t)c->cols_mult, (uint64_t)c->cols_wg,
           (uint64_t)c->chunk_frames, (uint64_t)c->rowg,
@@ -1017,13 +1030,14 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
 }
 
 HostIterState save_state(const pty_ctx* c) {
-  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->F, c->X};
+  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->os_next, c->F, c->X};
 }
 void load_state(pty_ctx* c, const HostIterState& st) {
   c->H_valid = st.H_valid;
   c->G_valid = st.G_valid;
   c->era_bg_shared = st.era_bg_shared;
   c->have_metric_state = st.have_metric_state;
+  c->os_next = st.os_next;
   c->F = st.F;
   c->X = st.X;
 }
@@ -1114,6 +1128,7 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
   if (o->op_n < 0 || o->op_every < 1) return fail(PTY_EINVAL, "bad OP_iters");
   if (iters < 0) return fail(PTY_EINVAL, "iters < 0");
   PTY_TRY(check_bound(c, o->method >= 4));
+  c->os_next = false;   // the caller may have changed O, P or the data since
   if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
   if (iters > c->metrics_cap) {
     dev_free(c->metrics);
