@@ -911,10 +911,12 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     PTY_TRY(bg_mean(c, nchunks, c->bbar));
   }
 
-  if (!bgm && upd == 0 && c->os_next) {
-    // dm.py:85: the projection of this f with P and the heatmap is the one the
-    // previous iteration's metric step computed into Os (f, P and the heatmap
-    // have not changed since)
+  // object-only projections: upd 0, or upd 2 away from its OP iterations
+  const bool plain_proj = upd == 0 || (upd == 2 && !op_step);
+  if (!bgm && plain_proj && c->os_next) {
+    // dm.py:85 / :130: the projection of this f with P and the heatmap is the
+    // one the previous iteration's metric step computed into Os (f, P and the
+    // heatmap have not changed since)
     StageTimer tm(c, PTY_STAGE_MISC);
     PTY_TRY(copy_d2d(c, c->O, c->Os, UV * sizeof(float2)));
   } else if (!fused_proj) {
@@ -960,7 +962,9 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     PTY_TRY(copy_d2d(c, c->Ps, c->P, MK * sizeof(float2)));
   }
   PTY_TRY(dm_project(c, o, it, upd, op_step, c->Os, c->Ps, true));
-  c->os_next = !bgm && upd == 0;   // Ps == P here; pmod_probe below changes only Ps
+  // a plain metric projection used P (Ps == P) and the heatmap; pmod_probe
+  // below changes only Ps, and P changes only on OP iterations
+  c->os_next = !bgm && plain_proj;
   if (pmodp) PTY_TRY(pmod_probe(c, c->Ps, o.alpha));
   if (bgm) PTY_TRY(bg_mean(c, nchunks, c->bbar_m));
   int ncol = 0;
