@@ -99,3 +99,18 @@ def test_non_finite_data_raise(gpu_device):
         pty.ERA(I, s.R.copy(), s.P, None, 3, method=1, device=gpu_device)
     O, info = pty.ERA(I, s.R.copy(), s.P, None, 3, method=1, device=gpu_device, check_finite=False)
     assert not np.isfinite(info["eMod"][0])
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("method,op", [(1, (1, 1)), (3, (1, 2))])
+def test_dm_projection_reuse_is_exact(method, op, graphs, gpu_device):
+    # one run of 8 iterations starts each object-only projection from the
+    # previous iteration's metric projection; 8 runs of one iteration never do
+    # (every run clears it): the iterates and metrics must agree bit for bit
+    print(f"DM method {method} OP_iters {op} graphs={graphs}: projection reuse", flush=True)
+    s = small_scan(frame=64, grid=(4, 4), step=12, jitter=1)
+    P0 = s.P if method == 1 else flat_start(s)
+    a = _run(s, gpu_device, 1, method, graphs, P0, chunks=(8,), OP_iters=op)
+    b = _run(s, gpu_device, 1, method, graphs, P0, chunks=(1,) * 8, OP_iters=op)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
