@@ -29,11 +29,23 @@ own formula; the tests live in ``tests/test_oracle_*.py``):
                                    the exits are P, O = H / (H + alpha)) and the
                                    exit spectra's modulus (mask * sqrt I) and
                                    phase (the probe's far field).
-* ``ERA`` (methods 1-6)         -- monotone eMod for method 1, exact-data fixed
-                                   point, an independent pure-loop ERA on tiny
-                                   inputs (direct DFT, explicit loops).
-* ``DM`` (methods 1-6)          -- exact-data fixed point, an independent
-                                   pure-loop DM on tiny inputs.
+* ``ERA`` / ``DM`` (methods 1-6) -- ``tests/loop_oracle.py``, a second
+                                   restatement of the reference loops with
+                                   explicit per-pixel loops and a direct DFT
+                                   (no code shared with this package): same O,
+                                   P, eMod, eCon, eSup, info['I'], heatmap and
+                                   background after every iteration for methods
+                                   1-6, OP_iters (1,2) / (2,3) (the stale-heatmap
+                                   branch), probe centring, the Pmod_probe
+                                   window and a mask (``tests/test_oracle_loops.py``,
+                                   which also shows that four planted mistakes
+                                   break the comparison).  Properties: ERA
+                                   eMod_{i+1} <= eSup_i <= eMod_i (methods 1-3),
+                                   non-increasing eMod with eSup_i <= eMod_i
+                                   (methods 4-6), the exact-data fixed point
+                                   (ERA 1; DM 1, 3 and 4 with a known
+                                   background), the centring roll against
+                                   scipy's centroid, the Pmod_probe window.
 * ``hio.HIO``                   -- Fienup's classic input-output rule, to which
                                    HIO reduces when Psup is a 0/1 support
                                    projection (binary probe, frames that do not
