@@ -104,6 +104,23 @@ int pty_heatmap(pty_ctx* ctx, double* H_dev, int32_t* valid);
 /* Current frame-averaged background (M x K float32, ifftshifted). */
 int pty_background(pty_ctx* ctx, float* b_dev);
 
+/* Resume from a recorded state: the values the reference's loop carries from
+ * one iteration to the next besides O and P (load them after pty_set_state,
+ * then pty_run with begin = 0 and iter0 = the next iteration's number).
+ *   pty_set_heatmap: the cached P heatmap H (U x V float64, era.py:143,163-189)
+ *     and O heatmap G (M x K float64, era.py:144,164); NULL = not cached (the
+ *     reference's None: recomputed when next needed).
+ *   pty_set_iterate: DM's iterate f (N x M x K, dm.py:45-97; NULL keeps it)
+ *     and the eCon reference, the previous iteration's metric object and probe
+ *     (dm.py:35-38,158-160; both NULL = the bound O and P).  The per-frame
+ *     background state goes through pty_set_background(per_frame = 1).
+ *   pty_get_iterate: copies out f (N x M x K, optional) and the per-frame
+ *     background state (N x M x K float32, optional). */
+int pty_set_heatmap(pty_ctx* ctx, const double* H_dev, const double* G_dev);
+int pty_set_iterate(pty_ctx* ctx, const pty_cfloat* f_dev, const pty_cfloat* O_ref_dev,
+                    const pty_cfloat* P_ref_dev);
+int pty_get_iterate(pty_ctx* ctx, pty_cfloat* f_dev, float* b_dev);
+
 /* Choose between the fused per-frame kernel (one work-group transforms a whole
  * frame through an Infinity-Cache-resident scratch slot; square 128 / 256
  * frames, methods without background) and the staged row / column kernels.

@@ -57,7 +57,8 @@ def _project(update, i, OP_iters, exits, O, P, R, P_heatmap, O_heatmap, alpha,
 
 def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
        probe_centering=False, method=None, hardware='cpu', alpha=1.0e-10,
-       dtype=None, full_output=True, callback=None, rng=None):
+       dtype=None, full_output=True, callback=None, rng=None, resume=None,
+       state_callback=None):
     """Difference map, one-rank restatement of dm.py:13-333.
 
     f_{i+1} = f_i - Ps f_i + Pm(2 Ps f_i - f_i)  (dm.py:94-97); metrics use
@@ -67,20 +68,25 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
     st = _era.preamble(I, R, P, O, iters, OP_iters, mask, background, method,
                        alpha, dtype, rng)
     method, update, OP_iters = st['method'], st['update'], st['OP_iters']
+    bg_mode = method in (4, 5, 6)
+    i0, P_heatmap, O_heatmap, background = _era._resume(st, resume, bg_mode)
     O, P, amp, Pamp = st['O'], st['P'], st['amp'], st['Pamp']
-    background, R, mask = st['background'], st['R'], st['mask']
+    R, mask = st['R'], st['mask']
     I_norm, exits = st['I_norm'], st['exits']
     MK = P.shape[0] * P.shape[1]
-    bg_mode = method in (4, 5, 6)
 
-    P_heatmap = None
-    O_heatmap = None
     eMods, eCons, eSups = [], [], []
     bak = _era._state(update, O, P).copy()
+    if resume is not None:
+        if resume.get('exits') is not None:
+            exits = np.array(resume['exits'], dtype=st['c_dtype'])
+        if resume.get('O_ref') is not None:
+            bak = _era._state(update, np.asarray(resume['O_ref'], st['c_dtype']),
+                              np.asarray(resume['P_ref'], st['c_dtype'])).copy()
     Os, Ps = O, P
     b_0 = np.empty_like(background) if bg_mode else None
 
-    for i in range(iters):
+    for i in range(i0, i0 + iters):
         if bg_mode:
             # leading modulus projection of f and b (dm.py:171)
             exits, background = _era.pmod_7(amp, background, exits, mask, alpha=alpha)
@@ -136,6 +142,11 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
         bak = cur.copy()
         if callback is not None:
             callback(i, O, P)
+        if state_callback is not None:
+            state_callback(i, dict(O=O.copy(), P=P.copy(), exits=exits.copy(),
+                                   background=None if background is None else background.copy(),
+                                   P_heatmap=_era._copy(P_heatmap), O_heatmap=_era._copy(O_heatmap),
+                                   O_ref=Os.copy(), P_ref=Ps.copy()))
 
     if not full_output:
         if update == 'O':

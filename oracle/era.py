@@ -16,6 +16,15 @@ Build additions (not in the reference, marked where they occur):
 * ``rng`` -- where the reference draws unseeded ``np.random.random`` numbers
   (default probe, default background) this uses ``np.random`` too, unless a
   ``numpy.random.Generator``-like object with ``.random(shape)`` is given.
+* ``resume`` / ``state_callback`` -- single-step checks start an iteration
+  from a recorded state: ``resume`` may hold ``i0`` (the global number of the
+  first iteration, which drives the OP_iters schedule and the Pmod_probe
+  window), the cached ``P_heatmap`` / ``O_heatmap``, the per-frame
+  ``background`` state, ``Pamp`` (the far-field probe modulus of the run's
+  first probe) and, for DM, the iterate ``exits`` and the eCon reference
+  ``O_ref`` / ``P_ref``.  ``state_callback(i, state)`` receives the
+  same quantities (copies) after iteration i.  Without them the loop is the
+  reference's.
 """
 
 import numpy as np
@@ -305,24 +314,41 @@ def _centre(P, O):
 # ERA (era.py:13-333)
 # ---------------------------------------------------------------------------
 
+def _copy(x):
+    return None if x is None else np.array(x, copy=True)
+
+
+def _resume(st, resume, bg_mode):
+    """(i0, P_heatmap, O_heatmap, background) of a resumed run."""
+    background = st['background']
+    if resume is None:
+        return 0, None, None, background
+    if resume.get('Pamp') is not None:
+        st['Pamp'] = np.asarray(resume['Pamp'], dtype=st['dtype'])
+    if bg_mode and resume.get('background') is not None:
+        background = np.array(resume['background'], dtype=st['dtype'])
+    return (int(resume.get('i0', 0)), _copy(resume.get('P_heatmap')),
+            _copy(resume.get('O_heatmap')), background)
+
+
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
         Pmod_probe=False, probe_centering=False, hardware='cpu', alpha=1.0e-10,
-        dtype=None, full_output=True, callback=None, rng=None):
+        dtype=None, full_output=True, callback=None, rng=None, resume=None,
+        state_callback=None):
     """Error-reduction ptychography, one-rank restatement of era.py:13-333."""
     st = preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha,
                   dtype, rng)
     method, update, OP_iters = st['method'], st['update'], st['OP_iters']
+    bg_mode = method in (4, 5, 6)
+    i0, P_heatmap, O_heatmap, background = _resume(st, resume, bg_mode)
     O, P, amp, Pamp = st['O'], st['P'], st['amp'], st['Pamp']
-    background, R, mask = st['background'], st['R'], st['mask']
+    R, mask = st['R'], st['mask']
     I_norm, exits = st['I_norm'], st['exits']
     MK = P.shape[0] * P.shape[1]
 
-    P_heatmap = None
-    O_heatmap = None
     eMods, eCons, eSups = [], [], []
-    bg_mode = method in (4, 5, 6)
 
-    for i in range(iters):
+    for i in range(i0, i0 + iters):
         bak = _state(update, O, P).copy()
 
         # modulus projection (era.py:160 / :230)
@@ -367,6 +393,10 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
         eMods.append(np.sqrt(eMod / I_norm))
         if callback is not None:
             callback(i, O, P)
+        if state_callback is not None:
+            state_callback(i, dict(O=O.copy(), P=P.copy(), exits=exits.copy(),
+                                   background=_copy(background) if bg_mode else None,
+                                   P_heatmap=_copy(P_heatmap), O_heatmap=_copy(O_heatmap)))
 
     if not full_output:
         if update == 'O':

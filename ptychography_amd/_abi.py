@@ -58,6 +58,9 @@ SIGNATURES = {
     "pty_intensity": (ctypes.c_int, [_vp, _vp]),
     "pty_heatmap": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_i32)]),
     "pty_background": (ctypes.c_int, [_vp, _vp]),
+    "pty_set_heatmap": (ctypes.c_int, [_vp, _vp, _vp]),
+    "pty_set_iterate": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "pty_get_iterate": (ctypes.c_int, [_vp, _vp, _vp]),
     "pty_set_path": (ctypes.c_int, [_vp, _i32, _i32, _i32]),
     "pty_set_graphs": (ctypes.c_int, [_vp, _i32]),
     "pty_profile": (ctypes.c_int, [_vp, _i32]),

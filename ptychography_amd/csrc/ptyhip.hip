@@ -1902,6 +1902,57 @@ int pty_background(pty_ctx* c, float* b) {
   return PTY_OK;
 }
 
+int pty_set_heatmap(pty_ctx* c, const double* H, const double* G) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (H) PTY_TRY(copy_d2d(c, c->H, H, (size_t)c->U * c->V * sizeof(double)));
+  if (G) PTY_TRY(copy_d2d(c, c->G, G, c->MK() * sizeof(double)));
+  c->H_valid = H != nullptr;
+  c->G_valid = G != nullptr;
+  c->os_next = false;
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_set_iterate(pty_ctx* c, const pty_cfloat* f, const pty_cfloat* O_ref, const pty_cfloat* P_ref) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (!c->O || !c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
+  if ((O_ref == nullptr) != (P_ref == nullptr))
+    return fail(PTY_EINVAL, "O_ref and P_ref go together");
+  if (f) {
+    PTY_TRY(ensure_F(c));
+    PTY_TRY(copy_d2d(c, c->F, f, c->NMK() * sizeof(float2)));
+    c->dm_ready = true;
+  }
+  const size_t UV = (size_t)c->U * c->V;
+  PTY_TRY(copy_d2d(c, c->Obak, O_ref ? (const void*)O_ref : (const void*)c->O, UV * sizeof(float2)));
+  PTY_TRY(copy_d2d(c, c->Pbak, P_ref ? (const void*)P_ref : (const void*)c->P, c->MK() * sizeof(float2)));
+  c->have_metric_state = false;
+  c->os_next = false;
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_get_iterate(pty_ctx* c, pty_cfloat* f, float* b) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (f) {
+    if (!c->F || !c->dm_ready) return fail(PTY_ESTATE, "no DM iterate (run DM or pty_set_iterate first)");
+    PTY_TRY(copy_d2d(c, f, c->F, c->NMK() * sizeof(float2)));
+  }
+  if (b) {
+    if (!c->bg_set) return fail(PTY_ESTATE, "no background state");
+    if (c->last_algo == 0 && c->era_bg_shared) {
+      // ERA keeps the frame mean once it has been taken (era.py:265-270)
+      hipLaunchKernelGGL(k_broadcast, dim3(4096), dim3(WG), 0, c->stream, b, (const float*)c->bbar,
+                         c->MK(), c->NMK());
+      PTY_LAUNCHED("k_broadcast");
+    } else {
+      PTY_TRY(copy_d2d(c, b, c->bg, c->NMK() * sizeof(float)));
+    }
+  }
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
 int pty_fft2(pty_ctx* c, pty_cfloat* data, int32_t nframes, int32_t inverse) {
   if (!c || !data) return fail(PTY_EINVAL, "NULL argument");
   if (nframes < 0) return fail(PTY_EINVAL, "nframes < 0");

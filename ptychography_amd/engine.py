@@ -205,6 +205,31 @@ class Engine:
         call("pty_background", self._ctx, _ptr(b))
         return b
 
+    # -- resuming from a recorded state ---------------------------------------
+    def set_heatmap(self, H=None, G=None):
+        """Cached P heatmap H (U x V) and O heatmap G (M x K), float64; None = not cached."""
+        H = None if H is None else H.to(self.device, torch.float64).contiguous()
+        G = None if G is None else G.to(self.device, torch.float64).contiguous()
+        self._sync_torch()
+        call("pty_set_heatmap", self._ctx, _ptr(H), _ptr(G))
+
+    def set_iterate(self, f=None, O_ref=None, P_ref=None):
+        """DM iterate f (N x M x K complex64) and the eCon reference (previous
+        metric object and probe; None = the bound O and P)."""
+        ts = [None if t is None else t.to(self.device, torch.complex64).contiguous()
+              for t in (f, O_ref, P_ref)]
+        self._sync_torch()
+        call("pty_set_iterate", self._ctx, *[_ptr(t) for t in ts])
+
+    def get_iterate(self, f=True, background=False):
+        """(f, per-frame background) copies; either may be skipped (None)."""
+        ft = torch.empty((self.N, self.M, self.K), dtype=torch.complex64, device=self.device) \
+            if f else None
+        bt = torch.empty((self.N, self.M, self.K), dtype=torch.float32, device=self.device) \
+            if background else None
+        call("pty_get_iterate", self._ctx, _ptr(ft), _ptr(bt))
+        return ft, bt
+
     # -- building blocks ----------------------------------------------------
     def fft2(self, data, inverse=False):
         """In-place batched 2-D FFT of a (n, M, K) complex64 tensor."""

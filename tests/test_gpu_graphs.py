@@ -102,15 +102,22 @@ def test_non_finite_data_raise(gpu_device):
 
 
 @pytest.mark.parametrize("graphs", [False, True])
-@pytest.mark.parametrize("method,op", [(1, (1, 1)), (3, (1, 2))])
-def test_dm_projection_reuse_is_exact(method, op, graphs, gpu_device):
+@pytest.mark.parametrize("method,op,centre,pmodp", [
+    (1, (1, 1), False, 0), (3, (1, 2), False, 0),
+    # two non-OP iterations in a row (the reuse happens), with the centring
+    # of the OP iterations resetting the heatmaps and the far-field probe
+    # constraint changing the metric probe only
+    (3, (1, 3), True, 5), (3, (2, 3), True, 5), (1, (1, 1), False, 5)])
+def test_dm_projection_reuse_is_exact(method, op, centre, pmodp, graphs, gpu_device):
     # one run of 8 iterations starts each object-only projection from the
     # previous iteration's metric projection; 8 runs of one iteration never do
     # (every run clears it): the iterates and metrics must agree bit for bit
-    print(f"DM method {method} OP_iters {op} graphs={graphs}: projection reuse", flush=True)
+    print(f"DM method {method} OP_iters {op} centring {centre} Pmod_probe {pmodp} "
+          f"graphs={graphs}: projection reuse", flush=True)
     s = small_scan(frame=64, grid=(4, 4), step=12, jitter=1)
-    P0 = s.P if method == 1 else flat_start(s)
-    a = _run(s, gpu_device, 1, method, graphs, P0, chunks=(8,), OP_iters=op)
-    b = _run(s, gpu_device, 1, method, graphs, P0, chunks=(1,) * 8, OP_iters=op)
+    P0 = s.P if method == 1 else np.roll(flat_start(s), (2, 3), axis=(0, 1))
+    kw = dict(OP_iters=op, probe_centering=centre, Pmod_probe_until=pmodp)
+    a = _run(s, gpu_device, 1, method, graphs, P0, chunks=(8,), **kw)
+    b = _run(s, gpu_device, 1, method, graphs, P0, chunks=(1,) * 8, **kw)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)

@@ -317,3 +317,31 @@ def test_planted_oracle_mistakes_are_caught(monkeypatch, plant, case):
     assert _max_traj_diff(alg, pb, 6, **kw) < 1e-10
     plant(monkeypatch)
     assert _max_traj_diff(alg, pb, 6, **kw) > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# the oracle's resume hook (used by the single-step GPU parity tests)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[2] != 1 or c[1] in (1, 4)],
+                         ids=lambda c: f"{c[0]}{c[1]}-{c[2]}-{c[3]}-{c[4]}")
+def test_oracle_resume_reproduces_the_run(case):
+    alg, method, OP, centre, pmodp, masked, offc = case
+    pb = _problem(seed=200 + method + 10 * (alg == "DM"), masked=masked, bg=method >= 4,
+                  offcentre=offc)
+    fn = oracle.ERA if alg == "ERA" else oracle.DM
+    kw = dict(OP_iters=OP, method=method, Pmod_probe=pmodp, probe_centering=centre,
+              mask=pb["mask"], background=pb["background"])
+    states = []
+    full = fn(pb["I"], pb["R"].copy(), pb["P"].copy(), pb["O"].copy(), 6,
+              state_callback=lambda i, st: states.append(st), **kw)
+    st = states[2]
+    resume = dict(i0=3, P_heatmap=st["P_heatmap"], O_heatmap=st["O_heatmap"],
+                  background=st["background"])
+    if alg == "DM":
+        resume.update(exits=st["exits"], O_ref=st["O_ref"], P_ref=st["P_ref"])
+    part = fn(pb["I"], pb["R"].copy(), st["P"], st["O"], 3, resume=resume, **kw)
+    for a, b in zip(full[:-1], part[:-1]):
+        assert np.array_equal(a, b)
+    for key in ("eMod", "eCon", "eSup"):
+        assert full[-1][key][3:] == part[-1][key]
