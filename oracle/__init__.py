@@ -15,7 +15,9 @@ Pinning status (what checks each function against something other than its
 own formula; the tests live in ``tests/test_oracle_*.py``):
 
 * ``multiroll``                 -- nested ``np.roll`` and a brute-force index loop.
-* ``make_exits``                -- brute-force index loop; periodic-roll formula.
+* ``make_exits``                -- brute-force index loop; periodic-roll formula;
+                                   the "slice" window mode is bit-identical
+                                   to the reference's roll.
 * ``psup_O`` / ``psup_P``       -- brute-force loops, the adjoint identity, the
                                    exact-data fixed point.
 * ``make_P_heatmap`` / ``make_O_heatmap`` -- adjoint applied to |P|^2 / |O|^2.
@@ -59,11 +61,11 @@ own formula; the tests live in ``tests/test_oracle_*.py``):
 
 from .era import (ERA, preamble, pmod_1, Pmod_1, pmod_7, Pmod_7, Pmod_P, psup_O,
                   psup_P, make_P_heatmap, make_O_heatmap, make_exits, multiroll,
-                  center_of_mass, probe_centre_shift)
+                  center_of_mass, probe_centre_shift, use_windows)
 from .dm import DM, model_error_1
 from .hio import HIO
 
 __all__ = ["ERA", "DM", "preamble", "pmod_1", "Pmod_1", "pmod_7", "Pmod_7",
            "Pmod_P", "psup_O", "psup_P", "make_P_heatmap", "make_O_heatmap",
            "make_exits", "multiroll", "center_of_mass", "probe_centre_shift",
-           "model_error_1", "HIO"]
+           "model_error_1", "HIO", "use_windows"]

@@ -47,14 +47,53 @@ def multiroll(x, shift):
     return np.roll(x, shift, axis=tuple(range(x.ndim)))
 
 
+# How frame windows of the object are read.  "roll" rolls the whole object per
+# frame, as the reference's make_exits / make_O_heatmap do: its cost model,
+# which the timed CPU baseline keeps.  "slice" reads the same window values
+# directly; when every window lies inside the object (R <= 0 after the
+# preamble, window inside U x V) the results are bit-identical (the same
+# values, multiplied and summed in the same order), which
+# tests/test_oracle_pinning.py checks.  Used for the larger fixture runs.
+WINDOWS = "roll"
+
+
+class use_windows:
+    """Context manager selecting the window mode ("roll" or "slice")."""
+
+    def __init__(self, mode):
+        if mode not in ("roll", "slice"):
+            raise ValueError("mode must be 'roll' or 'slice'")
+        self.mode = mode
+
+    def __enter__(self):
+        global WINDOWS
+        self.saved, WINDOWS = WINDOWS, self.mode
+        return self
+
+    def __exit__(self, *exc):
+        global WINDOWS
+        WINDOWS = self.saved
+
+
+def _inside(R, M, K, shape):
+    R = np.asarray(R)
+    return (R.max() <= 0 and (M - R[:, 0]).max() <= shape[0] and (K - R[:, 1]).max() <= shape[1])
+
+
 def make_exits(O, P, R, exits=None):
     """Exit waves ``psi_n = roll(O, r_n)[:M, :K] * P`` (era.py:426-433).
 
-    This keeps the reference's cost model: every frame rolls the whole object.
+    In the default "roll" mode this keeps the reference's cost model: every
+    frame rolls the whole object.
     """
     if exits is None:
         exits = np.empty((len(R),) + P.shape, dtype=P.dtype)
     M, K = P.shape
+    if WINDOWS == "slice" and _inside(R, M, K, O.shape):
+        for n in range(len(R)):
+            r0, r1 = int(R[n][0]), int(R[n][1])
+            exits[n] = O[-r0:M - r0, -r1:K - r1] * P
+        return exits
     for n in range(len(R)):
         exits[n] = multiroll(O, (R[n][0], R[n][1]))[:M, :K] * P
     return exits
@@ -72,8 +111,15 @@ def make_P_heatmap(P, R, shape):
 
 def make_O_heatmap(O, R, shape):
     """Sum over frames of the |O|^2 window, cropped to the probe shape (era.py:412-417)."""
-    heat = np.zeros(O.shape, dtype=O.real.dtype)
     o2 = (O * O.conj()).real
+    M, K = shape
+    if WINDOWS == "slice" and _inside(R, M, K, O.shape):
+        heat = np.zeros((M, K), dtype=O.real.dtype)
+        for r in R:
+            r0, r1 = int(r[0]), int(r[1])
+            heat += o2[-r0:M - r0, -r1:K - r1]
+        return heat
+    heat = np.zeros(O.shape, dtype=O.real.dtype)
     for r in R:
         heat += multiroll(o2, (r[0], r[1]))
     return heat[:shape[0], :shape[1]]

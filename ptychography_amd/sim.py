@@ -213,6 +213,67 @@ def make_scan(spec="c1", noise=None, **overrides):
     return Scan(I=I, R=R, mask=mask, P=P, O=O, B=B, spec=spec)
 
 
+def forward_sim(shape_P=(32, 64), shape_O=(128, 128), A=14, defocus=0., photons_pupil=1, ny=10,
+                nx=10, random_offset=None, background=None, mask=None, counting_statistics=False,
+                rng=None):
+    """The reference's ``forward_sim`` (ptychography/forward_models.py:12-203):
+    same arguments and returns ``(I, R, M, P, O, B)``, the same steps, with
+    every random draw taken from ``rng`` (a ``numpy.random.Generator``; default
+    seed 0) and a smoothed random object in place of the reference's two
+    bundled images.
+
+    * P: far-field disc of radius A times sqrt(photons_pupil) and the defocus
+      phase exp(-i pi defocus q^2), brought to the sample plane
+      (forward_models.py:125-138);
+    * R: an ny x nx grid from rint(linspace(0, M - U, n)), plus integer
+      offsets floor(random * random_offset), clipped to the object, shifted
+      so each column's maximum is 0 (:149-163);
+    * I = ifftshift(|fftn(O_n P)|^2) (:165-171); B = rint(random * background)
+      added to every frame (:173-178); M: `mask` random dead pixels
+      (False = bad) (:180-186); Poisson counts if counting_statistics (:189-190).
+    """
+    rng = np.random.default_rng(0) if rng is None else rng
+    Mf, Kf = shape_P
+    i = np.fft.fftfreq(Mf, 1.0 / Mf)
+    j = np.fft.fftfreq(Kf, 1.0 / Kf)
+    qi, qj = np.meshgrid(i, j, indexing="ij")
+    q2 = np.fft.fftshift(qi ** 2 + qj ** 2)
+    P = (q2 < A ** 2).astype(np.complex128) * np.sqrt(photons_pupil)
+    P *= np.exp(-1.0j * np.pi * defocus * q2)
+    P = np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(P)))
+
+    O = make_object(shape_O, rng, amp_range=(0.0, 1.0))
+
+    ys = np.rint(np.linspace(0, Mf - shape_O[0], ny)).astype(np.int64)
+    xs = np.rint(np.linspace(0, Kf - shape_O[1], nx)).astype(np.int64)
+    ys, xs = np.meshgrid(ys, xs, indexing="ij")
+    R = np.stack([ys.ravel(), xs.ravel()], axis=1)
+    if random_offset is not None:
+        R += (rng.random(R.shape) * int(random_offset)).astype(np.int64)
+        R[R > 0] = 0
+        R[:, 0] = np.maximum(R[:, 0], Mf - shape_O[0])
+        R[:, 1] = np.maximum(R[:, 1], Kf - shape_O[1])
+    R[:, 0] -= R[:, 0].max()
+    R[:, 1] -= R[:, 1].max()
+
+    I = frames_intensity(O, P, -R)
+    if background is not None:
+        B = np.rint(rng.random(I[0].shape) * background)
+        I += B
+    else:
+        B = 0
+    if mask is not None:
+        ss = (rng.random(mask) * I.shape[1]).astype(np.int64)
+        fs = (rng.random(mask) * I.shape[2]).astype(np.int64)
+        Mask = np.ones(I[0].shape, dtype=bool)
+        Mask[ss, fs] = False
+    else:
+        Mask = 1
+    if counting_statistics:
+        I = rng.poisson(I).astype(np.float64)
+    return I, R, Mask, P, O, B
+
+
 def c4_specs(count=16, seed0=100, frames=(64, 96, 128, 192, 256), positions=(100, 2000)):
     """The mixed multi-scan workload: scan s (seed seed0 + s) draws its frame
     side from ``frames`` and its position count from ``positions``, a raster

@@ -1,0 +1,173 @@
+"""Write the oracle fixtures the GPU parity tests of the larger workloads read.
+
+Run from the repository root:  python -m tests.golden.make_fixtures [name ...]
+
+The complex128 oracle (with the complex64 oracle beside it, for the drift
+bounds) is too slow to run inside a GPU test at these sizes, so its results
+are computed here once and committed as small .npz files:
+
+* ``c2_era3.npz`` / ``c2_dm3.npz`` -- the 1024 x 128^2 workload (Zernike
+  probe, beamstop plus dead pixels, Poisson noise at 1e7 photons), ERA method
+  3 and DM method 3 for 200 iterations: eMod / eCon / eSup per iteration of
+  both precisions; at iterations 1, 20 and 200 the probe and every 8th object
+  pixel of the complex128 run, and the complex64 run's relative error there.
+* ``c4_era3.npz`` -- the 16 mixed scans, ERA method 3, 5 iterations each:
+  per-scan metrics of both precisions.
+* ``chain_test_py.npz`` -- the reference's own caller (test.py:31-49): a
+  forward_sim scan with 100 dead pixels raised by 1e4 and a random-phase
+  probe, DM method 3 with probe centring then ERA method 3 with centring,
+  10 iterations each: metrics of both precisions.
+
+Inputs are regenerated by the tests from the same seeds (``sim`` and the
+start guesses below), so only results are stored.  The oracle reads windows
+by slicing here (``oracle.use_windows("slice")``, bit-identical to the
+reference's roll).
+"""
+
+import os
+import sys
+import time
+from multiprocessing import Pool
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+C2_ITERS = 200
+C2_SNAP = (1, 20, 200)
+C2_STRIDE = 8
+C4_ITERS = 5
+CHAIN_ITERS = 10
+
+
+# ---------------------------------------------------------------------------
+# inputs (shared with the tests, which import these)
+# ---------------------------------------------------------------------------
+
+def random_phase_probe(P, rng):
+    """test.py:39: the true probe's far-field modulus with a random phase."""
+    return np.fft.fftshift(np.fft.ifftn(np.abs(np.fft.fftn(P)) *
+                                        np.exp(2.0j * np.pi * rng.random(P.shape))))
+
+
+def c2_inputs():
+    from ptychography_amd import sim
+    from tests.scans import broadband_start
+    s = sim.make_scan("c2")
+    P0 = random_phase_probe(s.P, np.random.default_rng(201))
+    O0 = broadband_start(s.O.shape, seed=202)
+    return s, P0, O0
+
+
+def c4_inputs(k):
+    from ptychography_amd import sim
+    from tests.scans import broadband_start
+    spec = sim.c4_specs()[k]
+    s = sim.make_scan(spec)
+    P0 = random_phase_probe(s.P, np.random.default_rng(300 + k))
+    O0 = broadband_start(s.O.shape, seed=400 + k)
+    return s, P0, O0
+
+
+def chain_inputs():
+    from ptychography_amd import sim
+    rng = np.random.default_rng(31)
+    I, R, M, P, O, B = sim.forward_sim(shape_P=(128, 128), shape_O=(256, 256), A=32,
+                                       defocus=1e-2, photons_pupil=1, ny=20, nx=20,
+                                       random_offset=5, background=None, mask=100,
+                                       counting_statistics=False, rng=rng)
+    I = I + 10000. * ~M                              # test.py:36
+    P0 = random_phase_probe(P, rng)                  # test.py:39
+    return I, R, M, P0
+
+
+# ---------------------------------------------------------------------------
+# jobs
+# ---------------------------------------------------------------------------
+
+def _metrics(info):
+    return np.stack([info["eMod"], info["eCon"], info["eSup"]], axis=1)
+
+
+def _c2_run(args):
+    import oracle
+    alg, dtype = args
+    s, P0, O0 = c2_inputs()
+    fn = oracle.ERA if alg == "era" else oracle.DM
+    snaps = {}
+
+    def cb(i, O, P):
+        if i + 1 in C2_SNAP:
+            snaps[i + 1] = (O.copy(), P.copy())
+    t = time.time()
+    with oracle.use_windows("slice"):
+        out = fn(s.I, s.R.copy(), P0.copy(), O0.copy(), C2_ITERS, mask=s.mask, method=3,
+                 dtype=dtype, callback=cb)
+    print(f"c2 {alg} {dtype}: {time.time() - t:.0f} s", flush=True)
+    return _metrics(out[-1]), snaps
+
+
+def _c4_run(args):
+    import oracle
+    k, dtype = args
+    s, P0, O0 = c4_inputs(k)
+    t = time.time()
+    with oracle.use_windows("slice"):
+        out = oracle.ERA(s.I, s.R.copy(), P0, O0, C4_ITERS, mask=s.mask, method=3, dtype=dtype)
+    print(f"c4 scan {k} {s.I.shape} {dtype}: {time.time() - t:.0f} s", flush=True)
+    return _metrics(out[-1])
+
+
+def _chain_run(dtype):
+    import oracle
+    I, R, M, P0 = chain_inputs()
+    with oracle.use_windows("slice"):
+        O1, P1, i1 = oracle.DM(I, R.copy(), P0, None, CHAIN_ITERS, mask=M, method=3, dtype=dtype,
+                               probe_centering=True)
+        O2, P2, i2 = oracle.ERA(I, R.copy(), P1, O1, CHAIN_ITERS, mask=M, method=3, dtype=dtype,
+                                probe_centering=True)
+    return _metrics(i1), _metrics(i2)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def make_c2(pool):
+    for alg in ("era", "dm"):
+        (m128, s128), (m64, s64) = pool.map(_c2_run, [(alg, "double"), (alg, "single")])
+        out = dict(metrics=m128, metrics_single=m64, snap_iters=np.array(C2_SNAP),
+                   stride=np.array(C2_STRIDE))
+        for it in C2_SNAP:
+            O, P = s128[it]
+            Os, Ps = s64[it]
+            out[f"P_{it}"] = P.astype(np.complex64)
+            out[f"Osub_{it}"] = O[::C2_STRIDE, ::C2_STRIDE].astype(np.complex64)
+            out[f"single_err_O_{it}"] = np.array(_rel(Os, O))
+            out[f"single_err_P_{it}"] = np.array(_rel(Ps, P))
+        np.savez_compressed(os.path.join(HERE, f"c2_{alg}3.npz"), **out)
+
+
+def make_c4(pool):
+    from ptychography_amd import sim
+    n = len(sim.c4_specs())
+    m128 = [_c4_run((k, "double")) for k in range(n)]
+    m64 = pool.map(_c4_run, [(k, "single") for k in range(n)], chunksize=1)
+    np.savez_compressed(os.path.join(HERE, "c4_era3.npz"), metrics=np.stack(m128),
+                        metrics_single=np.stack(m64))
+
+
+def make_chain(pool):
+    (d128, e128), (d64, e64) = pool.map(_chain_run, ["double", "single"])
+    np.savez_compressed(os.path.join(HERE, "chain_test_py.npz"), dm=d128, era=e128,
+                        dm_single=d64, era_single=e64)
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or ["chain", "c2", "c4"]
+    with Pool(4) as pool:
+        for name in names:
+            {"chain": make_chain, "c2": make_c2, "c4": make_c4}[name](pool)

@@ -250,3 +250,21 @@ def test_preamble_side_effects_and_defaults():
                          None, 1e-10, None)
     assert st["method"] == 4 and st["c_dtype"] == np.complex64
     assert np.allclose(st["background"], 1.0) and st["background"].shape == s.I.shape
+
+
+@pytest.mark.parametrize("alg,method", [("ERA", 3), ("DM", 3), ("DM", 6)])
+def test_slice_windows_bit_identical_to_roll(alg, method):
+    s = small_scan(frame=16, grid=(3, 3), step=4, background_frac=0.05 if method > 3 else 0.0)
+    fn = oracle.ERA if alg == "ERA" else oracle.DM
+    bg = None if method < 4 else np.full(s.I.shape[1:], 0.3)
+    outs = []
+    for mode in ("roll", "slice"):
+        with oracle.use_windows(mode):
+            outs.append(fn(s.I, s.R.copy(), s.P * 1.1, None, 4, method=method, background=bg,
+                           probe_centering=True))
+    for a, b in zip(outs[0][:-1], outs[1][:-1]):
+        assert np.array_equal(a, b)
+    for key in ("eMod", "eCon", "eSup"):
+        assert outs[0][-1][key] == outs[1][-1][key]
+    assert np.array_equal(outs[0][-1]["I"], outs[1][-1]["I"])
+    assert oracle.era.WINDOWS == "roll"
