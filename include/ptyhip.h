@@ -229,6 +229,25 @@ int pty_make_exits(pty_ctx* ctx, const pty_cfloat* O_dev, const pty_cfloat* P_de
  * with the bound data; *emod_host = the un-normalised eMod sum. */
 int pty_pmod1(pty_ctx* ctx, pty_cfloat* exits_dev, float alpha, double* emod_host);
 
+/* Allocation-free variant of pty_pmod1, for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It has no counterpart in the reference.  No
+ * context: exits (N x M x K, in place), the ifftshifted amplitudes amp
+ * (N x M x K float32) and mask (M x K, 1 = good; NULL = all good) are passed
+ * directly, and the un-normalised eMod sum is written to emod_dev (one
+ * double, device memory).  Work is queued on `stream` (hipStream_t; NULL =
+ * default) and not waited for.  Its scratch (eMod partials, packed mask
+ * words, twiddle tables: 284,800 bytes) is one static array per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU race.  The first call on a device sets the array up with
+ * a synchronous copy, so make it outside graph capture.  Limits: M and K in
+ * {16, 32, 64, 96, 128, 192, 256}, N x M x K <= 2^34, at most 256 compute
+ * units; anything else is rejected before any work is queued.  It calls no
+ * hipMalloc / hipFree. */
+int pty_pmod1_noalloc(pty_cfloat* exits_dev, const float* amp_dev, const uint8_t* mask_dev,
+                      int32_t N, int32_t M, int32_t K, float alpha, double* emod_dev,
+                      void* stream);
+
 /* O_out = psup_O(exits, P, R, (U, V), H, alpha) (era.py:336-369).  fresh = 1
  * recomputes the heatmap into H_dev (make_P_heatmap, era.py:402-410); fresh = 0
  * reads it. */

@@ -68,6 +68,7 @@ SIGNATURES = {
     "pty_fft2": (ctypes.c_int, [_vp, _vp, _i32, _i32]),
     "pty_make_exits": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "pty_pmod1": (ctypes.c_int, [_vp, _vp, _f32, ctypes.POINTER(_f64)]),
+    "pty_pmod1_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _f32, _vp, _vp]),
     "pty_psup_O": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_psup_P": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _f32]),
     "pty_back_projection": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _f32]),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1183,6 +1184,77 @@ void twiddles(int L, std::vector<float2>& out) {
 
 }  // namespace
 
+// --------------------------------------------------------------------------
+// Scratch of the allocation-free modulus step (pty_pmod1_noalloc): the column
+// kernels' eMod partials, the packed mask words and the twiddle tables of every
+// supported line length, in one file-scope device array.  Each GPU holds its
+// own copy (at its own address) until the process ends; every call on a GPU
+// shares it, so calls that overlap on one GPU race.
+// --------------------------------------------------------------------------
+namespace {
+constexpr size_t NA_PARTS = 32768;   // eMod partials: at most 128 work-groups per CU x 256 CUs
+constexpr size_t NA_MASKW = 16 * 256;   // packed mask words: T x K, at most 16 x 256
+constexpr int NA_SIZES[] = {16, 32, 64, 96, 128, 192, 256};
+constexpr int NA_TW = 16 + 32 + 64 + 96 + 128 + 192 + 256;   // twiddle entries, all sides
+constexpr size_t NA_OFF_MASK = NA_PARTS * sizeof(double);
+constexpr size_t NA_OFF_TW = NA_OFF_MASK + NA_MASKW * sizeof(uint32_t);
+constexpr size_t NA_BYTES = NA_OFF_TW + NA_TW * sizeof(float2);   // 284,800 bytes
+static_assert(NA_BYTES == 284800, "scratch layout");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+namespace {
+constexpr int NA_MAX_DEV = 64;
+struct NoallocDevice {
+  unsigned char* base = nullptr;   // this device's address of `scratch`
+  int cus = 0;
+};
+NoallocDevice g_noalloc[NA_MAX_DEV];
+std::mutex g_noalloc_mu;
+
+int noalloc_tw_offset(int L) {
+  int off = 0;
+  for (int s : NA_SIZES) {
+    if (s == L) return off;
+    off += s;
+  }
+  return -1;
+}
+
+// Once per device, with it current: its address of `scratch` and the twiddle
+// tables (a synchronous copy from the host; nothing is allocated).
+int noalloc_setup(int dev, NoallocDevice** out) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  NoallocDevice& d = g_noalloc[dev];
+  if (!d.base) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PTY_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    float2 tw[NA_TW];
+    int k0 = 0;
+    for (int L : NA_SIZES) {
+      for (int k = 0; k < L; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)L;
+        tw[k0 + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      }
+      k0 += L;
+    }
+    e = hipMemcpy(static_cast<unsigned char*>(ptr) + NA_OFF_TW, tw, sizeof(tw), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(PTY_EHIP, std::string("twiddle upload: ") + hipGetErrorString(e));
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess || cus <= 0)
+      return fail(PTY_EHIP, std::string("compute-unit count: ") + hipGetErrorString(e));
+    d.cus = cus;
+    d.base = static_cast<unsigned char*>(ptr);
+  }
+  *out = &d;
+  return PTY_OK;
+}
+}  // namespace
+
 // Several scans advanced together: each scan's kernels go to one of up to
 // eight lane streams (default priority), so small scans fill the device
 // side by side; with graphs on, one iteration of every scan is captured as one
@@ -2043,6 +2115,68 @@ int pty_pmod1(pty_ctx* c, pty_cfloat* exits, float alpha, double* emod) {
   double s = 0.0;
   PTY_TRY(sum_region(c, c->part_emod, ncol, &s));
   if (emod) *emod = s;
+  return PTY_OK;
+}
+
+int pty_pmod1_noalloc(pty_cfloat* exits, const float* amp, const uint8_t* mask, int32_t N, int32_t M,
+                      int32_t K, float alpha, double* emod, void* stream) {
+  if (!exits || !amp || !emod) return fail(PTY_EINVAL, "NULL argument");
+  if (N <= 0) return fail(PTY_EINVAL, "N must be positive");
+  if (!pty_supported_frame(M) || !pty_supported_frame(K))
+    return fail(PTY_EUNSUP, "frame sides must be one of 16, 32, 64, 96, 128, 192, 256 (got " +
+                                std::to_string(M) + " x " + std::to_string(K) + ")");
+  if ((size_t)N * M * K > ((size_t)1 << 34))
+    return fail(PTY_EINVAL, "N x M x K above the maximum of 2^34 frame pixels");
+  int dev = 0;
+  PTY_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEV) return fail(PTY_EINVAL, "device index beyond 63");
+  NoallocDevice* d = nullptr;
+  PTY_TRY(noalloc_setup(dev, &d));
+  // the column step takes up to 128 work-groups per compute unit, each with one partial
+  if ((size_t)128 * d->cus > NA_PARTS)
+    return fail(PTY_EINVAL, "device has more compute units than the scratch's eMod partials cover");
+
+  // a launch description only: nothing of it is allocated
+  pty_ctx c;
+  c.N = N; c.M = M; c.K = K; c.U = M; c.V = K;
+  c.stream = static_cast<hipStream_t>(stream);
+  c.g.N = N; c.g.M = M; c.g.K = K; c.g.U = M; c.g.V = K;
+  c.g.off = nullptr;   // the plain-frame row kernels read no window offsets
+  c.cus = d->cus;
+  c.col_chunk = 1;
+  c.cap_col = (int)NA_PARTS;
+  c.twM = reinterpret_cast<float2*>(d->base + NA_OFF_TW) + noalloc_tw_offset(M);
+  c.twK = reinterpret_cast<float2*>(d->base + NA_OFF_TW) + noalloc_tw_offset(K);
+  double* parts = reinterpret_cast<double*>(d->base);
+  c.mask = mask;
+  c.maskbits = mask ? reinterpret_cast<uint32_t*>(d->base + NA_OFF_MASK) : nullptr;
+  if (mask) {
+    const int T = threads_per_line(M);
+    const int grid = (T * K + WG - 1) / WG;
+    switch (M) {
+#define C(Lv)                                                                                  \
+  case Lv:                                                                                     \
+    hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c.stream, mask, K, c.maskbits); \
+    break;
+      PTY_SIZES(C)
+#undef C
+      default: return fail(PTY_EUNSUP, "unsupported M");
+    }
+    PTY_LAUNCHED("k_pack_mask");
+  }
+  float2* x = reinterpret_cast<float2*>(exits);
+  PTY_TRY(rows_fwd<SRC_F>(&c, nullptr, nullptr, x, x, N * M));
+  ColArgs a = col_args(&c);
+  a.X = x;
+  a.amp = amp;
+  a.amp_stride = (long)c.MK();
+  a.alpha = alpha;
+  a.part = parts;
+  int ncol = 0;
+  PTY_TRY(cols<COL_ERA1>(&c, a, &ncol));
+  PTY_TRY(rows_inv<DST_STORE>(&c, x, x, nullptr, nullptr, 1.0f, nullptr, N * M));
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, c.stream, (const double*)parts, ncol, emod);
+  PTY_LAUNCHED("k_sum");
   return PTY_OK;
 }
 
