@@ -8,7 +8,10 @@ per frame, central beamstop masked.  Synthetic data, generated on the GPU by
 the library's own frame-formation / FFT kernels plus torch.poisson.
 
 A step is one ERA iteration (method 1: object update with the known probe)
-over all 4096 frames, with every input resident in HBM.  The same run also
+over all 4096 frames, with every input resident in HBM.  ``value`` is the
+median of --repeats timed passes of --steps iterations (the per-iteration
+FP64 metric readback included); the per-stage kernel durations come from one
+more pass of the same iterations with HIP events around every stage.  The same run also
 times the difference map with background refinement (method 4) at the same
 shape, reported under "extra", and the second half of the metric: wall time
 to eMod <= 1e-3 of ERA method 3 on the held-out 32 x 32-position scan of
@@ -45,12 +48,16 @@ HBM_PEAK_GBS = 8000.0
 # iteration: amp read 4 + psi' write 8 + psi' read 8; object read/write 16 + FP64 heatmap 8
 ERA1_BYTES_PX = 20
 OBJ_BYTES_PX = 24
+# DM methods 1-3: f read 8 + amp 4 + f write 8 + the overlap projection's read of f 8
+DM1_BYTES_PX = 28
 # DM method 4 (f, b, amp through two modulus steps + two overlap reads + metric amp read)
 DM4_BYTES_PX = 76
-# HIO (update 'sample') on the path's kernels, per frame pixel: row forward f -> m (read 8,
-# write 8), columns 20, row inverse with both linear combinations (read m and f 16, write
-# both 16), the object gather 8 (the pending Psup term comes from the object window)
-HIO_BYTES_PX = 76
+# HIO (update 'sample'), compulsory per frame pixel: iterate read 8 + amplitude 4, the
+# Psup argument written 8 and read by the object update 8, the next iterate written 8
+HIO_BYTES_PX = 36
+# what the HIO kernels move by design (row forward f -> m 16, columns 20, row inverse with
+# both linear combinations 32, object gather 8): a diagnostic, not the denominator
+HIO_KERNEL_BYTES_PX = 76
 
 # algorithmic bytes per frame pixel of each stage's launch (ERA method 1)
 STAGE_BYTES_PX_STAGED = {
@@ -88,6 +95,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed passes of --steps iterations; the median is reported")
     ap.add_argument("--dm-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=48,
@@ -247,38 +256,32 @@ def learned_init(dev, target=1e-3, max_iters=1500, reps=5):
 
 
 def make_scan_gpu(grid, dev):
-    """C3 recipe at full size, intensities formed on the GPU."""
-    import torch
+    """The C3 recipe at full size (``sim.make_scan_gpu``, the generator the
+    full-size tests use), intensities formed on the GPU."""
     from ptychography_amd import sim
-    from ptychography_amd.engine import Engine
-
     spec = sim.RECIPES["c3"].with_(grid=(grid, grid))
-    rng = np.random.default_rng(spec.seed)
-    M = spec.frame
-    off = sim.make_positions(spec.grid, spec.step, spec.jitter, rng)
-    U = int(off[:, 0].max()) + M
-    V = int(off[:, 1].max()) + M
-    O = sim.make_object((U, V), rng, spec.amp_range, spec.sigma)
-    P = sim.make_probe(M, spec.pupil_radius, spec.defocus, spec.zernike_rms, rng)
-    R = (-off).astype(np.int32)
-    N = len(R)
-    gen = Engine(N, M, M, U, V, R, device=dev)
-    O_t = torch.from_numpy(O.astype(np.complex64)).to(dev)
-    P_t = torch.from_numpy(P.astype(np.complex64)).to(dev)
-    gen.set_state(O_t, P_t)
-    I = gen.intensity()
-    gen.close()
-    del O_t
-    scale = spec.photons / float(I.sum(dtype=torch.float64).item() / N)
-    I *= scale
-    P = P * np.sqrt(scale)
-    B = sim._rescale(sim._gauss_smooth(rng.random((M, M)), 2.0 * spec.sigma), 0.5, 1.5)
-    B *= spec.background_frac * float(I.mean().item()) / B.mean()
-    I += torch.from_numpy(B.astype(np.float32)).to(dev)[None]
-    torch.manual_seed(spec.seed)
-    I = torch.poisson(I)
-    mask = sim.make_mask(M, spec.beamstop_radius, 0.0, rng)
-    return dict(I=I, R=R, P=P, O=O, B=B, mask=mask, N=N, M=M, U=U, V=V, spec=spec)
+    sc = sim.make_scan_gpu(spec, dev)
+    N, M = sc["I"].shape[:2]
+    U, V = sc["O"].shape
+    mask = sc["mask"]
+    if np.ndim(mask) == 0:
+        mask = np.ones((M, M), bool)
+    sc.update(R=sc["R"].astype(np.int32), mask=mask, N=N, M=M, U=U, V=V)
+    return sc
+
+
+def cpu_model():
+    """(model name, logical CPUs) of this host."""
+    name = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count()
 
 
 def widening(scan, dev, iters=6):
@@ -312,6 +315,8 @@ def widening(scan, dev, iters=6):
     return {"hio_workload": f"hio-sample-beta0.9-{N}x{M}x{M}-object{U}x{V}",
             "hio_ms_per_iter": 1e3 * per, "hio_iters_per_s": 1.0 / per,
             "hio_roofline_frac": (HIO_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per / (HBM_PEAK_GBS * 1e9),
+            "hio_kernel_bytes_frac": (HIO_KERNEL_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per
+                                     / (HBM_PEAK_GBS * 1e9),
             "back_projection_call_s": bp[-1]}
 
 
@@ -346,7 +351,8 @@ def cpu_baseline(scan, nframes):
             "sample": (f"oracle.ERA method 1, complex128, {nframes} of {scan['N']} frames "
                        f"(256^2) with the full {scan['U']}x{scan['V']} object, 2 iterations "
                        f"({total:.1f} s wall); second iteration {per_iter:.3f} s scaled by "
-                       f"{scan['N']}/{nframes}; host {platform.processor() or platform.machine()}")}
+                       f"{scan['N']}/{nframes}; host {cpu_model()[0]}, {cpu_model()[1]} logical "
+                       f"CPUs, one used (NumPy's FFT is single-threaded)")}
 
 
 def main():
@@ -379,12 +385,23 @@ def main():
         eng.set_path(args.path != "staged", args.frame_threads, args.frame_slots)
     O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
     eng.run(0, 1, args.warmup, iter0=0, begin=True)
+    it0, passes = args.warmup, []
+    for _ in range(max(1, args.repeats)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        metrics = eng.run(0, 1, args.steps, iter0=it0)
+        torch.cuda.synchronize()
+        passes.append(time.perf_counter() - t0)
+        it0 += args.steps
+    dt = float(np.median(passes))
+    # kernel durations: one more pass of the same iterations with a HIP event
+    # pair around every stage on the engine's stream
     eng.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    metrics = eng.run(0, 1, args.steps, iter0=args.warmup)
+    eng.run(0, 1, args.steps, iter0=it0)
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt_prof = time.perf_counter() - t0
     prof = eng.profile_read()
     eng.profile(False)
     eng.close()
@@ -402,28 +419,46 @@ def main():
     dom_kernel = ("k_frame (fused rows+columns+modulus)" if fused and dom == "cols" else
                   STAGE_KERNEL.get(dom, "k_rows_gather" if rowg else "k_gather_O"))
     iter_bytes = ERA1_BYTES_PX * px + OBJ_BYTES_PX * U * V
+    # 5 n log2 n per 2-D transform of n = M^2 points, one forward and one inverse per frame
+    fft_flops = 2 * N * 5.0 * M * M * np.log2(M * M)
 
-    # ---- DM with background refinement (method 4), same shape ----
-    dm = None
-    if args.dm_steps > 0:
+    # ---- difference map at the same shape: with background refinement
+    # (method 4, the workload's own algorithm), object-only (method 1) and
+    # object + probe with OP_iters (1, 3) (method 3) ----
+    def dm_timing(method, OP_iters=(1, 1)):
         eng = Engine(N, M, M, U, V, scan["R"], device=dev)
         if args.path != "default" or args.frame_threads or args.frame_slots:
             eng.set_path(args.path != "staged", args.frame_threads, args.frame_slots)
         O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
-        b0 = np.sqrt(np.fft.ifftshift(scan["B"])).astype(np.float32)
-        eng.set_background(torch.from_numpy(b0), per_frame=False)
-        eng.run(1, 4, 2, iter0=0, begin=True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        dm_metrics = eng.run(1, 4, args.dm_steps, iter0=2)
-        torch.cuda.synchronize()
-        dm_dt = time.perf_counter() - t1
+        if method >= 4:
+            b0 = np.sqrt(np.fft.ifftshift(scan["B"])).astype(np.float32)
+            eng.set_background(torch.from_numpy(b0), per_frame=False)
+        eng.run(1, method, 2, iter0=0, begin=True, OP_iters=OP_iters)
+        it, times = 2, []
+        for _ in range(max(1, args.repeats)):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            m = eng.run(1, method, args.dm_steps, iter0=it, OP_iters=OP_iters)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t1)
+            it += args.dm_steps
         eng.close()
         del amp, O_t, P_t
-        dm_its = args.dm_steps / dm_dt
-        dm = {"dm_bg_iters_per_s": dm_its, "dm_bg_ms_per_step": 1e3 * dm_dt / args.dm_steps,
-              "dm_bg_roofline_frac": (DM4_BYTES_PX * px + OBJ_BYTES_PX * U * V) * dm_its / (HBM_PEAK_GBS * 1e9),
-              "dm_bg_final_eMod": float(dm_metrics[-1, 0])}
+        per = float(np.median(times)) / args.dm_steps
+        return per, float(m[-1, 0])
+
+    dm = None
+    if args.dm_steps > 0:
+        per4, e4 = dm_timing(4)
+        per1, e1 = dm_timing(1)
+        per3, e3 = dm_timing(3, (1, 3))
+        dm = {"dm_bg_iters_per_s": 1.0 / per4, "dm_bg_ms_per_step": 1e3 * per4,
+              "dm_bg_roofline_frac": (DM4_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per4 / (HBM_PEAK_GBS * 1e9),
+              "dm_bg_final_eMod": e4,
+              "dm1_iters_per_s": 1.0 / per1, "dm1_ms_per_step": 1e3 * per1,
+              "dm1_roofline_frac": (DM1_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per1 / (HBM_PEAK_GBS * 1e9),
+              "dm3_op13_iters_per_s": 1.0 / per3, "dm3_op13_ms_per_step": 1e3 * per3,
+              "repeats": max(1, args.repeats), "timing": "median of the repeats"}
 
     learned = None
     if not args.no_learned and os.path.exists(WEIGHTS):
@@ -462,6 +497,11 @@ def main():
                      "kernel_ms": stage_avg[dom],
                      "algorithmic_bytes_per_launch": dom_bytes},
         "iteration_roofline_frac": iter_bytes * its / (HBM_PEAK_GBS * 1e9),
+        "repeats": max(1, args.repeats),
+        "ms_per_step_passes": [1e3 * t / args.steps for t in passes],
+        "ms_per_step_profiled_pass": 1e3 * dt_prof / args.steps,
+        "fft_gflops": fft_flops * its / 1e9,
+        "fft_flop_per_iter": fft_flops,
         "stage_ms_per_launch": stage_avg,
         "object_update": "k_rows_gather (inverse row FFT fused)" if rowg else "k_gather_O",
         "final_eMod": float(metrics[-1, 0]),
