@@ -20,7 +20,7 @@ import torch
 
 from .engine import Engine, require_gpu
 
-__all__ = ["ERA", "solve"]
+__all__ = ["ERA", "solve", "update_progress"]
 
 ALGO_ERA = 0
 ALGO_DM = 1
@@ -191,6 +191,48 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
                init_info=init_info, iters=iters, graphs=bool(graphs))
 
 
+def _round_half_away(x):
+    return int(math.floor(x + 0.5)) if x >= 0 else -int(math.floor(-x + 0.5))
+
+
+def update_progress(progress, algorithm, i, emod, esup, stream=None):
+    """The reference's one-line progress bar (ptychography/era.py:612-629),
+    written to ``stream`` (default ``sys.stdout``) and flushed.
+
+    ``"\r{algorithm}: [{15-character bar}] {percent}% {i} {emod} {esup} {status}"``
+    plus five spaces; status is "Done...\r\n" at progress >= 1, "Halt...\r\n"
+    below 0, and an error note for a non-number.  The reference runs under
+    Python 2, so the bar length rounds half away from zero and floats print
+    with 12 significant digits, as Python 2's ``str`` does.  ERA / DM pass
+    (i / max(1, iters - 1), name, i, eCon, eMod) (era.py:212, dm.py:152).
+    """
+    import sys
+    bar = 15
+    status = ""
+    if isinstance(progress, int):
+        progress = float(progress)
+    if not isinstance(progress, float):
+        progress = 0
+        status = "error: progress var must be float\r\n"
+    if progress < 0:
+        progress = 0
+        status = "Halt...\r\n"
+    if progress >= 1:
+        progress = 1
+        status = "Done...\r\n"
+    block = _round_half_away(bar * progress)
+
+    def fmt(v):
+        return "%.12g" % v if isinstance(v, (float, np.floating)) else str(v)
+    text = "\r{0}: [{1}] {2}% {3} {4} {5} {6} {7}".format(
+        algorithm, "#" * block + "-" * (bar - block), int(progress * 100), i, fmt(emod),
+        fmt(esup), status, " " * 5)
+    out = sys.stdout if stream is None else stream
+    out.write(text)
+    out.flush()
+    return text
+
+
 def check_metrics(metrics):
     """Raise if an iteration's eMod is not finite (NaN / inf in the data or the
     state): the reference would carry the NaNs on silently (era.py:561-571)."""
@@ -249,7 +291,7 @@ def finish(job, metrics, full_output=True, check_finite=True):
 def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
           Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
           full_output=True, device=None, callback=None, rng=None, init=None, init_net=None,
-          graphs=False, check_finite=True):
+          graphs=False, check_finite=True, progress=False):
     """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns.
 
     ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
@@ -263,14 +305,21 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
                   rng=rng, init=init, init_net=init_net, graphs=graphs)
     try:
         eng, kw = job.eng, job.run_kwargs()
-        if callback is None:
+        name = 'ERA' if algorithm == ALGO_ERA else 'DM'
+        if progress:
+            print('algrithm progress iteration convergence modulus error')   # era.py:151
+        if callback is None and not progress:
             metrics = eng.run(algorithm, job.method, iters, iter0=0, begin=True, **kw)
         else:
             rows = []
             for i in range(iters):
                 rows.append(eng.run(algorithm, job.method, 1, iter0=i, begin=(i == 0), **kw))
-                callback(i, job.O_t.cpu().numpy().astype(job.c_dtype),
-                         job.P_t.cpu().numpy().astype(job.c_dtype))
+                if progress:
+                    update_progress(i / max(1.0, float(iters - 1)), name, i, float(rows[-1][0, 1]),
+                                    float(rows[-1][0, 0]))
+                if callback is not None:
+                    callback(i, job.O_t.cpu().numpy().astype(job.c_dtype),
+                             job.P_t.cpu().numpy().astype(job.c_dtype))
             metrics = np.concatenate(rows, axis=0) if rows else np.zeros((0, 3))
             if iters == 0:
                 eng.run(algorithm, job.method, 0, iter0=0, begin=True, **kw)
@@ -283,7 +332,7 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmod_probe=False,
         probe_centering=False, hardware='cpu', alpha=1.0e-10, dtype=None, full_output=True,
         device=None, callback=None, rng=None, init=None, init_net=None, graphs=False,
-        check_finite=True):
+        check_finite=True, progress=False):
     """Error-reduction algorithm (ptychography/era.py:13).  ``hardware`` is accepted and
     ignored, as in the reference; the work always runs on the GPU ``device``.
 
@@ -295,10 +344,13 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmo
     (bit-identical to launching the kernels one by one; a failed capture falls
     back to eager launches, reported in ``info['graphs']``),
     ``check_finite`` (default True) to raise ``FloatingPointError`` when eMod
-    becomes NaN / inf instead of returning NaNs.
+    becomes NaN / inf instead of returning NaNs, ``progress=True`` to print
+    the reference's header and per-iteration progress bar (``update_progress``;
+    the reference always prints them, era.py:151,212).
     """
     return solve(ALGO_ERA, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
                  background=background, method=method, Pmod_probe=Pmod_probe,
                  probe_centering=probe_centering, alpha=alpha, dtype=dtype,
                  full_output=full_output, device=device, callback=callback, rng=rng,
-                 init=init, init_net=init_net, graphs=graphs, check_finite=check_finite)
+                 init=init, init_net=init_net, graphs=graphs, check_finite=check_finite,
+                 progress=progress)

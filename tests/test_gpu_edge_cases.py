@@ -88,3 +88,17 @@ def test_mask_of_ones_equals_no_mask(solver, gpu_device):
     b = fn(s.I, s.R.copy(), s.P, _O0(s), 5, method=3, mask=np.ones((64, 64), bool))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert np.array_equal(np.asarray(a[2]["eMod"]), np.asarray(b[2]["eMod"]))
+
+
+def test_progress_bar_prints_and_leaves_results_alone(gpu_device, capsys):
+    import ptychography_amd as pty
+    from tests.scans import small_scan
+    s = small_scan(frame=32, grid=(3, 3), step=8)
+    O1, i1 = pty.ERA(s.I, s.R.copy(), s.P, None, 4, method=1, device=gpu_device)
+    capsys.readouterr()
+    O2, i2 = pty.ERA(s.I, s.R.copy(), s.P, None, 4, method=1, device=gpu_device, progress=True)
+    out = capsys.readouterr().out
+    assert out.startswith("algrithm progress iteration convergence modulus error\n")
+    assert out.count("\rERA: [") == 4 and "Done...\r\n" in out
+    assert np.array_equal(O1, O2) and i1["eMod"] == i2["eMod"]
+    assert f" 3 {i2['eCon'][-1]:.12g} {i2['eMod'][-1]:.12g} " in out

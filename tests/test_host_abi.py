@@ -104,3 +104,24 @@ def test_facade_fails_loudly_without_gpu():
     R = np.array([[0, 0], [-2, -2]])
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         pty.ERA(I, R, np.ones((16, 16)), None, 1, method=1)
+
+
+def test_update_progress_reproduces_the_reference_line():
+    import io
+    from ptychography_amd import update_progress
+
+    def line(*a):
+        buf = io.StringIO()
+        return update_progress(*a, stream=buf), buf.getvalue()
+
+    t, written = line(0.5, 'ERA', 3, 0.25, 0.125)
+    assert t == written == "\rERA: [########-------] 50% 3 0.25 0.125  " + " " * 5
+    # 15 / 6 = 2.5 rounds half away from zero (Python 2), 1/6 -> 16 %
+    assert line(1.0 / 6, 'DM', 1, 0.5, 0.5)[0].startswith("\rDM: [###------------] 16% 1 ")
+    assert line(1.0, 'ERA', 9, 0.1, 0.2)[0] == ("\rERA: [###############] 100% 9 0.1 0.2 "
+                                                 "Done...\r\n" + " " * 6)
+    assert "Halt...\r\n" in line(-0.5, 'ERA', 0, 1.0, 1.0)[0]
+    assert "error: progress var must be float" in line("x", 'ERA', 0, 1.0, 1.0)[0]
+    # Python 2's str(float): 12 significant digits
+    assert " 0.123456789012 " in line(0.0, 'ERA', 0, 0.1234567890123456, 2.0)[0]
+    assert line(3, 'ERA', 0, 1.0, 1.0)[0].startswith("\rERA: [###############] 100%")
