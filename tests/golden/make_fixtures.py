@@ -16,7 +16,9 @@ are computed here once and committed as small .npz files:
 * ``chain_test_py.npz`` -- the reference's own caller (test.py:31-49): a
   forward_sim scan with 100 dead pixels raised by 1e4 and a random-phase
   probe, DM method 3 with probe centring then ERA method 3 with centring,
-  10 iterations each: metrics of both precisions.
+  30 iterations each (test.py's `iters` argument): metrics of both precisions;
+  and the same chain from a broadband object instead of the flat one
+  (``*_wc``: well-conditioned, 10 iterations each).
 
 Inputs are regenerated by the tests from the same seeds (``sim`` and the
 start guesses below), so only results are stored.  The oracle reads windows
@@ -40,7 +42,8 @@ C2_ITERS = 200
 C2_SNAP = (1, 20, 200)
 C2_STRIDE = 8
 C4_ITERS = 5
-CHAIN_ITERS = 10
+CHAIN_ITERS = 30
+CHAIN_WC_ITERS = 10
 
 
 # ---------------------------------------------------------------------------
@@ -121,13 +124,21 @@ def _c4_run(args):
     return _metrics(out[-1])
 
 
-def _chain_run(dtype):
+def chain_broadband_object():
+    from tests.scans import broadband_start
+    return broadband_start((256, 256), seed=32)
+
+
+def _chain_run(args):
     import oracle
+    dtype, wc = args
     I, R, M, P0 = chain_inputs()
+    O0 = chain_broadband_object() if wc else None
+    iters = CHAIN_WC_ITERS if wc else CHAIN_ITERS
     with oracle.use_windows("slice"):
-        O1, P1, i1 = oracle.DM(I, R.copy(), P0, None, CHAIN_ITERS, mask=M, method=3, dtype=dtype,
+        O1, P1, i1 = oracle.DM(I, R.copy(), P0, O0, iters, mask=M, method=3, dtype=dtype,
                                probe_centering=True)
-        O2, P2, i2 = oracle.ERA(I, R.copy(), P1, O1, CHAIN_ITERS, mask=M, method=3, dtype=dtype,
+        O2, P2, i2 = oracle.ERA(I, R.copy(), P1, O1, iters, mask=M, method=3, dtype=dtype,
                                 probe_centering=True)
     return _metrics(i1), _metrics(i2)
 
@@ -161,9 +172,12 @@ def make_c4(pool):
 
 
 def make_chain(pool):
-    (d128, e128), (d64, e64) = pool.map(_chain_run, ["double", "single"])
+    runs = pool.map(_chain_run, [("double", False), ("single", False), ("double", True),
+                                 ("single", True)])
+    (d128, e128), (d64, e64), (w128, x128), (w64, x64) = runs
     np.savez_compressed(os.path.join(HERE, "chain_test_py.npz"), dm=d128, era=e128,
-                        dm_single=d64, era_single=e64)
+                        dm_single=d64, era_single=e64, dm_wc=w128, era_wc=x128,
+                        dm_wc_single=w64, era_wc_single=x64)
 
 
 if __name__ == "__main__":
