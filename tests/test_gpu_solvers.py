@@ -1,6 +1,6 @@
 """GPU ERA / DM against the NumPy oracle on identical seeded inputs.
 
-Bar (stated tolerances), per iteration over the first 10-20 iterations:
+Bar (stated tolerances), per iteration over the first 12-20 iterations:
 
 * iterate error  |x_gpu - x_ref|_2 / |x_ref|_2  (object on the illuminated
   region, sum |P|^2 >= 1e-6 max; probe everywhere) must not exceed
@@ -11,7 +11,9 @@ Bar (stated tolerances), per iteration over the first 10-20 iterations:
   probe-updating runs amplify round-off (their complex64 NumPy runs drift to
   1e-4..1e-3 within 10-20 iterations), so they are held to the drift bound;
 * eMod / eCon / eSup per iteration: relative error <= max(1e-4, 10 * the
-  running max of the complex64 oracle's relative error);
+  running max of the complex64 oracle's relative error); for DM, whose
+  iterates drift furthest, also <= 10 * the complex64 oracle's iterate drift
+  (the metrics are functions of the iterates);
 * final eMod within 1 %.
 """
 
@@ -76,12 +78,17 @@ def _check_traj(traj_g, traj_r, region, upd, single=None, tol=ITER_TOL):
     print("bound             :", " ".join(f"{x:.1e}" for x in bound))
     bad = np.nonzero(eg > bound)[0]
     assert bad.size == 0, f"iteration {bad[0]}: rel L2 {eg[bad[0]]:.2e} > bound {bound[bad[0]]:.2e}"
-    return eg.max()
+    return None if single is None else es
 
 
-def _check_metrics(ig, ir, single_info=None, keys=("eMod", "eCon", "eSup"), tol=METRIC_TOL):
+def _check_metrics(ig, ir, single_info=None, keys=("eMod", "eCon", "eSup"), tol=METRIC_TOL,
+                   iter_drift=None):
     # eCon is a difference of successive iterates: its relative error is the
-    # iterate error divided by eCon itself, so it gets a 10x looser floor
+    # iterate error divided by eCon itself (eCon ~ 1e-2 turns a 1e-4 iterate
+    # drift into 1e-2), so it gets a 10x looser floor and 3x the drift factor.
+    # The kernels' own eCon error is pinned separately, step by step, by
+    # test_gpu_step_parity.py: within 1e-5 relative or 6e-6 absolute per step
+    # for every method, from the oracle's own state.
     for key in keys:
         tol = METRIC_TOL * (10 if key == "eCon" else 1)
         factor = DRIFT_FACTOR * (3 if key == "eCon" else 1)
@@ -94,6 +101,11 @@ def _check_metrics(ig, ir, single_info=None, keys=("eMod", "eCon", "eSup"), tol=
             s = np.array(single_info[key])
             ds = np.maximum.accumulate(np.abs(s - r) / np.maximum(np.abs(r), 1e-12))
             bound = np.maximum(bound, factor * ds)
+        if iter_drift is not None:
+            # the metrics are functions of the iterates: where the complex64
+            # oracle's iterates have drifted by d, the metrics may differ by
+            # up to DRIFT_FACTOR * d as well
+            bound = np.maximum(bound, DRIFT_FACTOR * np.asarray(iter_drift)[:len(bound)])
         bad = np.nonzero(err > bound)[0]
         assert bad.size == 0, f"{key}: iteration {bad[0]} rel err {err[bad[0]]:.2e} > {bound[bad[0]]:.2e}"
     assert abs(ig["eMod"][-1] - ir["eMod"][-1]) <= 0.01 * ir["eMod"][-1]
@@ -184,10 +196,10 @@ def test_dm_probe_methods(method, gpu_device):
     P0 = flat_start(s)
     O0 = s.O if method == 2 else _O0(s)
     out_g, out_r, tg, tr, single = _run_both(pty.DM, oracle.DM, (s.I, s.R, P0, O0),
-                                     dict(method=method), 12)
+                                     dict(method=method), 20)
     region = _illuminated(s.P, s.R, s.O.shape)
-    _check_traj(tg, tr, region, "P" if method == 2 else "OP", single)
-    _check_metrics(out_g[-1], out_r[-1], single[0][-1])
+    drift = _check_traj(tg, tr, region, "P" if method == 2 else "OP", single)
+    _check_metrics(out_g[-1], out_r[-1], single[0][-1], iter_drift=drift)
 
 
 def test_dm_centering_schedule(gpu_device):
@@ -195,10 +207,10 @@ def test_dm_centering_schedule(gpu_device):
     s = small_scan(frame=32, grid=(5, 5), step=6, jitter=1)
     P0 = np.roll(flat_start(s), (2, 3), axis=(0, 1))
     out_g, out_r, tg, tr, single = _run_both(pty.DM, oracle.DM, (s.I, s.R, P0, _O0(s)),
-                                     dict(method=3, OP_iters=(1, 2), probe_centering=True), 10)
+                                     dict(method=3, OP_iters=(1, 2), probe_centering=True), 20)
     region = _illuminated(s.P, s.R, s.O.shape)
-    _check_traj(tg, tr, region, "OP", single)
-    _check_metrics(out_g[-1], out_r[-1], single[0][-1])
+    drift = _check_traj(tg, tr, region, "OP", single)
+    _check_metrics(out_g[-1], out_r[-1], single[0][-1], iter_drift=drift)
 
 
 @pytest.mark.parametrize("method,frame,op", [(4, 32, 1), (6, 32, 1), (4, 64, 1), (6, 64, (1, 2))])
@@ -211,12 +223,41 @@ def test_dm_background_methods(method, frame, op, gpu_device):
     P0 = s.P if method == 4 else flat_start(s)
     B0 = np.random.default_rng(4).random((frame, frame)) * 0.5
     out_g, out_r, tg, tr, single = _run_both(pty.DM, oracle.DM, (s.I, s.R, P0, _O0(s)),
-                                     dict(method=method, background=B0, mask=s.mask, OP_iters=op), 10)
+                                     dict(method=method, background=B0, mask=s.mask, OP_iters=op), 20)
     region = _illuminated(s.P, s.R, s.O.shape)
-    _check_traj(tg, tr, region, "O" if method == 4 else "OP", single)
-    _check_metrics(out_g[-1], out_r[-1], single[0][-1])
+    drift = _check_traj(tg, tr, region, "O" if method == 4 else "OP", single)
+    _check_metrics(out_g[-1], out_r[-1], single[0][-1], iter_drift=drift)
     eb = rel_l2(single[0][-1]["background"], out_r[-1]["background"])
     assert rel_l2(out_g[-1]["background"], out_r[-1]["background"]) < max(1e-4, DRIFT_FACTOR * eb)
+
+
+@pytest.mark.parametrize("method,op", [(1, (1, 1)), (3, (1, 3))])
+def test_dm_default_path_without_callback(method, op, gpu_device):
+    # without a callback one pty_run covers all iterations, so an object-only
+    # projection starts from the previous iteration's metric projection (the
+    # per-iteration calls of the tests above never take that path)
+    # With OP_iters (1, 3) the reference's DM itself diverges on this scan
+    # after about 7 iterations (eMod 37 at the 9th, 1.8e7 at the 18th, in both
+    # precisions of the oracle), so method 3 runs 6 iterations (the projection
+    # is reused at iterations 2 and 5), from a perturbed true probe.
+    iters = 20 if method == 1 else 6
+    print(f"DM method {method}, OP_iters {op}, no callback, {iters} iterations", flush=True)
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=1)
+    P0 = s.P if method == 1 else s.P * (1.0 + 0.2 * np.random.default_rng(8).standard_normal(s.P.shape))
+    args = (s.I, s.R.copy(), P0.copy(), _O0(s))
+    kw = dict(method=method, OP_iters=op)
+    out_g = pty.DM(*args, iters, device=gpu_device, **kw)
+    out_r = oracle.DM(s.I, s.R.copy(), P0.copy(), _O0(s), iters, **kw)
+    out_s = oracle.DM(s.I, s.R.copy(), P0.copy(), _O0(s), iters, dtype='single', **kw)
+    _check_metrics(out_g[-1], out_r[-1], out_s[-1])
+    region = _illuminated(s.P, s.R, s.O.shape)
+    eO = rel_l2(out_g[0][region], out_r[0][region])
+    bO = max(ITER_TOL, DRIFT_FACTOR * rel_l2(out_s[0][region], out_r[0][region]))
+    print(f"final object {eO:.2e} (bound {bO:.2e})")
+    assert eO <= bO
+    if method == 3:
+        eP = rel_l2(out_g[1], out_r[1])
+        assert eP <= max(ITER_TOL, DRIFT_FACTOR * rel_l2(out_s[1], out_r[1]))
 
 
 def test_r_shifted_in_place_and_errors(gpu_device):
