@@ -13,6 +13,8 @@ through properties that do not need the oracle (too slow at this size):
 * the fused inverse-row / object-update kernel against the unfused kernels;
 * DM: from the true object and probe on noise-free data the DM iterate f is
   consistent and satisfies the modulus constraint, so f, O and P stay put;
+  the same with background refinement (method 4, C3's own algorithm) from the
+  true background on data that include it;
   DM with background refinement (method 4) on the noisy data is
   bit-identical between two runs and its metrics stay finite.
 
@@ -163,6 +165,36 @@ def test_c3_dm_fixed_point(gpu_device):
     ep = rel_l2(P_t.cpu().numpy(), scan["P"])
     print(f"object change {eo:.2e}, probe change {ep:.2e}")
     assert eo <= 1e-5 and ep <= 1e-5
+    eng.close()
+    del keep, O_t, P_t, scan
+    torch.cuda.empty_cache()
+
+
+def test_c3_dm_background_fixed_point(gpu_device):
+    # C3's own algorithm, DM with background refinement (method 4), on exact
+    # data I = |FFT(O_n P)|^2 + B from the true object, probe and background B
+    # (the recipe's smooth 1 % background), started from them: the modulus step
+    # with background (Pmod_7) then leaves f and b unchanged, the object
+    # projection returns O and the frame mean of b is b, so every iteration
+    # is the identity.  No beamstop here: Pmod_7 zeroes the masked pixels of
+    # f and b (era.py:589-592), which is no fixed point of exact data.
+    print("full size: DM method 4 fixed point with the true background", flush=True)
+    spec = sim.RECIPES["c3"].with_(beamstop_radius=0.0)
+    scan = sim.make_scan_gpu(spec, gpu_device, poisson=False)
+    assert scan["B"] is not None and np.isscalar(scan["mask"])
+    eng, keep = _engine(scan, gpu_device)
+    O_t, P_t = _state(eng, gpu_device, scan["O"], scan["P"])
+    b_true = np.sqrt(np.fft.ifftshift(scan["B"])).astype(np.float32)
+    eng.set_background(torch.from_numpy(b_true), per_frame=False)
+    m = eng.run(1, 4, 3, begin=True)
+    print("DM-bg eMod at the true object, probe and background:", m[:, 0])
+    assert np.all(m[:, 0] <= 2e-5)
+    H = eng.heatmap().cpu().numpy()
+    lit = H >= 1e-3 * H.max()
+    eo = rel_l2(O_t.cpu().numpy()[lit], scan["O"][lit])
+    eb = rel_l2(eng.background().cpu().numpy(), b_true)
+    print(f"object change {eo:.2e}, background change {eb:.2e}")
+    assert eo <= 1e-5 and eb <= 1e-5
     eng.close()
     del keep, O_t, P_t, scan
     torch.cuda.empty_cache()
