@@ -297,6 +297,12 @@ int pty_nn_gemm_probe(const uint16_t* A, const uint16_t* B, float* C, int M, int
 int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const uint16_t* wT,
                        const float* bias, uint16_t* y, float* yf, int B, int H, int W, int Cin,
                        int Cout, int relu, void* stream);
+/* The first layer (Cin = 1, Cout <= 64) with the network input formed on the
+ * fly from centred float32 intensities I (B x H x W): x = bf16(sqrt(max(I, 0))
+ * / scale), scale > 0 (the prediction's input normalisation). */
+int pty_nn_conv3x3_fwd_intensity(const float* I, float scale, const uint16_t* w, const float* bias,
+                                 uint16_t* y, float* yf, int B, int H, int W, int Cout, int relu,
+                                 void* stream);
 /* dx = conv3x3^T(dy, w); mask (optional, shape of dx): dx = 0 where mask <= 0
  * (the ReLU backward of the layer that produced this conv's input). */
 int pty_nn_conv3x3_dgrad(const uint16_t* dy, const uint16_t* w, uint16_t* dx,

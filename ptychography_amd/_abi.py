@@ -90,6 +90,8 @@ SIGNATURES = {
     "pty_nn_gemm_probe": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp]),
     "pty_nn_conv3x3_fwd": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32,
                                           _i32, _i32, _vp]),
+    "pty_nn_conv3x3_fwd_intensity": (ctypes.c_int, [_vp, _f32, _vp, _vp, _vp, _vp, _i32, _i32, _i32,
+                                                    _i32, _i32, _vp]),
     "pty_nn_conv3x3_dgrad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp]),
     "pty_nn_weight_t": (ctypes.c_int, [_vp, _vp, _i32, _i32, _vp]),
     "pty_nn_conv3x3_wgrad": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32,

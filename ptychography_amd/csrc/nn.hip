@@ -517,11 +517,26 @@ __global__ __launch_bounds__(WG_THREADS) void k_wgrad(const uint16_t* __restrict
   if (do_bias && tid < BN) wsb[(size_t)z * Cout + n0 + tid] = bsum;
 }
 
+// The first layer's input value at element i: a bf16 activation, or (FROM_I)
+// the network input formed from a centred float32 intensity,
+// bf16(sqrt(max(I, 0)) / scale) -- the square root, scaling and rounding of
+// the prediction's input done on the fly instead of in a separate pass.
+template <bool FROM_I>
+__device__ __forceinline__ float c1_input(const void* __restrict__ x, size_t i, float scale) {
+  if constexpr (FROM_I) {
+    const float I = static_cast<const float*>(x)[i];
+    return bf2f(f2bf(sqrtf(fmaxf(I, 0.f)) / scale));
+  } else {
+    return bf2f(static_cast<const uint16_t*>(x)[i]);
+  }
+}
+
 // first layer: one input channel, N <= 64 outputs.  N % 8 == 0: a thread
 // computes 8 channels of one pixel and stores them as one 16-byte vector
 // (consecutive threads -> consecutive channel groups); otherwise one pixel
 // per thread.
-__global__ __launch_bounds__(256) void k_conv_c1(const uint16_t* __restrict__ x,
+template <bool FROM_I>
+__global__ __launch_bounds__(256) void k_conv_c1(const void* __restrict__ x, float scale,
                                                  const uint16_t* __restrict__ w, int B, int H,
                                                  int W, int N, ConvOut o) {
   __shared__ float sw[9 * 64];
@@ -540,7 +555,7 @@ __global__ __launch_bounds__(256) void k_conv_c1(const uint16_t* __restrict__ x,
     for (int t = 0; t < 9; ++t) {
       const int yy = y + t / 3 - 1, x2 = xx + t % 3 - 1;
       v[t] = ((unsigned)yy < (unsigned)H && (unsigned)x2 < (unsigned)W)
-                 ? bf2f(x[pix + (long)(t / 3 - 1) * W + (t % 3 - 1)])
+                 ? c1_input<FROM_I>(x, pix + (long)(t / 3 - 1) * W + (t % 3 - 1), scale)
                  : 0.f;
     }
     if (vec) {
@@ -1165,7 +1180,8 @@ int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const uint16_t* wT,
   const ConvOut o{y, yf, bias, nullptr, relu};
   const size_t P = (size_t)B * H * W;
   if (Cin == 1 && Cout <= 64) {
-    hipLaunchKernelGGL(k_conv_c1, dim3(grid_for(P)), dim3(256), 0, s, x, w, B, H, W, Cout, o);
+    hipLaunchKernelGGL(k_conv_c1<false>, dim3(grid_for(P)), dim3(256), 0, s, (const void*)x, 1.f,
+                       w, B, H, W, Cout, o);
     NN_LAUNCHED("k_conv_c1");
     return PTY_OK;
   }
@@ -1181,6 +1197,20 @@ int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const uint16_t* wT,
   ConvGeom g{B, H, W, Cin, Cout};
   return igemm(B * H * W, Cout, 9 * Cin, LoadFwdA{x, g}, LoadFwdB{w, g},
                EpiConv{y, yf, bias, Cout, relu}, s);
+}
+
+int pty_nn_conv3x3_fwd_intensity(const float* I, float scale, const uint16_t* w, const float* bias,
+                                 uint16_t* y, float* yf, int B, int H, int W, int Cout, int relu,
+                                 void* stream) {
+  if (!I || !w || (!y && !yf) || B <= 0 || H <= 0 || W <= 0 || Cout <= 0 || Cout > 64 ||
+      !(scale > 0.f))
+    return nn_fail(PTY_EINVAL, "bad first-layer conv args (Cout <= 64, scale > 0)");
+  const ConvOut o{y, yf, bias, nullptr, relu};
+  const size_t P = (size_t)B * H * W;
+  hipLaunchKernelGGL(k_conv_c1<true>, dim3(grid_for(P)), dim3(256), 0, (hipStream_t)stream,
+                     (const void*)I, scale, w, B, H, W, Cout, o);
+  NN_LAUNCHED("k_conv_c1");
+  return PTY_OK;
 }
 
 int pty_nn_conv3x3_dgrad(const uint16_t* dy, const uint16_t* w, uint16_t* dx,

@@ -105,10 +105,19 @@ class InitNet:
         self._refresh_transposed()
 
     # -- forward / backward ---------------------------------------------------
-    def forward(self, x, keep=False):
-        """x: (B, H, W, 1) bf16 -> (B, H, W, 2) fp32.  keep: save activations for backward."""
+    def forward(self, x, keep=False, intensity_scale=None):
+        """x: (B, H, W, 1) bf16 -> (B, H, W, 2) fp32.  keep: save activations for backward.
+
+        With ``intensity_scale`` x is instead (B, H, W) float32 centred
+        intensities, and the first layer forms its input
+        bf16(sqrt(max(x, 0)) / intensity_scale) as it reads them (inference)."""
         s = self._stream()
-        B, H, W, C = x.shape
+        from_I = intensity_scale is not None
+        if from_I:
+            assert not keep and x.dtype == torch.float32 and x.is_contiguous()
+            B, H, W = x.shape
+        else:
+            B, H, W, C = x.shape
         acts = [x]
         pi = 0
         out_f = None
@@ -122,9 +131,15 @@ class InitNet:
                                                   device=self.dev)
                 yf = torch.empty((B, H, W, p["cout"]), dtype=torch.float32, device=self.dev) \
                     if last else None
-                _abi.call("pty_nn_conv3x3_fwd", _p(cur), _p(p["wb"]), _p(p["wT"]), _p(p["b"]),
-                          _p(y), _p(yf),
-                          B, H, W, p["cin"], p["cout"], int(p["relu"]), s)
+                if from_I and pi == 1:
+                    assert p["cin"] == 1
+                    _abi.call("pty_nn_conv3x3_fwd_intensity", _p(cur), float(intensity_scale),
+                              _p(p["wb"]), _p(p["b"]), _p(y), _p(yf), B, H, W, p["cout"],
+                              int(p["relu"]), s)
+                else:
+                    _abi.call("pty_nn_conv3x3_fwd", _p(cur), _p(p["wb"]), _p(p["wT"]), _p(p["b"]),
+                              _p(y), _p(yf),
+                              B, H, W, p["cin"], p["cout"], int(p["relu"]), s)
                 if last:
                     out_f = yf
                     acts.append(None)
@@ -342,11 +357,12 @@ def load_weights(path, device=None):
 def predict_patches(net, I, photons, batch=256):
     """Complex patches (n, M, K) complex64 from centred intensities I (n, M, K) on device."""
     frame = I.shape[1]
+    I = I.to(torch.float32).contiguous()
+    scale = input_scale(photons, frame)
     out = []
     for s in range(0, I.shape[0], batch):
-        xb = (torch.sqrt(I[s:s + batch].to(torch.float32).clamp_min(0)) /
-              input_scale(photons, frame)).to(torch.bfloat16)[..., None].contiguous()
-        yb = net.forward(xb)
+        # the first layer forms bf16(sqrt(I) / scale) itself (k_conv_c1)
+        yb = net.forward(I[s:s + batch], intensity_scale=scale)
         out.append(torch.complex(yb[..., 0], yb[..., 1]))
     return torch.cat(out).contiguous()
 

@@ -5,6 +5,14 @@ object seed the network never saw (training objects use seed 5); ERA method 3
 runs from the flat object (the plain start) and from the gated learned start,
 with the same probe and the frames resident on the GPU.  The weights are the
 committed ones, trained on the GPU by ``python -m ptychography_amd.train_init``.
+
+Both starts get the same probe: the true one, or an approximate one (the true
+probe times 1 + 0.3 x real Gaussian noise per pixel), which ERA method 3 then
+refines along with the object.  Bar: eMod <= 1e-3 in at most a third of the
+iterations and a third of the wall time (learned start's set-up included).
+From test.py:39's random-phase probe neither start reaches 1e-3 within 3000
+iterations on this scan (both stall near eMod 0.12), so that start is not a
+case of this test.
 """
 
 import os
@@ -29,11 +37,17 @@ def heldout():
     return s
 
 
-def _era(s, I_d, dev, iters, init=None, net=None, full=True):
+def approximate_probe(P):
+    rng = np.random.default_rng(5)
+    return P * (1.0 + 0.3 * rng.standard_normal(P.shape))
+
+
+def _era(s, I_d, dev, iters, init=None, net=None, full=True, P0=None):
     U, V = s.O.shape
+    P0 = s.P if P0 is None else P0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    r = ERA(I_d, s.R.copy(), s.P.copy(), np.ones((U, V), np.complex64), iters, mask=1, method=3,
+    r = ERA(I_d, s.R.copy(), P0.copy(), np.ones((U, V), np.complex64), iters, mask=1, method=3,
             device=dev, init=init, init_net=net, full_output=full)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, r
@@ -45,26 +59,30 @@ def _iters_to(e, target):
 
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
-def test_learned_start_reaches_the_target_faster(heldout, gpu_device):
+@pytest.mark.parametrize("probe", ["true", "approximate"])
+def test_learned_start_reaches_the_target_faster(heldout, gpu_device, probe):
     s = heldout
+    P0 = s.P if probe == "true" else approximate_probe(s.P)
+    print(f"learned vs plain start, {probe} probe", flush=True)
     net = load_weights(WEIGHTS, device=gpu_device)
     I_d = torch.from_numpy(s.I.astype(np.float32)).to(gpu_device)
-    _, (_, _, info_p) = _era(s, I_d, gpu_device, 1500)
-    _, (_, _, info_l) = _era(s, I_d, gpu_device, 1500, "learned", net)
+    _, (_, _, info_p) = _era(s, I_d, gpu_device, 1500, P0=P0)
+    _, (_, _, info_l) = _era(s, I_d, gpu_device, 1500, "learned", net, P0=P0)
     init = info_l["init"]
     print("plain start eMod %.3e, learned %.3e, init %.1f ms" %
           (init["eMod_plain_start"], init["eMod_learned_start"], 1e3 * init["init_seconds"]))
-    assert init["used_learned"] and init["eMod_learned_start"] < 0.2 * init["eMod_plain_start"]
+    assert init["used_learned"] and init["eMod_learned_start"] < 0.5 * init["eMod_plain_start"]
     k_p, k_l = _iters_to(info_p["eMod"], TARGET), _iters_to(info_l["eMod"], TARGET)
     print(f"iterations to eMod <= {TARGET}: plain {k_p}, learned {k_l}")
     assert k_p is not None and k_l is not None and 3 * k_l <= k_p
     # wall time to the target, everything resident; median of 3
-    _era(s, I_d, gpu_device, k_p, full=False)
-    _era(s, I_d, gpu_device, k_l, "learned", net, full=False)
-    t_p = np.median([_era(s, I_d, gpu_device, k_p, full=False)[0] for _ in range(3)])
-    t_l = np.median([_era(s, I_d, gpu_device, k_l, "learned", net, full=False)[0] for _ in range(3)])
+    _era(s, I_d, gpu_device, k_p, full=False, P0=P0)
+    _era(s, I_d, gpu_device, k_l, "learned", net, full=False, P0=P0)
+    t_p = np.median([_era(s, I_d, gpu_device, k_p, full=False, P0=P0)[0] for _ in range(3)])
+    t_l = np.median([_era(s, I_d, gpu_device, k_l, "learned", net, full=False, P0=P0)[0]
+                     for _ in range(3)])
     print(f"wall to target: plain {1e3 * t_p:.1f} ms, learned {1e3 * t_l:.1f} ms ({t_p / t_l:.2f}x)")
-    assert t_p > 2.0 * t_l
+    assert t_p >= 3.0 * t_l
 
 
 def test_untrained_network_is_rejected_by_the_gate(heldout, gpu_device):

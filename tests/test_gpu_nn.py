@@ -312,3 +312,17 @@ def test_batch_gather_and_augmentation(gpu_device):
             assert codes == [0] * B
         else:
             assert len(set(codes)) > 1
+
+
+def test_first_layer_forms_its_input_from_intensities(gpu_device):
+    # forward(I, intensity_scale) builds bf16(sqrt(max(I, 0)) / scale) inside
+    # the first conv; the result must equal forward() on that input formed by torch
+    from ptychography_amd.learned import InitNet
+    net = InitNet(seed=3, device=gpu_device)
+    g = torch.Generator(device=gpu_device).manual_seed(9)
+    I = torch.rand((8, 64, 64), device=gpu_device, generator=g) * 400.0 - 5.0   # some negative
+    scale = 3.7
+    x = (torch.sqrt(I.clamp_min(0)) / scale).to(torch.bfloat16)[..., None].contiguous()
+    a = net.forward(x)
+    b = net.forward(I.contiguous(), intensity_scale=scale)
+    assert torch.equal(a, b)
