@@ -457,6 +457,26 @@ def main():
         per = float(np.median(times)) / args.dm_steps
         return per, float(m[-1, 0])
 
+    # ERA method 3 (object + probe, the algorithm of the 128^2 and mixed-scan
+    # workloads) at the same shape, from the true probe
+    era3 = None
+    if args.dm_steps > 0:
+        eng = Engine(N, M, M, U, V, scan["R"], device=dev)
+        O_t, P_t, amp = bind(eng, scan, dev, np.ones((U, V), np.complex64), scan["P"])
+        eng.run(0, 3, 2, iter0=0, begin=True)
+        it, times = 2, []
+        for _ in range(max(1, args.repeats)):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            eng.run(0, 3, args.dm_steps, iter0=it)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t1)
+            it += args.dm_steps
+        eng.close()
+        del amp, O_t, P_t
+        per = float(np.median(times)) / args.dm_steps
+        era3 = {"era3_iters_per_s": 1.0 / per, "era3_ms_per_step": 1e3 * per}
+
     dm = None
     if args.dm_steps > 0:
         per4, e4 = dm_timing(4)
@@ -469,6 +489,7 @@ def main():
               "dm1_roofline_frac": (DM1_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per1 / (HBM_PEAK_GBS * 1e9),
               "dm3_op13_iters_per_s": 1.0 / per3, "dm3_op13_ms_per_step": 1e3 * per3,
               "repeats": max(1, args.repeats), "timing": "median of the repeats"}
+        dm.update(era3 or {})
 
     learned = None
     if not args.no_learned and os.path.exists(WEIGHTS):

@@ -632,7 +632,7 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
         const int2 o = s_o[b];
         const int j = v - o.y;
         const bool jin = b0f + f < cnt && (unsigned)j < (unsigned)K;
-        const float2* Sn = a.S + (size_t)n * M * K + (jin ? j : 0);
+        const float2* Sn = a.S ? a.S + (size_t)n * M * K + (jin ? j : 0) : nullptr;
         const float2* Pj = a.P + (jin ? j : 0);
 #pragma unroll
         for (int k = 0; k < PY; ++k) {
@@ -640,7 +640,7 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
           ok[f][k] = jin && (unsigned)i < (unsigned)M;
           const size_t ro = ok[f][k] ? (size_t)i * K : 0;
           ps[f][k] = ok[f][k] ? Pj[ro] : make_float2(0.f, 0.f);
-          ss[f][k] = ok[f][k] ? Sn[ro] : make_float2(0.f, 0.f);
+          ss[f][k] = (ok[f][k] && Sn) ? Sn[ro] : make_float2(0.f, 0.f);
         }
       }
 #pragma unroll
@@ -654,6 +654,16 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
         }
       }
     }
+  }
+  if (!a.S) {
+    // heatmap only (make_P_heatmap, era.py:402-410): the sums above, in the
+    // same frame order, without the exit waves
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+      const int u = u0 + ty + k * (WG / TILE_W);
+      if (u < g.U && v < g.V) a.H[(size_t)u * g.V + v] = hacc[k];
+    }
+    return;
   }
   double r[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll

@@ -700,6 +700,10 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   const bool staged = !(c->use_frame && !bgm);
   // object-only step with a valid heatmap: inverse row FFT fused into the update
   const bool rowg = upd == 0 && staged && c->rowg && c->H_valid;
+  // single-pass object + probe step: the heatmap of the current probe in a
+  // pass of its own, then the inverse row FFT fused into the object update,
+  // which also stores psi' (into F) for the probe update
+  const bool rowg_op = fusedOP && staged && c->rowg && c->F;
 
   if (!fusedO && !fusedP && !fusedOP) {
     StageTimer tm(c, PTY_STAGE_MISC);
@@ -744,7 +748,8 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
         PTY_TRY(cols<COL_ERA1>(c, a, &nc));
       }
       ncol += nc;
-      if (!rowg) PTY_TRY(rows_inv<DST_STORE>(c, Xc, Xc, nullptr, nullptr, 1.0f, nullptr, nf * c->M));
+      if (!rowg && !rowg_op)
+        PTY_TRY(rows_inv<DST_STORE>(c, Xc, Xc, nullptr, nullptr, 1.0f, nullptr, nf * c->M));
     }
   }
 
@@ -757,6 +762,13 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     c->H_valid = true;
   } else if (upd == 1) {
     PTY_TRY(probe_update(c, c->X, c->O, c->P, c->P, c->G, !c->G_valid, 1, o.alpha, fusedP));
+    c->G_valid = true;
+  } else if (rowg_op) {
+    c->dm_ready = false;   // F now holds psi', not a DM iterate: a DM run must begin again
+    PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o.alpha, false, false));
+    c->H_valid = true;
+    PTY_TRY(rows_gather(c, c->X, o.alpha, true, false, &ngather, c->F));
+    PTY_TRY(probe_update(c, c->F, c->O, c->P, c->P, c->G, 1, 1, o.alpha, true, esup_probe));
     c->G_valid = true;
   } else if (fusedOP) {
     PTY_TRY(gather(c, c->X, c->P, c->O, c->O, c->H, 1, 1, o.alpha, true, false));
@@ -801,8 +813,8 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   } else if (fusedOP) {
     const int nb = (int)((MK + WG - 1) / WG);
     rg.p[S_ECON_ON] = c->part_econO;
-    rg.p[S_ECON_OD] = c->part_econO + c->ntiles;
-    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = c->ntiles;
+    rg.p[S_ECON_OD] = c->part_econO + ngather;
+    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ngather;
     rg.p[S_ECON_PN] = c->part_econP;
     rg.p[S_ECON_PD] = c->part_econP + nb;
     rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = nb;
@@ -1156,6 +1168,9 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
     }
   }
   if (o->algorithm == 1 && !c->dm_ready) return fail(PTY_ESTATE, "DM run needs begin=1 first");
+  // ERA's object + probe steps keep psi' in F when the fused row gather runs
+  // them (allocated here, never inside a capture)
+  if (o->algorithm == 0 && (o->method == 3 || o->method == 6) && c->rowg) PTY_TRY(ensure_F(c));
   c->last_algo = o->algorithm;
   if (c->use_frame) PTY_TRY(ensure_slots(c));  // never allocate inside a capture
   return PTY_OK;
