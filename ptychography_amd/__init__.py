@@ -10,6 +10,7 @@ from .dm import DM
 from .batch import ERA_many, DM_many
 from .back_projection import Back_projection
 from .hio import HIO
+from .cxi import read_cxi, write_cxi
 from . import sim
 
-__all__ = ["ERA", "DM", "update_progress", "ERA_many", "DM_many", "Back_projection", "HIO", "sim"]
+__all__ = ["ERA", "DM", "update_progress", "ERA_many", "DM_many", "Back_projection", "HIO", "sim", "write_cxi", "read_cxi"]
