@@ -85,6 +85,25 @@ int pty_set_background(pty_ctx* ctx, const float* b_dev, int32_t per_frame);
  * it now from the bound probe. */
 int pty_set_pamp(pty_ctx* ctx, const float* pamp_dev);
 
+/* Sub-pixel positions (the Fourier-shift roll of float positions,
+ * bagOfns.py:729-764, used on the sample by utility_Ptych.py:42-48; the
+ * reference's ERA / DM, era.py:426-433, take integer R only).  The context's
+ * R (pty_create) holds each position's integer window; d_host (N x 2 float64,
+ * (y, x) per frame) the remainder: frame n's window start is -R[n] + d[n] and
+ * it sees the probe shifted by d[n], P_n = IFFT2(FFT2(P) ramp_n) with
+ * ramp_n[ky, kx] = exp(-2 pi i (d_n0 f(ky) / M + d_n1 f(kx) / K)), f(k) =
+ * ((k + N/2 - 1) mod N) - (N/2 - 1).  The object update uses P_n in place of
+ * P (numerator sum conj(P_n) psi_n, heatmap sum |P_n|^2).  Shifts must be
+ * finite with |d| <= 1; NULL returns the context to integer positions.  The
+ * stack P_n (N x M x K, allocated here on first use) is rebuilt from the bound
+ * P by every pty_run with begin = 1 and after pty_set_state.  Runs with
+ * shifts set take methods 1 and 4 (object and background updates; the probe
+ * stays fixed) without Pmod_probe: anything else is PTY_EUNSUP.  The entry
+ * points that take the probe as an argument (pty_make_exits, pty_psup_O,
+ * pty_back_projection, pty_hio_modulus, pty_add_exits) ignore the shifts.
+ * Synchronous. */
+int pty_set_subpixel(pty_ctx* ctx, const double* d_host);
+
 /* Run `iters` iterations with global iteration numbers iter0, iter0+1, ...
  * (they drive the OP_iters schedule and the Pmod_probe window).  begin = 1
  * starts a new solve (DM: f = make_exits(O, P), eCon reference = O/P;

@@ -34,6 +34,7 @@ def _project(update, i, OP_iters, exits, O, P, R, P_heatmap, O_heatmap, alpha,
     """
     cm = None
     if update == 'O':
+        # P may be the per-frame probe stack of sub-pixel positions
         O, P_heatmap = _era.psup_O(exits, P, R, O.shape, P_heatmap, alpha)
     elif update == 'P':
         P, O_heatmap = _era.psup_P(exits, O, R, O_heatmap, alpha)
@@ -85,15 +86,21 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
                               np.asarray(resume['P_ref'], st['c_dtype'])).copy()
     Os, Ps = O, P
     b_0 = np.empty_like(background) if bg_mode else None
+    # sub-pixel positions: the frames see per-frame probes (object-only methods)
+    Pf = st['Pn']
+    if Pf is not None and Pmod_probe is not None and Pmod_probe is not False:
+        raise ValueError("sub-pixel positions are modelled without the far-field probe constraint")
 
     for i in range(i0, i0 + iters):
         if bg_mode:
             # leading modulus projection of f and b (dm.py:171)
             exits, background = _era.pmod_7(amp, background, exits, mask, alpha=alpha)
 
-        O, P, P_heatmap, O_heatmap, cm = _project(
-            update, i, OP_iters, exits, O, P, R, P_heatmap, O_heatmap, alpha,
-            probe_centering, metric=False)
+        O, Pp, P_heatmap, O_heatmap, cm = _project(
+            update, i, OP_iters, exits, O, P if Pf is None else Pf, R, P_heatmap, O_heatmap,
+            alpha, probe_centering, metric=False)
+        if Pf is None:
+            P = Pp
         if cm is not None:
             # dm.py:82 / :200 -- the remembered exits follow the probe shift
             exits = _era.multiroll(exits, (0, -cm[0], -cm[1]))
@@ -102,7 +109,7 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
             P = _era.Pmod_P(Pamp, P, mask, alpha)
 
         f_in = exits.copy()
-        ex_0 = _era.make_exits(O, P, R)
+        ex_0 = _era.make_exits(O, P if Pf is None else Pf, R)
         if bg_mode:
             b_0[:] = np.mean(background, axis=0)          # dm.py:208-213
             exits -= ex_0                                 # f - Ps f
@@ -122,13 +129,14 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
         eSups.append(float(np.sqrt(MK * np.sum((d * d.conj()).real) / I_norm)))
 
         # metrics (dm.py:104-137 / :231-271)
-        Os, Ps, P_heatmap, O_heatmap, _ = _project(
-            update, i, OP_iters, exits, O.copy(), P.copy(), R, P_heatmap,
+        Os, Psp, P_heatmap, O_heatmap, _ = _project(
+            update, i, OP_iters, exits, O.copy(), P.copy() if Pf is None else Pf, R, P_heatmap,
             O_heatmap, alpha, probe_centering, metric=True)
+        Ps = Psp if Pf is None else P.copy()
         if Pmod_probe is not None and i < Pmod_probe:
             Ps = _era.Pmod_P(Pamp, Ps, mask, alpha)
 
-        ex_m = _era.make_exits(Os, Ps, R)
+        ex_m = _era.make_exits(Os, Ps if Pf is None else Pf, R)
         if bg_mode:
             b_0 = np.empty_like(background)
             b_0[:] = np.mean(background, axis=0)          # dm.py:263-268
@@ -155,7 +163,7 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
             return P
         return O, P
 
-    ex_f = _era.make_exits(Os, Ps, R)
+    ex_f = _era.make_exits(Os, Ps if Pf is None else Pf, R)
     info = {}
     info['I'] = np.fft.fftshift(np.abs(np.fft.fftn(ex_f, axes=(-2, -1))) ** 2,
                                 axes=(-2, -1))

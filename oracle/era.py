@@ -86,26 +86,30 @@ def make_exits(O, P, R, exits=None):
     In the default "roll" mode this keeps the reference's cost model: every
     frame rolls the whole object.
     """
+    # P may be a (N, M, K) stack of per-frame probes (sub-pixel positions,
+    # oracle/subpixel.py): frame n then uses P[n]
+    per_frame = P.ndim == 3
+    M, K = P.shape[-2:]
     if exits is None:
-        exits = np.empty((len(R),) + P.shape, dtype=P.dtype)
-    M, K = P.shape
+        exits = np.empty((len(R), M, K), dtype=P.dtype)
     if WINDOWS == "slice" and _inside(R, M, K, O.shape):
         for n in range(len(R)):
             r0, r1 = int(R[n][0]), int(R[n][1])
-            exits[n] = O[-r0:M - r0, -r1:K - r1] * P
+            exits[n] = O[-r0:M - r0, -r1:K - r1] * (P[n] if per_frame else P)
         return exits
     for n in range(len(R)):
-        exits[n] = multiroll(O, (R[n][0], R[n][1]))[:M, :K] * P
+        exits[n] = multiroll(O, (R[n][0], R[n][1]))[:M, :K] * (P[n] if per_frame else P)
     return exits
 
 
 def make_P_heatmap(P, R, shape):
-    """Sum of |P|^2 placed at every frame window of the object (era.py:402-410)."""
-    M, K = P.shape
+    """Sum of |P|^2 placed at every frame window of the object (era.py:402-410)
+    (per-frame |P_n|^2 for a stack of per-frame probes)."""
+    M, K = P.shape[-2:]
     heat = np.zeros(shape, dtype=P.real.dtype)
     p2 = (P.conj() * P).real
-    for r in R:
-        heat[-r[0]:M - r[0], -r[1]:K - r[1]] += p2
+    for n, r in enumerate(R):
+        heat[-r[0]:M - r[0], -r[1]:K - r[1]] += p2[n] if P.ndim == 3 else p2
     return heat
 
 
@@ -134,13 +138,13 @@ def psup_O(exits, P, R, O_shape, P_heatmap=None, alpha=1.0e-10):
 
     ``H`` (the P heatmap) is recomputed only when ``P_heatmap is None``.
     """
-    M, K = P.shape
+    M, K = P.shape[-2:]
     numer = np.zeros(O_shape, P.dtype)
     if P_heatmap is None:
         P_heatmap = make_P_heatmap(P, R, O_shape)
     Pc = P.conj()
-    for r, ex in zip(R, exits):
-        numer[-r[0]:M - r[0], -r[1]:K - r[1]] += ex * Pc
+    for n, (r, ex) in enumerate(zip(R, exits)):
+        numer[-r[0]:M - r[0], -r[1]:K - r[1]] += ex * (Pc[n] if P.ndim == 3 else Pc)
     return numer / (P_heatmap + alpha), P_heatmap
 
 
@@ -300,9 +304,13 @@ def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype
     else:
         raise ValueError("dtype must be None, 'single' or 'double'")
 
+    from . import subpixel
+    frac = subpixel.is_fractional(R)
+    if frac and method not in (1, 4):
+        raise ValueError("sub-pixel positions are modelled for the object-only methods 1 and 4")
     if O is None:
-        shape = (I.shape[1] + R[:, 0].max() - R[:, 0].min(),
-                 I.shape[2] + R[:, 1].max() - R[:, 1].min())
+        shape = (I.shape[1] + int(np.ceil(R[:, 0].max() - R[:, 0].min())),
+                 I.shape[2] + int(np.ceil(R[:, 1].max() - R[:, 1].min())))
         O = np.ones(shape, dtype=c_dtype)
     if P is None:
         P = _random(rng, I[0].shape) + 1J * _random(rng, I[0].shape)
@@ -318,7 +326,15 @@ def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype
     R[:, 0] -= R[:, 0].max()
     R[:, 1] -= R[:, 1].max()
 
-    exits = make_exits(O, P, R)
+    # sub-pixel positions: integer windows and per-frame probes (oracle/subpixel.py)
+    Pn = None
+    if frac:
+        R, d = subpixel.split_positions(R)
+        Pn = subpixel.frame_probes(P, d)
+    elif np.issubdtype(R.dtype, np.floating):
+        R = R.astype(np.int64)      # integer-valued floats: the integer model
+
+    exits = make_exits(O, P if Pn is None else Pn, R)
 
     if method in (4, 5, 6):
         if background is None:
@@ -335,7 +351,7 @@ def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype
     return dict(method=method, update=update, dtype=r_dtype, c_dtype=c_dtype,
                 OP_iters=OP_iters, O=O, P=P, amp=amp, Pamp=Pamp,
                 background=background, R=R, mask=mask, I_norm=I_norm,
-                N=len(I), exits=exits)
+                N=len(I), exits=exits, Pn=Pn)
 
 
 def _state(update, O, P):
@@ -393,6 +409,11 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
     MK = P.shape[0] * P.shape[1]
 
     eMods, eCons, eSups = [], [], []
+    # the probes the frames see (per-frame for sub-pixel positions; P never
+    # changes in the object-only methods those are modelled for)
+    Pn = st['Pn']
+    if Pn is not None and Pmod_probe is not None and Pmod_probe is not False:
+        raise ValueError("sub-pixel positions are modelled without the far-field probe constraint")
 
     for i in range(i0, i0 + iters):
         bak = _state(update, O, P).copy()
@@ -406,7 +427,7 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
 
         # consistency projection (era.py:163-189 / :233-259)
         if update == 'O':
-            O, P_heatmap = psup_O(exits, P, R, O.shape, P_heatmap, alpha)
+            O, P_heatmap = psup_O(exits, P if Pn is None else Pn, R, O.shape, P_heatmap, alpha)
         elif update == 'P':
             P, O_heatmap = psup_P(exits, O, R, O_heatmap, alpha)
         else:
@@ -429,7 +450,7 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
             background[:] = np.mean(background, axis=0)
 
         psi_mod = exits
-        exits = make_exits(O, P, R)
+        exits = make_exits(O, P if Pn is None else Pn, R)
 
         # build addition: support error on the frames just projected
         d = psi_mod - exits

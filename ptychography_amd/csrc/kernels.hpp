@@ -28,7 +28,16 @@ constexpr int WG = 256;
 struct Geom {
   int N, M, K, U, V;
   const int2* __restrict__ off;  // (y, x) window offset per frame
+  // sub-pixel positions: frame n sees its own probe Pn[n] ([N][M][K], the
+  // probe Fourier-shifted by the position's fractional part) in place of the
+  // shared P; null for integer positions
+  const float2* Pn;
 };
+
+// the probe row frame row `row` (= n M + i) sees
+__device__ __forceinline__ const float2* probe_row(const Geom& g, const float2* P, int row, int i) {
+  return g.Pn ? g.Pn + (size_t)row * g.K : P + (size_t)i * g.K;
+}
 
 // ---------------------------------------------------------------------------
 // block reduction of doubles (wave64 shuffles, then LDS across waves)
@@ -137,8 +146,8 @@ void k_rows_fwd(Geom g, const float2* O, const float2* P,
 #pragma unroll
       for (int e = 0; e < E; ++e) v[e] = n0[e];
     } else {
-      const int i = min(grp * LPW + line, nrows - 1) % M;
-      const float2* Prow = P + (size_t)i * L;
+      const int row = min(grp * LPW + line, nrows - 1);
+      const float2* Prow = probe_row(g, P, row, row % M);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         float2 psi = cmul(n0[e], Prow[layA<L>(e, t)]);
@@ -245,7 +254,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
         const int M = g.M, n = row / M, i = row % M;
         const int2 o = g.off[n];
         const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
-        const float2* Prow = P + (size_t)i * L;
+        const float2* Prow = probe_row(g, P, row, i);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int c = layA<L>(e, t);
@@ -633,7 +642,7 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
         const int j = v - o.y;
         const bool jin = b0f + f < cnt && (unsigned)j < (unsigned)K;
         const float2* Sn = a.S ? a.S + (size_t)n * M * K + (jin ? j : 0) : nullptr;
-        const float2* Pj = a.P + (jin ? j : 0);
+        const float2* Pj = (g.Pn ? g.Pn + (size_t)n * M * K : a.P) + (jin ? j : 0);
 #pragma unroll
         for (int k = 0; k < PY; ++k) {
           const int i = u0 + ty + k * (WG / TILE_W) - o.x;
@@ -817,7 +826,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
   }
   for (int bt = 0; bt < nbatch; ++bt) {
     const int i = m_cur.y & 0xffff;
-    const float2* Prow = a.P + (size_t)i * L;
+    const float2* Prow = probe_row(g, a.P, m_cur.x, i);
     float2 pv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) pv[e] = Prow[layA<L>(e, t)];
@@ -1038,7 +1047,7 @@ __global__ __launch_bounds__(WG) void k_make_exits(Geom g, const float2* O, cons
     const int n = (int)(idx / MK);
     const int p = (int)(idx % MK), i = p / g.K, j = p % g.K;
     const int2 o = g.off[n];
-    out[idx] = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[p]);
+    out[idx] = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], g.Pn ? g.Pn[idx] : P[p]);
   }
 }
 
@@ -1074,12 +1083,35 @@ __global__ __launch_bounds__(WG) void k_esup(Geom g, const float2* S, const floa
       const int n = row / M, i = row - n * M;
       const int2 o = g.off[n];
       const float2 d = S[(size_t)row * K + j] -
-                       cmul(O[(size_t)(o.x + i) * g.V + o.y + j], P[(size_t)i * K + j]);
+                       cmul(O[(size_t)(o.x + i) * g.V + o.y + j], probe_row(g, P, row, i)[j]);
       r[0] += (double)norm2(d);
     }
   }
   block_sum<1>(r, red);
   if (threadIdx.x == 0) part[blockIdx.x] = r[0];
+}
+
+// Pn[n, ky, kx] = Phat[ky, kx] exp(-2 pi i (d_n0 f(ky) / M + d_n1 f(kx) / K)):
+// the spectrum of the probe shifted by d_n (y[r] = x[r - d]), with the
+// frequency of bin k taken as ((k + N/2 - 1) mod N) - (N/2 - 1), i.e. in
+// [-(N/2 - 1), N/2] (bagOfns.py:729-764 with phase_ramp, :966-979)
+__global__ __launch_bounds__(WG) void k_subpixel_ramp(const float2* __restrict__ Phat,
+                                                      const double2* __restrict__ d, float2* Pn,
+                                                      int N, int M, int K) {
+  const size_t MK = (size_t)M * K, total = MK * N;
+  for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * WG) {
+    const int n = (int)(idx / MK), p = (int)(idx % MK);
+    const int ky = p / K, kx = p % K;
+    const int fy = (ky + M / 2 - 1) % M - (M / 2 - 1);
+    const int fx = (kx + K / 2 - 1) % K - (K / 2 - 1);
+    const double2 dn = d[n];
+    const double ang = -2.0 * M_PI * (dn.x * fy / M + dn.y * fx / K);
+    double sn, cs;
+    sincos(ang, &sn, &cs);
+    const float2 h = Phat[p];
+    Pn[idx] = make_float2((float)(h.x * cs - h.y * sn), (float)(h.x * sn + h.y * cs));
+  }
 }
 
 // partials of sum |bak - cur|^2 and sum |cur|^2 (convergence error)

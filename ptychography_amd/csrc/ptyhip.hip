@@ -184,6 +184,13 @@ struct pty_ctx {
   float2* slots = nullptr;   // frame_slots * M*K scratch
   size_t slots_cap = 0;      // slots allocated
 
+  // sub-pixel positions: per-frame probes Pn[n] = P shifted by d_n (the
+  // fractional part of frame n's position); Geom::Pn points at them while set
+  double2* sub_d = nullptr;  // N shifts d_n (y, x)
+  float2* Pn = nullptr;      // N*M*K per-frame probes
+  bool sub_set = false;      // shifts given: runs use the per-frame probes
+  bool Pn_valid = false;     // Pn holds the shifts of the bound P
+
   // hipGraph replay of whole iterations
   struct GraphEntry {
     std::vector<uint64_t> key;
@@ -667,12 +674,26 @@ int chunk_frames(const pty_ctx* c) {
   return std::min(G, c->N);
 }
 
+// the fused per-frame kernel reads the one shared probe: sub-pixel positions
+// take the staged kernels
+bool frame_path(const pty_ctx* c) { return c->use_frame && !c->g.Pn; }
+
+// The entry points that take the probe as an argument apply that one probe
+// to every frame, whatever sub-pixel shifts the context holds.
+struct SharedProbe {
+  pty_ctx* c;
+  const float2* saved;
+  explicit SharedProbe(pty_ctx* c_) : c(c_), saved(c_->g.Pn) { c->g.Pn = nullptr; }
+  ~SharedProbe() { c->g.Pn = saved; }
+};
+
 // frames [n0, n0 + n) seen as a scan of their own by the row / column kernels
 struct ChunkView {
   pty_ctx* c;
   Geom saved;
   ChunkView(pty_ctx* c_, int n0, int n) : c(c_), saved(c_->g) {
     c->g.off = saved.off + n0;
+    if (saved.Pn) c->g.Pn = saved.Pn + (size_t)n0 * c->MK();
     c->g.N = n;
   }
   ~ChunkView() { c->g = saved; }
@@ -697,7 +718,7 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   // update (no centring, no far-field constraint) from the probe update
   const bool esup_probe = op_step && !o.probe_centering && !pmodp;
   const bool esup_id = (upd == 0 && !pmodp) || esup_probe;
-  const bool staged = !(c->use_frame && !bgm);
+  const bool staged = !(frame_path(c) && !bgm);
   // object-only step with a valid heatmap: inverse row FFT fused into the update
   const bool rowg = upd == 0 && staged && c->rowg && c->H_valid;
   // single-pass object + probe step: the heatmap of the current probe in a
@@ -939,7 +960,7 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
 
   // f <- f - Ps f + Pm(2 Ps f - f)    (dm.py:91-97, :216-224)
   int nstep = 0;
-  if (c->use_frame && !bgm) {
+  if (frame_path(c) && !bgm) {
     FrameArgs f = frame_args(c);
     f.O = c->O;
     f.P = c->P;
@@ -981,7 +1002,7 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   if (pmodp) PTY_TRY(pmod_probe(c, c->Ps, o.alpha));
   if (bgm) PTY_TRY(bg_mean(c, nchunks, c->bbar_m));
   int ncol = 0;
-  if (c->use_frame && !bgm) {
+  if (frame_path(c) && !bgm) {
     FrameArgs f = frame_args(c);
     f.O = c->Os;
     f.P = c->Ps;
@@ -1043,7 +1064,8 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(uintptr_t)c->O, (uint64_t)(uintptr_t)c->P, (uint64_t)(uintptr_t)c->F,
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
-          (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp};
+          (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp,
+          (uint64_t)(uintptr_t)c->g.Pn};
 }
 
 HostIterState save_state(const pty_ctx* c) {
@@ -1138,6 +1160,33 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
 }
 
 // validation and per-run set-up of pty_run (before any iteration)
+// Pn[n] = IFFT(FFT(P) ramp(d_n)): the forward transform of P into Xp, one
+// phase ramp per frame, then the batched inverse transform of the stack
+int build_subpixel_probes(pty_ctx* c) {
+  if (!c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
+  StageTimer tm(c, PTY_STAGE_MISC);
+  c->g.Pn = nullptr;
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->P, c->Xp, c->M));
+  ColArgs a = col_args(c);
+  a.X = c->Xp;
+  a.nframes = 1;
+  a.chunk = 1;
+  PTY_TRY(cols<COL_FWD>(c, a));
+  hipLaunchKernelGGL(k_subpixel_ramp, dim3(2048), dim3(WG), 0, c->stream, (const float2*)c->Xp,
+                     (const double2*)c->sub_d, c->Pn, c->N, c->M, c->K);
+  PTY_LAUNCHED("k_subpixel_ramp");
+  ColArgs b = col_args(c);
+  b.X = c->Pn;
+  b.nframes = c->N;
+  b.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+  PTY_TRY(cols<COL_INV>(c, b));
+  PTY_TRY(rows_inv<DST_STORE>(c, c->Pn, c->Pn, nullptr, nullptr, 1.0f, nullptr, c->N * c->M));
+  c->g.Pn = c->Pn;
+  c->Pn_valid = true;
+  c->H_valid = false;   // the heatmap is the sum of the per-frame |Pn|^2
+  return PTY_OK;
+}
+
 int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t begin) {
   if (!c || !o) return fail(PTY_EINVAL, "NULL argument");
   if (o->method < 1 || o->method > 6) return fail(PTY_EINVAL, "method must be 1..6");
@@ -1145,6 +1194,13 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
   if (o->op_n < 0 || o->op_every < 1) return fail(PTY_EINVAL, "bad OP_iters");
   if (iters < 0) return fail(PTY_EINVAL, "iters < 0");
   PTY_TRY(check_bound(c, o->method >= 4));
+  if (c->sub_set) {
+    if (o->method != 1 && o->method != 4)
+      return fail(PTY_EUNSUP, "sub-pixel positions are modelled for the object-only methods 1 and 4");
+    if (o->pmod_probe_until > iter0)
+      return fail(PTY_EUNSUP, "sub-pixel positions are modelled without the far-field probe constraint");
+    if (begin || !c->Pn_valid) PTY_TRY(build_subpixel_probes(c));
+  }
   c->os_next = false;   // the caller may have changed O, P or the data since
   if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
   if (iters > c->metrics_cap) {
@@ -1489,6 +1545,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->H); dev_free(c->G); dev_free(c->bg); dev_free(c->bbar); dev_free(c->bbar_m);
   dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->pnum); dev_free(c->pg);
   dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
+  dev_free(c->sub_d); dev_free(c->Pn);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1546,6 +1603,28 @@ int pty_set_state(pty_ctx* c, pty_cfloat* O, pty_cfloat* P) {
   c->O = reinterpret_cast<float2*>(O);
   c->P = reinterpret_cast<float2*>(P);
   c->H_valid = c->G_valid = false;
+  c->Pn_valid = false;
+  c->g.Pn = nullptr;
+  return PTY_OK;
+}
+
+int pty_set_subpixel(pty_ctx* c, const double* d_host) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->H_valid = false;
+  c->Pn_valid = false;
+  c->g.Pn = nullptr;
+  if (!d_host) {
+    c->sub_set = false;
+    return PTY_OK;
+  }
+  for (int k = 0; k < 2 * c->N; ++k)
+    if (!std::isfinite(d_host[k]) || std::fabs(d_host[k]) > 1.0)
+      return fail(PTY_EINVAL, "sub-pixel shifts must be finite and within [-1, 1]");
+  if (!c->sub_d) PTY_TRY(dev_alloc(&c->sub_d, (size_t)c->N));
+  if (!c->Pn) PTY_TRY(dev_alloc(&c->Pn, c->NMK()));
+  PTY_HIP(hipMemcpy(c->sub_d, d_host, (size_t)c->N * sizeof(double2), hipMemcpyHostToDevice));
+  c->sub_set = true;
   return PTY_OK;
 }
 
@@ -1953,7 +2032,7 @@ int pty_intensity(pty_ctx* c, float* I) {
   const bool dm = c->last_algo == 1 && c->have_metric_state;
   const float2* Ob = dm ? c->Os : c->O;
   const float2* Pb = dm ? c->Ps : c->P;
-  if (c->use_frame) {
+  if (frame_path(c)) {
     FrameArgs f = frame_args(c);
     f.O = Ob;
     f.P = Pb;
@@ -2062,6 +2141,7 @@ int pty_fft2(pty_ctx* c, pty_cfloat* data, int32_t nframes, int32_t inverse) {
 
 int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfloat* ex) {
   if (!c || !O || !P || !ex) return fail(PTY_EINVAL, "NULL argument");
+  SharedProbe shared(c);
   hipLaunchKernelGGL(k_make_exits, dim3(4096), dim3(WG), 0, c->stream, c->g,
                      reinterpret_cast<const float2*>(O), reinterpret_cast<const float2*>(P),
                      reinterpret_cast<float2*>(ex));
@@ -2077,6 +2157,7 @@ int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, const pty_cfloat* 
   if (f == m) return fail(PTY_EINVAL, "f and m must be different buffers");
   if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
   if (!(beta != 0.0f) || !std::isfinite(beta)) return fail(PTY_EINVAL, "beta must be finite and non-zero");
+  SharedProbe shared(c);
   float2* F = reinterpret_cast<float2*>(f);
   float2* Mb = reinterpret_cast<float2*>(m);
   const float2* Ob = reinterpret_cast<const float2*>(O);
@@ -2104,6 +2185,7 @@ int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, const pty_cfloat* 
 
 int pty_add_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, float scale, pty_cfloat* f) {
   if (!c || !O || !P || !f) return fail(PTY_EINVAL, "NULL argument");
+  SharedProbe shared(c);
   hipLaunchKernelGGL(k_add_exits, dim3(4096), dim3(WG), 0, c->stream, c->g,
                      reinterpret_cast<const float2*>(O), reinterpret_cast<const float2*>(P), scale,
                      reinterpret_cast<float2*>(f));
@@ -2198,6 +2280,7 @@ int pty_pmod1_noalloc(pty_cfloat* exits, const float* amp, const uint8_t* mask, 
 int pty_psup_O(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* P, pty_cfloat* O_out, double* H,
                int32_t fresh, float alpha) {
   if (!c || !ex || !P || !O_out || !H) return fail(PTY_EINVAL, "NULL argument");
+  SharedProbe shared(c);
   PTY_TRY(gather(c, reinterpret_cast<const float2*>(ex), reinterpret_cast<const float2*>(P), nullptr,
                  reinterpret_cast<float2*>(O_out), H, fresh ? 1 : 0, 1, alpha, false, false));
   PTY_HIP(hipStreamSynchronize(c->stream));
@@ -2221,6 +2304,7 @@ int pty_back_projection(pty_ctx* c, const pty_cfloat* P, pty_cfloat* exits_out, 
                         double* H, float alpha) {
   if (!c || !P || !O_out || !H) return fail(PTY_EINVAL, "NULL argument");
   if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
+  SharedProbe shared(c);
   const float2* Pd = reinterpret_cast<const float2*>(P);
   PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, Pd, c->Xp, c->M));
   ColArgs a = col_args(c);

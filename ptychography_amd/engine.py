@@ -46,7 +46,11 @@ class Engine:
     def __init__(self, N, M, K, U, V, R, device=None):
         self.device = require_gpu(device)
         self.N, self.M, self.K, self.U, self.V = int(N), int(M), int(K), int(U), int(V)
-        Rc = np.ascontiguousarray(np.asarray(R, dtype=np.int32).reshape(self.N, 2))
+        Ra = np.asarray(R)
+        if np.issubdtype(Ra.dtype, np.floating) and np.any(Ra != np.rint(Ra)):
+            raise ValueError("window positions must be integers; fractional parts go through "
+                             "set_subpixel")
+        Rc = np.ascontiguousarray(Ra.astype(np.int32).reshape(self.N, 2))
         self._R = Rc
         self._ctx = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -115,6 +119,18 @@ class Engine:
             pamp = pamp.to(self.device, torch.float32).contiguous()
             self._sync_torch()
         call("pty_set_pamp", self._ctx, _ptr(pamp))
+
+    def set_subpixel(self, d=None):
+        """Fractional position parts d (N x 2: (y, x) per frame, |d| <= 1):
+        frame n sees the probe Fourier-shifted by d[n]; None = integer positions."""
+        if d is None:
+            call("pty_set_subpixel", self._ctx, None)
+            return
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        if d.shape != (self.N, 2):
+            raise ValueError(f"sub-pixel shifts must have shape {(self.N, 2)}, got {d.shape}")
+        self._sync_torch()
+        call("pty_set_subpixel", self._ctx, d.ctypes.data_as(ctypes.c_void_p))
 
     # -- running ------------------------------------------------------------
     @staticmethod

@@ -115,9 +115,19 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
         raise ValueError("dtype must be None, 'single' or 'double'")
 
     if not isinstance(R, np.ndarray):
-        raise TypeError("R must be a numpy integer array (it is shifted in place, era.py:504-505)")
+        raise TypeError("R must be a numpy array (it is shifted in place, era.py:504-505)")
+    # float positions with fractional parts: integer windows plus per-frame
+    # Fourier-shifted probes (object-only methods, no far-field probe constraint)
+    frac = is_fractional(R)
+    if frac and method not in (1, 4):
+        raise ValueError("sub-pixel positions are modelled for the object-only methods 1 and 4")
+    if frac and pmod_probe_until(Pmod_probe) > 0:
+        raise ValueError("sub-pixel positions are modelled without the far-field probe constraint")
+    if frac and init == 'learned':
+        raise ValueError("init='learned' stitches patches at integer positions only")
     if O is None:
-        shape = (M + int(R[:, 0].max() - R[:, 0].min()), K + int(R[:, 1].max() - R[:, 1].min()))
+        shape = (M + int(np.ceil(R[:, 0].max() - R[:, 0].min())),
+                 K + int(np.ceil(R[:, 1].max() - R[:, 1].min())))
         O = np.ones(shape, dtype=c_dtype)
     if P is None:
         P = _random(rng, (M, K)) + 1J * _random(rng, (M, K))
@@ -131,7 +141,8 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
     R[:, 1] -= R[:, 1].max()
 
     U, V = O.shape
-    eng = Engine(N, M, K, U, V, R, device=dev)
+    R_win, d = split_positions(R) if frac else (R, None)
+    eng = Engine(N, M, K, U, V, R_win, device=dev)
     try:
         mask_c = None
         mask_s = None
@@ -161,6 +172,8 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
                                            P_t, O_t, alpha=alpha)
         del I_t
         eng.set_state(O_t, P_t)
+        if frac:
+            eng.set_subpixel(d)
 
         bg_init = None
         if method >= 4:
@@ -189,6 +202,21 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
                r_dtype=r_dtype, c_dtype=c_dtype, bg_init=bg_init, background=background,
                OP_iters=OP_iters, until=until, probe_centering=probe_centering, alpha=alpha,
                init_info=init_info, iters=iters, graphs=bool(graphs))
+
+
+def is_fractional(R):
+    """True for float positions with a non-integer entry."""
+    return np.issubdtype(R.dtype, np.floating) and bool(np.any(R != np.rint(R)))
+
+
+def split_positions(R):
+    """Windows and remainders of positions R <= 0 (after the in-place shift):
+    frame n's window starts at s = -R[n]; w = rint(s) is its integer window
+    (returned as the int32 position -w) and d = s - w in [-1/2, 1/2] the shift
+    of the probe it sees."""
+    s = -np.asarray(R, dtype=np.float64)
+    w = np.rint(s)
+    return (-w).astype(np.int32), s - w
 
 
 def _round_half_away(x):

@@ -45,3 +45,37 @@ def broadband_start(shape, seed=13, eps=0.1):
     """
     rng = np.random.default_rng(seed)
     return 1.0 + eps * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+
+
+def _freq_index(n):
+    """FFT bin frequencies in [-(n/2 - 1), n/2] (bagOfns.make_xy(origin=(0, 0)))."""
+    j = np.arange(n)
+    return ((j + n // 2 - 1) % n) - (n // 2 - 1)
+
+
+def _smooth(shape, band, rng):
+    F = np.zeros(shape, complex)
+    ky = _freq_index(shape[0])[:, None]
+    kx = _freq_index(shape[1])[None, :]
+    sel = (np.abs(ky) <= band) & (np.abs(kx) <= band)
+    F[sel] = rng.standard_normal(sel.sum()) + 1j * rng.standard_normal(sel.sum())
+    return np.fft.ifft2(F) * np.sqrt(F.size)
+
+
+def band_limited_scan(seed=0, M=32, N=12, U=56):
+    """Object band-limited to |k| <= 4 of its grid, a Gaussian probe with a
+    weak quadratic phase, fractional positions already shifted so their
+    maxima are 0 (the preamble leaves them unchanged), and the centred
+    intensities of the reference's object-shift exits (the oracle's
+    restatement of utility_Ptych.makeExits, the data-making checker)."""
+    from oracle import subpixel
+    rng = np.random.default_rng(seed)
+    O = 1 + 0.3 * _smooth((U, U), 4, rng)
+    y, x = np.mgrid[:M, :M]
+    P = np.exp(-((y - M / 2 + 0.3) ** 2 + (x - M / 2 - 0.7) ** 2) / (2 * 3.0 ** 2)) * \
+        np.exp(1j * 0.05 * (x - M / 2) ** 2)
+    R = -rng.uniform(0, U - M, size=(N, 2))
+    R -= R.max(axis=0)
+    ex = subpixel.make_exits_object_shift(O, P, R)
+    I = np.fft.fftshift(np.abs(np.fft.fft2(ex)) ** 2, axes=(-2, -1))
+    return O, P, R, I
