@@ -104,6 +104,20 @@ int pty_set_pamp(pty_ctx* ctx, const float* pamp_dev);
  * Synchronous. */
 int pty_set_subpixel(pty_ctx* ctx, const double* d_host);
 
+/* Per-frame position terms for refining the positions (the coordinate
+ * gradient of the reference's notebooks,
+ * ipython_notbooks/1dcoordsUpdate_1dsample_2dprobe.ipynb, for the probe-shift
+ * model): with S_n = FFT2(O_win P_n), D_n,a = FFT2(O_win dP_n / dd_n,a) (a = y,
+ * x), r = |S_n| - amp_n on the good pixels and J_a = Re(conj(S_n) D_n,a) /
+ * |S_n|, terms_host (N x 6 float64) receives per frame f_n = sum r^2 (the
+ * frame's share of the unnormalised eMod sum, era.py:560-572), the gradient
+ * 2 sum r J_y, 2 sum r J_x of f_n in the frame's shift d_n, and the
+ * Gauss-Newton matrix 2 sum J_y J_y, 2 sum J_y J_x, 2 sum J_x J_x.  Uses the
+ * bound O, P, data and the shifts of pty_set_subpixel (required; all zeros
+ * for integer positions).  Allocates two N x M x K derivative stacks on first
+ * use.  Synchronous. */
+int pty_position_terms(pty_ctx* ctx, double* terms_host);
+
 /* Run `iters` iterations with global iteration numbers iter0, iter0+1, ...
  * (they drive the OP_iters schedule and the Pmod_probe window).  begin = 1
  * starts a new solve (DM: f = make_exits(O, P), eCon reference = O/P;

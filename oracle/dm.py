@@ -59,7 +59,7 @@ def _project(update, i, OP_iters, exits, O, P, R, P_heatmap, O_heatmap, alpha,
 def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
        probe_centering=False, method=None, hardware='cpu', alpha=1.0e-10,
        dtype=None, full_output=True, callback=None, rng=None, resume=None,
-       state_callback=None):
+       state_callback=None, windows=None):
     """Difference map, one-rank restatement of dm.py:13-333.
 
     f_{i+1} = f_i - Ps f_i + Pm(2 Ps f_i - f_i)  (dm.py:94-97); metrics use
@@ -67,7 +67,7 @@ def DM(I, R, P, O, iters, OP_iters=1, mask=1, background=None, Pmod_probe=False,
     ``Pmod_probe`` / ``probe_centering`` / ``method`` differs from ERA's.
     """
     st = _era.preamble(I, R, P, O, iters, OP_iters, mask, background, method,
-                       alpha, dtype, rng)
+                       alpha, dtype, rng, windows)
     method, update, OP_iters = st['method'], st['update'], st['OP_iters']
     bg_mode = method in (4, 5, 6)
     i0, P_heatmap, O_heatmap, background = _era._resume(st, resume, bg_mode)

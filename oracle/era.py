@@ -265,11 +265,14 @@ def _random(rng, shape):
 
 
 def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype,
-             rng=None):
+             rng=None, windows=None):
     """Resolve method/update/dtype, default O and P, amplitudes, mask and R.
 
     ``R`` is shifted IN PLACE so that each column's maximum is 0 (era.py:504-505).
     Returns a dict with the same quantities the reference's tuple carries.
+    ``windows=(Rw, d)`` (test hook, for runs that continue a sub-pixel run
+    with moved positions) gives the integer windows and the per-frame shifts
+    directly, bypassing the shift and split of R.
     """
     if method is None:
         if O is None and P is None:
@@ -305,7 +308,7 @@ def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype
         raise ValueError("dtype must be None, 'single' or 'double'")
 
     from . import subpixel
-    frac = subpixel.is_fractional(R)
+    frac = subpixel.is_fractional(R) or windows is not None
     if frac and method not in (1, 4):
         raise ValueError("sub-pixel positions are modelled for the object-only methods 1 and 4")
     if O is None:
@@ -323,12 +326,16 @@ def preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha, dtype
     if mask.ndim > 0:
         mask = np.fft.ifftshift(mask)
 
-    R[:, 0] -= R[:, 0].max()
-    R[:, 1] -= R[:, 1].max()
+    if windows is None:
+        R[:, 0] -= R[:, 0].max()
+        R[:, 1] -= R[:, 1].max()
 
     # sub-pixel positions: integer windows and per-frame probes (oracle/subpixel.py)
     Pn = None
-    if frac:
+    if windows is not None:
+        R = np.asarray(windows[0], dtype=np.int64)
+        Pn = subpixel.frame_probes(P, np.asarray(windows[1], dtype=np.float64))
+    elif frac:
         R, d = subpixel.split_positions(R)
         Pn = subpixel.frame_probes(P, d)
     elif np.issubdtype(R.dtype, np.floating):
@@ -396,10 +403,10 @@ def _resume(st, resume, bg_mode):
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
         Pmod_probe=False, probe_centering=False, hardware='cpu', alpha=1.0e-10,
         dtype=None, full_output=True, callback=None, rng=None, resume=None,
-        state_callback=None):
+        state_callback=None, windows=None):
     """Error-reduction ptychography, one-rank restatement of era.py:13-333."""
     st = preamble(I, R, P, O, iters, OP_iters, mask, background, method, alpha,
-                  dtype, rng)
+                  dtype, rng, windows)
     method, update, OP_iters = st['method'], st['update'], st['OP_iters']
     bg_mode = method in (4, 5, 6)
     i0, P_heatmap, O_heatmap, background = _resume(st, resume, bg_mode)

@@ -46,7 +46,9 @@ def roll(x, shift):
     ky = freq_index(N0)[:, None]
     kx = freq_index(N1)[None, :]
     ramp = np.exp(-2.0j * np.pi * (shift[0] * ky / N0 + shift[1] * kx / N1))
-    return np.fft.ifftn(np.fft.fftn(x, axes=(-2, -1)) * ramp, axes=(-2, -1))
+    xh = np.fft.fftn(x, axes=(-2, -1))
+    # complex64 input: single-precision transforms, the ramp formed in double
+    return np.fft.ifftn(xh * ramp.astype(xh.dtype), axes=(-2, -1))
 
 
 def is_fractional(R):
@@ -72,3 +74,51 @@ def make_exits_object_shift(O, P, R):
     bagOfns.roll(O, R_n)[:M, :K] * P, R_n float (after the preamble's shift)."""
     M, K = P.shape
     return np.stack([roll(O, (float(r[0]), float(r[1])))[:M, :K] * P for r in R])
+
+
+def probe_derivatives(P, d):
+    """(N, 2, M, K): d P_n / d d_n along y and x, P_n = roll(P, d_n), i.e.
+    IFFT(FFT(P) ramp_n (-2 pi i f / N)) on each axis (the derivative of
+    bagOfns.roll's phase ramp, bagOfns.py:729-764)."""
+    M, K = P.shape
+    ky = freq_index(M)[:, None]
+    kx = freq_index(K)[None, :]
+    Ph = np.fft.fftn(P)
+    out = np.empty((len(d), 2) + P.shape, Ph.dtype)
+    for n, (dy, dx) in enumerate(d):
+        # the ramp and its derivative factors in double, rounded to P's precision
+        ramp = Ph * np.exp(-2.0j * np.pi * (dy * ky / M + dx * kx / K))
+        out[n, 0] = np.fft.ifftn((ramp * (-2.0j * np.pi * ky / M)).astype(Ph.dtype))
+        out[n, 1] = np.fft.ifftn((ramp * (-2.0j * np.pi * kx / K)).astype(Ph.dtype))
+    return out
+
+
+def position_terms(O, P, Rw, d, amp, mask):
+    """Per frame n, with psi_hat = FFT(O_win(Rw_n) P_n), r = |psi_hat| - amp
+    over the good pixels (amp and mask ifftshifted, as the preamble holds
+    them) and J_a = Re(conj(psi_hat) dpsi_hat / d d_a) / |psi_hat|:
+    (N, 6) = f_n = sum r^2, the gradient 2 sum r J_y, 2 sum r J_x and the
+    Gauss-Newton matrix 2 sum J_y J_y, 2 sum J_y J_x, 2 sum J_x J_x.  The
+    modulus error the reference minimises (era.py:560-572) as a function of
+    the frame positions, the coordinate-gradient of its notebooks
+    (ipython_notbooks/1dcoordsUpdate_1dsample_2dprobe.ipynb) for the
+    probe-shift model."""
+    M, K = P.shape
+    # complex64 inputs keep the transforms in single precision (the probe
+    # stacks are formed in double and rounded, as on the GPU)
+    cdt = np.result_type(O.dtype, P.dtype)
+    Pn = frame_probes(P, d).astype(cdt)
+    dP = probe_derivatives(P, d).astype(cdt)
+    mask = np.broadcast_to(np.asarray(mask, bool), (M, K))
+    out = np.zeros((len(d), 6))
+    for n in range(len(d)):
+        r0, r1 = int(Rw[n][0]), int(Rw[n][1])
+        Ow = O[-r0:M - r0, -r1:K - r1]
+        ps = np.fft.fftn(Ow * Pn[n]).astype(np.complex128)
+        a = np.abs(ps)
+        r = np.where(mask, a - amp[n], 0.0)
+        J = [np.where(mask, (ps.conj() * np.fft.fftn(Ow * dP[n, k])).real / np.maximum(a, 1e-300), 0.0)
+             for k in range(2)]
+        out[n] = [np.sum(r * r), 2 * np.sum(r * J[0]), 2 * np.sum(r * J[1]),
+                  2 * np.sum(J[0] * J[0]), 2 * np.sum(J[0] * J[1]), 2 * np.sum(J[1] * J[1])]
+    return out

@@ -1094,10 +1094,12 @@ __global__ __launch_bounds__(WG) void k_esup(Geom g, const float2* S, const floa
 // Pn[n, ky, kx] = Phat[ky, kx] exp(-2 pi i (d_n0 f(ky) / M + d_n1 f(kx) / K)):
 // the spectrum of the probe shifted by d_n (y[r] = x[r - d]), with the
 // frequency of bin k taken as ((k + N/2 - 1) mod N) - (N/2 - 1), i.e. in
-// [-(N/2 - 1), N/2] (bagOfns.py:729-764 with phase_ramp, :966-979)
+// [-(N/2 - 1), N/2] (bagOfns.py:729-764 with phase_ramp, :966-979).
+// deriv = 1 / 2: times -2 pi i f(ky) / M / -2 pi i f(kx) / K, the spectrum of
+// d Pn / d d_n0 / d d_n1.
 __global__ __launch_bounds__(WG) void k_subpixel_ramp(const float2* __restrict__ Phat,
                                                       const double2* __restrict__ d, float2* Pn,
-                                                      int N, int M, int K) {
+                                                      int N, int M, int K, int deriv) {
   const size_t MK = (size_t)M * K, total = MK * N;
   for (size_t idx = (size_t)blockIdx.x * WG + threadIdx.x; idx < total;
        idx += (size_t)gridDim.x * WG) {
@@ -1110,7 +1112,54 @@ __global__ __launch_bounds__(WG) void k_subpixel_ramp(const float2* __restrict__
     double sn, cs;
     sincos(ang, &sn, &cs);
     const float2 h = Phat[p];
-    Pn[idx] = make_float2((float)(h.x * cs - h.y * sn), (float)(h.x * sn + h.y * cs));
+    double re = h.x * cs - h.y * sn, im = h.x * sn + h.y * cs;
+    if (deriv) {
+      // times -i w: (re, im) -> (w im, -w re)
+      const double w = deriv == 1 ? 2.0 * M_PI * fy / M : 2.0 * M_PI * fx / K;
+      const double t = re;
+      re = w * im;
+      im = -w * t;
+    }
+    Pn[idx] = make_float2((float)re, (float)im);
+  }
+}
+
+// Position terms of frame n = blockIdx.x from its spectra S = FFT(O_win P_n)
+// and Dy / Dx = FFT(O_win dP_n/dd_n0 / dd_n1), amplitudes amp (ifftshifted)
+// and the ifftshifted mask: with r = |S| - amp on good pixels and
+// J_a = Re(conj(S) D_a) / |S|, out[n] = (sum r^2, 2 sum r J_y, 2 sum r J_x,
+// 2 sum J_y J_y, 2 sum J_y J_x, 2 sum J_x J_x) in FP64: the modulus error of
+// the frame, its gradient in the frame's shift and the Gauss-Newton matrix.
+__global__ __launch_bounds__(WG) void k_position_terms(const float2* __restrict__ S,
+                                                       const float2* __restrict__ Dy,
+                                                       const float2* __restrict__ Dx,
+                                                       const float* __restrict__ amp,
+                                                       const uint8_t* __restrict__ mask, int MK,
+                                                       double* __restrict__ out) {
+  __shared__ double red[4 * 6];
+  const size_t base = (size_t)blockIdx.x * MK;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int p = threadIdx.x; p < MK; p += WG) {
+    if (mask && !mask[p]) continue;
+    const float2 s = S[base + p], dy = Dy[base + p], dx = Dx[base + p];
+    const double m = sqrt((double)s.x * s.x + (double)s.y * s.y);
+    const double r = m - (double)amp[base + p];
+    double jy = 0.0, jx = 0.0;
+    if (m > 0.0) {
+      jy = ((double)s.x * dy.x + (double)s.y * dy.y) / m;
+      jx = ((double)s.x * dx.x + (double)s.y * dx.y) / m;
+    }
+    v[0] += r * r;
+    v[1] += r * jy;
+    v[2] += r * jx;
+    v[3] += jy * jy;
+    v[4] += jy * jx;
+    v[5] += jx * jx;
+  }
+  block_sum<6>(v, red);
+  if (threadIdx.x == 0) {
+    out[6 * blockIdx.x] = v[0];
+    for (int k = 1; k < 6; ++k) out[6 * blockIdx.x + k] = 2.0 * v[k];
   }
 }
 

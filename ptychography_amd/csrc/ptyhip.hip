@@ -190,6 +190,9 @@ struct pty_ctx {
   float2* Pn = nullptr;      // N*M*K per-frame probes
   bool sub_set = false;      // shifts given: runs use the per-frame probes
   bool Pn_valid = false;     // Pn holds the shifts of the bound P
+  float2* Dy = nullptr;      // N*M*K position-derivative stacks / their spectra
+  float2* Dx = nullptr;
+  double* pos_terms = nullptr;  // N x 6 per-frame position terms
 
   // hipGraph replay of whole iterations
   struct GraphEntry {
@@ -1160,6 +1163,21 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
 }
 
 // validation and per-run set-up of pty_run (before any iteration)
+// dst[n] = IFFT(FFT(P) ramp(d_n)) (deriv 0) or its derivative in d_n0 / d_n1
+// (deriv 1 / 2), from the spectrum of P in Xp
+int probe_stack(pty_ctx* c, float2* dst, int deriv) {
+  hipLaunchKernelGGL(k_subpixel_ramp, dim3(2048), dim3(WG), 0, c->stream, (const float2*)c->Xp,
+                     (const double2*)c->sub_d, dst, c->N, c->M, c->K, deriv);
+  PTY_LAUNCHED("k_subpixel_ramp");
+  ColArgs b = col_args(c);
+  b.X = dst;
+  b.nframes = c->N;
+  b.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+  PTY_TRY(cols<COL_INV>(c, b));
+  PTY_TRY(rows_inv<DST_STORE>(c, dst, dst, nullptr, nullptr, 1.0f, nullptr, c->N * c->M));
+  return PTY_OK;
+}
+
 // Pn[n] = IFFT(FFT(P) ramp(d_n)): the forward transform of P into Xp, one
 // phase ramp per frame, then the batched inverse transform of the stack
 int build_subpixel_probes(pty_ctx* c) {
@@ -1172,15 +1190,7 @@ int build_subpixel_probes(pty_ctx* c) {
   a.nframes = 1;
   a.chunk = 1;
   PTY_TRY(cols<COL_FWD>(c, a));
-  hipLaunchKernelGGL(k_subpixel_ramp, dim3(2048), dim3(WG), 0, c->stream, (const float2*)c->Xp,
-                     (const double2*)c->sub_d, c->Pn, c->N, c->M, c->K);
-  PTY_LAUNCHED("k_subpixel_ramp");
-  ColArgs b = col_args(c);
-  b.X = c->Pn;
-  b.nframes = c->N;
-  b.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
-  PTY_TRY(cols<COL_INV>(c, b));
-  PTY_TRY(rows_inv<DST_STORE>(c, c->Pn, c->Pn, nullptr, nullptr, 1.0f, nullptr, c->N * c->M));
+  PTY_TRY(probe_stack(c, c->Pn, 0));
   c->g.Pn = c->Pn;
   c->Pn_valid = true;
   c->H_valid = false;   // the heatmap is the sum of the per-frame |Pn|^2
@@ -1545,7 +1555,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->H); dev_free(c->G); dev_free(c->bg); dev_free(c->bbar); dev_free(c->bbar_m);
   dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->pnum); dev_free(c->pg);
   dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
-  dev_free(c->sub_d); dev_free(c->Pn);
+  dev_free(c->sub_d); dev_free(c->Pn); dev_free(c->Dy); dev_free(c->Dx); dev_free(c->pos_terms);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1605,6 +1615,54 @@ int pty_set_state(pty_ctx* c, pty_cfloat* O, pty_cfloat* P) {
   c->H_valid = c->G_valid = false;
   c->Pn_valid = false;
   c->g.Pn = nullptr;
+  return PTY_OK;
+}
+
+int pty_position_terms(pty_ctx* c, double* terms_host) {
+  if (!c || !terms_host) return fail(PTY_EINVAL, "NULL argument");
+  if (!c->sub_set) return fail(PTY_ESTATE, "pty_set_subpixel has not been called");
+  if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
+  if (!c->O || !c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
+  if (!c->Dy) PTY_TRY(dev_alloc(&c->Dy, c->NMK()));
+  if (!c->Dx) PTY_TRY(dev_alloc(&c->Dx, c->NMK()));
+  if (!c->pos_terms) PTY_TRY(dev_alloc(&c->pos_terms, 6 * (size_t)c->N));
+  if (!c->Pn_valid) PTY_TRY(build_subpixel_probes(c));   // leaves FFT(P) in Xp
+  else {
+    PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->P, c->Xp, c->M));
+    ColArgs a = col_args(c);
+    a.X = c->Xp;
+    a.nframes = 1;
+    a.chunk = 1;
+    PTY_TRY(cols<COL_FWD>(c, a));
+  }
+  PTY_TRY(probe_stack(c, c->Dy, 1));
+  PTY_TRY(probe_stack(c, c->Dx, 2));
+  // spectra of O_win P_n, O_win dP_n/dd_y, O_win dP_n/dd_x: the derivative
+  // stacks are transformed in place (each frame row reads its own probe row
+  // before the same line writes it)
+  ColArgs a = col_args(c);
+  a.nframes = c->N;
+  a.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+  const int nrows = c->N * c->M;
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
+  a.X = c->X;
+  PTY_TRY(cols<COL_FWD>(c, a));
+  for (float2* D : {c->Dy, c->Dx}) {
+    c->g.Pn = D;
+    const int r = rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, D, nrows);
+    c->g.Pn = c->Pn;
+    PTY_TRY(r);
+    a.X = D;
+    PTY_TRY(cols<COL_FWD>(c, a));
+  }
+  hipLaunchKernelGGL(k_position_terms, dim3(c->N), dim3(WG), 0, c->stream, (const float2*)c->X,
+                     (const float2*)c->Dy, (const float2*)c->Dx, c->amp, c->mask, (int)c->MK(),
+                     c->pos_terms);
+  PTY_LAUNCHED("k_position_terms");
+  PTY_HIP(hipMemcpyAsync(terms_host, c->pos_terms, 6 * (size_t)c->N * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  c->os_next = false;   // X no longer holds what a DM projection reuse expects
   return PTY_OK;
 }
 

@@ -132,6 +132,14 @@ class Engine:
         self._sync_torch()
         call("pty_set_subpixel", self._ctx, d.ctypes.data_as(ctypes.c_void_p))
 
+    def position_terms(self):
+        """(N, 6) float64 per frame: modulus error f_n, its gradient in the
+        frame's shift (y, x) and the Gauss-Newton matrix (yy, yx, xx)."""
+        out = np.zeros((self.N, 6), dtype=np.float64)
+        self._sync_torch()
+        call("pty_position_terms", self._ctx, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
     # -- running ------------------------------------------------------------
     @staticmethod
     def opts(algorithm, method, OP_iters=(1, 1), Pmod_probe_until=0, probe_centering=False,

@@ -84,7 +84,7 @@ class Job:
 
 def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
             Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None, device=None,
-            rng=None, init=None, init_net=None, graphs=False):
+            rng=None, init=None, init_net=None, graphs=False, refine_positions=0):
     """Host preamble of ERA / DM (era.py:435-548) on a new engine; returns a ``Job``."""
     dev = require_gpu(device)
     if algorithm not in (ALGO_ERA, ALGO_DM):
@@ -118,13 +118,18 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
         raise TypeError("R must be a numpy array (it is shifted in place, era.py:504-505)")
     # float positions with fractional parts: integer windows plus per-frame
     # Fourier-shifted probes (object-only methods, no far-field probe constraint)
-    frac = is_fractional(R)
+    refine_positions = int(refine_positions or 0)
+    if refine_positions < 0:
+        raise ValueError("refine_positions must be >= 0")
+    # refining positions runs the sub-pixel model (from zero shifts for integer R)
+    frac = is_fractional(R) or refine_positions > 0
+    what = "position refinement" if refine_positions else "sub-pixel positions"
     if frac and method not in (1, 4):
-        raise ValueError("sub-pixel positions are modelled for the object-only methods 1 and 4")
+        raise ValueError(f"{what}: modelled for the object-only methods 1 and 4")
     if frac and pmod_probe_until(Pmod_probe) > 0:
-        raise ValueError("sub-pixel positions are modelled without the far-field probe constraint")
+        raise ValueError(f"{what}: modelled without the far-field probe constraint")
     if frac and init == 'learned':
-        raise ValueError("init='learned' stitches patches at integer positions only")
+        raise ValueError(f"{what}: init='learned' stitches patches at integer positions only")
     if O is None:
         shape = (M + int(np.ceil(R[:, 0].max() - R[:, 0].min())),
                  K + int(np.ceil(R[:, 1].max() - R[:, 1].min())))
@@ -201,12 +206,34 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
     return Job(algorithm=algorithm, eng=eng, O_t=O_t, P_t=P_t, method=method, update=update,
                r_dtype=r_dtype, c_dtype=c_dtype, bg_init=bg_init, background=background,
                OP_iters=OP_iters, until=until, probe_centering=probe_centering, alpha=alpha,
-               init_info=init_info, iters=iters, graphs=bool(graphs))
+               init_info=init_info, iters=iters, graphs=bool(graphs), R_win=R_win, d=d,
+               refine=refine_positions)
 
 
 def is_fractional(R):
     """True for float positions with a non-integer entry."""
     return np.issubdtype(R.dtype, np.floating) and bool(np.any(R != np.rint(R)))
+
+
+def position_step(terms, d, damping=1e-3, max_step=0.5, max_shift=1.0):
+    """One Gauss-Newton step on every frame's shift from its position terms
+    (``Engine.position_terms``: f, gradient (y, x), matrix (yy, yx, xx)):
+    delta = -(H + damping tr(H) I)^-1 g, each component clipped to
+    +-max_step, the new shift to +-max_shift.  With the object fixed the
+    modulus error is a sum of per-frame terms, so the frames step
+    independently."""
+    T = np.asarray(terms, dtype=np.float64)
+    g = T[:, 1:3]
+    H = np.empty((len(T), 2, 2))
+    H[:, 0, 0], H[:, 0, 1], H[:, 1, 0], H[:, 1, 1] = T[:, 3], T[:, 4], T[:, 4], T[:, 5]
+    tr = T[:, 3] + T[:, 5]
+    ok = tr > 0
+    delta = np.zeros_like(g)
+    if np.any(ok):
+        A = H[ok] + (damping * tr[ok])[:, None, None] * np.eye(2)
+        delta[ok] = -np.linalg.solve(A, g[ok][..., None])[..., 0]
+    delta = np.clip(delta, -max_step, max_step)
+    return np.clip(np.asarray(d, dtype=np.float64) + delta, -max_shift, max_shift)
 
 
 def split_positions(R):
@@ -296,6 +323,8 @@ def finish(job, metrics, full_output=True, check_finite=True):
         info['eSup'] = [float(x) for x in metrics[:, 2]]
         H = eng.heatmap()
         info['heatmap'] = None if H is None else H.cpu().numpy().astype(r_dtype)
+        if job.refine:
+            info['R'] = np.asarray(job.R_win, np.float64) - job.d
         background = job.background
         if job.method >= 4:
             if job.iters > 0:
@@ -319,38 +348,54 @@ def finish(job, metrics, full_output=True, check_finite=True):
 def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None,
           Pmod_probe=False, probe_centering=False, alpha=1.0e-10, dtype=None,
           full_output=True, device=None, callback=None, rng=None, init=None, init_net=None,
-          graphs=False, check_finite=True, progress=False):
+          graphs=False, check_finite=True, progress=False, refine_positions=0):
     """Shared body of ERA and DM: host preamble (era.py:435-548), device loop, returns.
 
     ``init='learned'`` (with ``init_net``, a trained ``learned.InitNet``) replaces
     the starting object by the network's stitched estimate when its
     first-iteration eMod beats that of the ``O`` passed in; ``info['init']``
     records both eMods, the choice and the time spent (or, for frames of
-    another size than the network's, that the plain start was kept and why)."""
+    another size than the network's, that the plain start was kept and why).
+
+    ``refine_positions=k`` (object-only methods 1 and 4) refines the frame
+    positions: after every k iterations each frame's shift takes one
+    Gauss-Newton step on its modulus error with the object fixed
+    (``position_step``), within +-1 pixel of the window its starting position
+    rounds to; ``info['R']`` returns the refined positions (same convention
+    as the shifted ``R``)."""
     job = prepare(algorithm, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
                   background=background, method=method, Pmod_probe=Pmod_probe,
                   probe_centering=probe_centering, alpha=alpha, dtype=dtype, device=device,
-                  rng=rng, init=init, init_net=init_net, graphs=graphs)
+                  rng=rng, init=init, init_net=init_net, graphs=graphs,
+                  refine_positions=refine_positions)
     try:
         eng, kw = job.eng, job.run_kwargs()
         name = 'ERA' if algorithm == ALGO_ERA else 'DM'
         if progress:
             print('algrithm progress iteration convergence modulus error')   # era.py:151
-        if callback is None and not progress:
-            metrics = eng.run(algorithm, job.method, iters, iter0=0, begin=True, **kw)
-        else:
-            rows = []
-            for i in range(iters):
-                rows.append(eng.run(algorithm, job.method, 1, iter0=i, begin=(i == 0), **kw))
-                if progress:
-                    update_progress(i / max(1.0, float(iters - 1)), name, i, float(rows[-1][0, 1]),
-                                    float(rows[-1][0, 0]))
-                if callback is not None:
-                    callback(i, job.O_t.cpu().numpy().astype(job.c_dtype),
-                             job.P_t.cpu().numpy().astype(job.c_dtype))
-            metrics = np.concatenate(rows, axis=0) if rows else np.zeros((0, 3))
-            if iters == 0:
-                eng.run(algorithm, job.method, 0, iter0=0, begin=True, **kw)
+        per_iter = callback is not None or progress
+        rows = []
+        i = 0
+        while i < iters:
+            n = iters - i
+            if job.refine:
+                n = min(n, job.refine - i % job.refine)
+            if per_iter:
+                n = 1
+            rows.append(eng.run(algorithm, job.method, n, iter0=i, begin=(i == 0), **kw))
+            if progress:
+                update_progress(i / max(1.0, float(iters - 1)), name, i, float(rows[-1][0, 1]),
+                                float(rows[-1][0, 0]))
+            if callback is not None:
+                callback(i, job.O_t.cpu().numpy().astype(job.c_dtype),
+                         job.P_t.cpu().numpy().astype(job.c_dtype))
+            i += n
+            if job.refine and i % job.refine == 0 and i < iters:
+                job.d = position_step(eng.position_terms(), job.d)
+                eng.set_subpixel(job.d)
+        metrics = np.concatenate(rows, axis=0) if rows else np.zeros((0, 3))
+        if iters == 0:
+            eng.run(algorithm, job.method, 0, iter0=0, begin=True, **kw)
     except BaseException:
         job.eng.close()
         raise
@@ -360,7 +405,7 @@ def solve(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, met
 def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmod_probe=False,
         probe_centering=False, hardware='cpu', alpha=1.0e-10, dtype=None, full_output=True,
         device=None, callback=None, rng=None, init=None, init_net=None, graphs=False,
-        check_finite=True, progress=False):
+        check_finite=True, progress=False, refine_positions=0):
     """Error-reduction algorithm (ptychography/era.py:13).  ``hardware`` is accepted and
     ignored, as in the reference; the work always runs on the GPU ``device``.
 
@@ -374,11 +419,13 @@ def ERA(I, R, P, O, iters, OP_iters=1, mask=1, background=None, method=None, Pmo
     ``check_finite`` (default True) to raise ``FloatingPointError`` when eMod
     becomes NaN / inf instead of returning NaNs, ``progress=True`` to print
     the reference's header and per-iteration progress bar (``update_progress``;
-    the reference always prints them, era.py:151,212).
+    the reference always prints them, era.py:151,212), ``refine_positions=k``
+    to refine the positions every k iterations (``solve``).  ``R`` may hold
+    float positions (sub-pixel model, methods 1 and 4).
     """
     return solve(ALGO_ERA, I, R, P, O, iters, OP_iters=OP_iters, mask=mask,
                  background=background, method=method, Pmod_probe=Pmod_probe,
                  probe_centering=probe_centering, alpha=alpha, dtype=dtype,
                  full_output=full_output, device=device, callback=callback, rng=rng,
                  init=init, init_net=init_net, graphs=graphs, check_finite=check_finite,
-                 progress=progress)
+                 progress=progress, refine_positions=refine_positions)

@@ -79,3 +79,27 @@ def band_limited_scan(seed=0, M=32, N=12, U=56):
     ex = subpixel.make_exits_object_shift(O, P, R)
     I = np.fft.fftshift(np.abs(np.fft.fft2(ex)) ** 2, axes=(-2, -1))
     return O, P, R, I
+
+
+def raster_subpixel_scan(seed=0, M=32, G=6, step=9.0, band=6, sigma=5.0):
+    """G x G raster of fractional positions (step plus a uniform [0, 1)
+    jitter), an object band-limited to |k| <= band, a Gaussian probe of width
+    sigma with a weak quadratic phase, and the centred intensities of the
+    probe-shift model (the oracle's make_exits with per-frame probes).
+    Returns (O, P, R, I) with R already shifted so its maxima are 0."""
+    from oracle import make_exits, subpixel
+    rng = np.random.default_rng(seed)
+    N = G * G
+    yy, xx = np.mgrid[:G, :G]
+    R = -np.stack([yy.ravel() * step, xx.ravel() * step], 1) - rng.uniform(0, 1, (N, 2))
+    R -= R.max(axis=0)
+    U = M + int(np.ceil(np.ptp(R[:, 0])))
+    V = M + int(np.ceil(np.ptp(R[:, 1])))
+    O = 1 + 0.4 * _smooth((U, V), band, rng)
+    y, x = np.mgrid[:M, :M]
+    P = (np.exp(-((y - M / 2) ** 2 + (x - M / 2) ** 2) / (2 * sigma ** 2)) *
+         np.exp(1j * 0.03 * ((x - M / 2) ** 2 + (y - M / 2) ** 2)))
+    Rw, d = subpixel.split_positions(R)
+    ex = make_exits(O, subpixel.frame_probes(P, d), Rw)
+    I = np.fft.fftshift(np.abs(np.fft.fft2(ex)) ** 2, axes=(-2, -1))
+    return O, P, R, I

@@ -186,3 +186,104 @@ def test_engine_rejects_fractional_windows_and_bad_shifts(gpu_device):
         eng.run(pera.ALGO_ERA, 3, 1, iter0=0, begin=True)
     finally:
         eng.close()
+
+
+# ---- position refinement ----------------------------------------------------
+
+def _rms(Rest, Rtrue):
+    s, t = -np.asarray(Rest, float), -np.asarray(Rtrue, float)
+    err = (s - s.mean(0)) - (t - t.mean(0))
+    return float(np.sqrt((err ** 2).mean()))
+
+
+@pytest.mark.parametrize("size", [dict(M=32, G=6, step=9.0), dict(M=64, G=8, step=18.0)])
+def test_position_terms_match_the_oracle(size, gpu_device):
+    """GPU per-frame terms (f, gradient, Gauss-Newton matrix) at shifts 0.2
+    pixel off the truth, against the complex128 oracle's: relative L2 over
+    the frames, per column, <= max(1e-4, 10 x the error of the same oracle
+    run in complex64).  Near the fit, r = |S| - amp is small against |S|, so
+    f and the gradient lose digits in single precision on both sides."""
+    from tests.scans import raster_subpixel_scan
+    O, P, R, I = raster_subpixel_scan(seed=31, **size)
+    N, M, K = I.shape
+    Rw, d = sp.split_positions(R)
+    rng = np.random.default_rng(31)
+    d0 = d + rng.uniform(-0.2, 0.2, d.shape)
+    amp = np.sqrt(np.fft.ifftshift(I, axes=(-2, -1)))
+    ref = sp.position_terms(O, P, Rw, d0, amp, True)
+    ref32 = sp.position_terms(O.astype(np.complex64), P.astype(np.complex64), Rw, d0,
+                              amp.astype(np.float32), True)
+    eng = Engine(N, M, K, O.shape[0], O.shape[1], Rw, device=gpu_device)
+    try:
+        a, I_norm = eng.prepare_amp(torch.from_numpy(I.astype(np.float32)), None)
+        eng.set_data(a, None, I_norm)
+        O_t = torch.from_numpy(O.astype(np.complex64)).to(gpu_device)
+        P_t = torch.from_numpy(P.astype(np.complex64)).to(gpu_device)
+        eng.set_state(O_t, P_t)
+        eng.set_subpixel(d0)
+        got = eng.position_terms()
+    finally:
+        eng.close()
+    errs = [rel_l2(got[:, k], ref[:, k]) for k in range(6)]
+    bound = [max(1e-4, 10 * rel_l2(ref32[:, k], ref[:, k])) for k in range(6)]
+    print("position terms rel L2 (f, gy, gx, hyy, hyx, hxx):", " ".join(f"{e:.1e}" for e in errs))
+    print("bound                                          :", " ".join(f"{e:.1e}" for e in bound))
+    assert all(e <= b for e, b in zip(errs, bound))
+
+
+def test_refinement_matches_the_oracle_loop(gpu_device):
+    """ERA method 1 with a position step every 5 iterations, 20 iterations,
+    from the same broadband object: the refined positions within 2e-3 pixel
+    of the oracle's loop (float32 against float64 objects feed each step),
+    eMod per iteration within 1e-3 relative."""
+    from tests.scans import raster_subpixel_scan
+    from tests.test_oracle_subpixel import refine_with_oracle
+    O, P, R, I = raster_subpixel_scan(seed=32)
+    rng = np.random.default_rng(32)
+    R0 = R + rng.uniform(-0.4, 0.4, R.shape)
+    R0 -= R0.max(axis=0)
+    U = P.shape[0] + int(np.ceil(np.ptp(R0[:, 0])))
+    V = P.shape[1] + int(np.ceil(np.ptp(R0[:, 1])))
+    O0 = broadband_start((U, V))
+    _, Rr, e_o = refine_with_oracle(I, R0, P, 20, 5, O=O0)
+    _, info = pty.ERA(I, R0.copy(), P.copy(), O0.copy(), 20, method=1, refine_positions=5,
+                      device=gpu_device)
+    dmax = np.abs(info["R"] - Rr).max()
+    e_g = np.array(info["eMod"])
+    print(f"refined positions: max |gpu - oracle| {dmax:.2e} px; eMod rel "
+          f"{np.abs(e_g - e_o).max() / min(e_o):.2e}")
+    assert dmax < 2e-3
+    assert np.allclose(e_g, e_o, rtol=1e-3)
+
+
+def test_refinement_improves_positions(gpu_device):
+    """256 frames of 64^2 (step 10 pixels, probe width 9: each object pixel
+    seen by many frames) from positions perturbed by up to 0.4 pixel:
+    refining every 5 iterations over 150 halves the position error and the
+    final eMod against the run that keeps the perturbed positions (the
+    oracle's loop: 0.224 -> 0.074 pixel, eMod 3.7e-4 vs 1.9e-3).  With little
+    overlap (step 18) the object absorbs position errors and the positions
+    are not observable: the error then stays near its start."""
+    from tests.scans import raster_subpixel_scan
+    O, P, R, I = raster_subpixel_scan(seed=33, M=64, G=16, step=10.0, band=10, sigma=9.0)
+    rng = np.random.default_rng(33)
+    R0 = R + rng.uniform(-0.4, 0.4, R.shape)
+    R0 -= R0.max(axis=0)
+    U = P.shape[0] + int(np.ceil(np.ptp(R0[:, 0])))
+    V = P.shape[1] + int(np.ceil(np.ptp(R0[:, 1])))
+    O0 = np.ones((U, V), np.complex128)
+    _, info_r = pty.ERA(I, R0.copy(), P.copy(), O0.copy(), 150, method=1, refine_positions=5,
+                        device=gpu_device)
+    _, info_n = pty.ERA(I, R0.copy(), P.copy(), O0.copy(), 150, method=1, device=gpu_device)
+    r0, r1 = _rms(R0, R), _rms(info_r["R"], R)
+    print(f"position rms {r0:.3f} -> {r1:.3f} px; final eMod {info_n['eMod'][-1]:.2e} "
+          f"(fixed positions) vs {info_r['eMod'][-1]:.2e} (refined)")
+    assert r1 < 0.5 * r0
+    assert info_r["eMod"][-1] < 0.5 * info_n["eMod"][-1]
+
+
+def test_refinement_rejects_probe_methods(gpu_device):
+    from tests.scans import raster_subpixel_scan
+    O, P, R, I = raster_subpixel_scan(seed=34)
+    with pytest.raises(ValueError):
+        pty.ERA(I, R.copy(), P.copy(), None, 2, method=3, refine_positions=2, device=gpu_device)

@@ -26,7 +26,7 @@ import pytest
 import oracle
 from oracle import subpixel as sp
 from oracle.era import make_exits
-from tests.scans import band_limited_scan
+from tests.scans import band_limited_scan, raster_subpixel_scan
 
 
 def test_freq_index_numbers_bins_as_make_xy():
@@ -156,3 +156,111 @@ def test_probe_updates_are_rejected(kw):
     O, P, R, I = band_limited_scan(seed=7, M=16, N=4, U=28)
     with pytest.raises(ValueError):
         oracle.ERA(I, R.copy(), P.copy(), np.ones_like(O), 1, **kw)
+
+
+# ---- position refinement ----------------------------------------------------
+
+def _perturbed_state(seed=12, M=32, N=10, U=64):
+    O, P, R, I = band_limited_scan(seed=seed, M=M, N=N, U=U)
+    Rw, d = sp.split_positions(R)
+    rng = np.random.default_rng(seed)
+    d0 = d + rng.uniform(-0.2, 0.2, d.shape)
+    amp = np.sqrt(np.fft.ifftshift(I, axes=(-2, -1)))
+    return O, P, Rw, d, d0, amp
+
+
+def test_position_terms_match_finite_differences():
+    """The gradient against central differences of f_n, the Gauss-Newton
+    matrix against the Jacobian of |psi_hat| by central differences."""
+    O, P, Rw, d, d0, amp = _perturbed_state()
+    T = sp.position_terms(O, P, Rw, d0, amp, True)
+    h = 1e-5
+
+    def mag(dd):
+        ex = make_exits(O, sp.frame_probes(P, dd), Rw)
+        return np.abs(np.fft.fftn(ex, axes=(-2, -1)))
+
+    fd_g, J = [], []
+    for a in range(2):
+        e = np.zeros_like(d0)
+        e[:, a] = h
+        fp = sp.position_terms(O, P, Rw, d0 + e, amp, True)[:, 0]
+        fm = sp.position_terms(O, P, Rw, d0 - e, amp, True)[:, 0]
+        fd_g.append((fp - fm) / (2 * h))
+        J.append((mag(d0 + e) - mag(d0 - e)) / (2 * h))
+    fd_g = np.stack(fd_g, 1)
+    assert np.abs(T[:, 1:3] - fd_g).max() < 1e-6 * np.abs(fd_g).max()
+    H = np.stack([2 * np.sum(J[0] * J[0], axis=(1, 2)), 2 * np.sum(J[0] * J[1], axis=(1, 2)),
+                  2 * np.sum(J[1] * J[1], axis=(1, 2))], 1)
+    assert np.abs(T[:, 3:] - H).max() < 1e-6 * np.abs(H).max()
+    # f_n is the frame's share of the unnormalised modulus error
+    ex = make_exits(O, sp.frame_probes(P, d0), Rw)
+    f = np.sum((np.abs(np.fft.fftn(ex, axes=(-2, -1))) - amp) ** 2, axis=(1, 2))
+    assert np.allclose(T[:, 0], f, rtol=1e-12)
+
+
+def test_position_terms_planted_sign_mistake():
+    """PLANTED: the derivative probes of the opposite shift give a gradient
+    that disagrees with the finite differences."""
+    O, P, Rw, d, d0, amp = _perturbed_state(seed=13)
+    right = sp.probe_derivatives
+    T_ok = sp.position_terms(O, P, Rw, d0, amp, True)
+    try:
+        sp.probe_derivatives = lambda P, d: -right(P, d)
+        T_bad = sp.position_terms(O, P, Rw, d0, amp, True)
+    finally:
+        sp.probe_derivatives = right
+    assert np.all(np.sign(T_bad[:, 1:3]) == -np.sign(T_ok[:, 1:3]))
+
+
+def test_position_step_moves_towards_the_true_shifts():
+    from ptychography_amd.era import position_step
+    O, P, Rw, d, d0, amp = _perturbed_state(seed=14)
+    T = sp.position_terms(O, P, Rw, d0, amp, True)
+    d1 = position_step(T, d0)
+    f1 = sp.position_terms(O, P, Rw, d1, amp, True)[:, 0]
+    assert np.all(f1 < T[:, 0])
+    assert np.abs(d1 - d).max() < 0.3 * np.abs(d0 - d).max()
+
+
+def refine_with_oracle(I, R, P, iters, every, method=1, background=None, O=None):
+    """ERA with position refinement, on the oracle: runs of `every` iterations,
+    then one position step (the facade's ``position_step``) on the oracle's
+    position terms.  Returns the object, the positions and the eMod list."""
+    from ptychography_amd.era import position_step
+    Rs = np.asarray(R, np.float64).copy()
+    Rs -= Rs.max(axis=0)
+    Rw, d = sp.split_positions(Rs)
+    U = P.shape[0] + int(np.ceil(np.ptp(Rs[:, 0])))
+    V = P.shape[1] + int(np.ceil(np.ptp(Rs[:, 1])))
+    O = np.ones((U, V), complex) if O is None else np.array(O, complex)
+    amp = np.sqrt(np.fft.ifftshift(I, axes=(-2, -1)))
+    e, i = [], 0
+    while i < iters:
+        n = min(every - i % every, iters - i)
+        out = oracle.ERA(I, Rs.copy(), P.copy(), O, n, method=method, background=background,
+                         windows=(Rw, d))
+        O = out[0]
+        e += out[-1]["eMod"]
+        i += n
+        if i % every == 0 and i < iters:
+            d = position_step(sp.position_terms(O, P, Rw, d, amp, True), d)
+    return O, Rw - d, e
+
+
+def test_refinement_recovers_perturbed_positions():
+    O, P, R, I = raster_subpixel_scan(seed=15)
+    rng = np.random.default_rng(15)
+    R0 = R + rng.uniform(-0.4, 0.4, R.shape)
+
+    def rms(Rest):
+        s, t = -Rest, -R
+        err = (s - s.mean(0)) - (t - t.mean(0))
+        return float(np.sqrt((err ** 2).mean()))
+
+    _, Rr, e_r = refine_with_oracle(I, R0, P, 100, 5)
+    Rn = R0 - R0.max(axis=0)
+    _, e_n = oracle.ERA(I, Rn.copy(), P.copy(), np.ones_like(O), 100, method=1)
+    e_n = e_n["eMod"]
+    assert rms(Rr) < 0.5 * rms(Rn)
+    assert e_r[-1] < 0.5 * e_n[-1]
