@@ -289,6 +289,10 @@ enum ColMode {
   COL_DMB_UPD,    // DM background step: b0 = 2 bbar - b_n, Pmod_7, b_n += b0' - bbar
   COL_FWD,        // plain forward column FFT (2-D FFT building block)
   COL_INV,        // plain inverse column FFT (input in spectrum order), times scale
+  // DM with background, iterate carried as spectra F = FFT2(f) (see k_cols):
+  COL_DMBF_LEAD,  // F = spectrum of the input; Xo = col-IFFT(Pmod_7(F, b)); sums b'
+  COL_DMBF,       // one DM step on (F, b) from the spectra of O P; X = col-IFFT(F_new),
+                  // Xo = col-IFFT(Pmod_7(F_new, b_new)); sums b_new and b'
 };
 
 struct ColArgs {
@@ -306,6 +310,9 @@ struct ColArgs {
   double* part;          // [gridDim.y * gridDim.x] eMod / error partials
   double* part2;         // [gridDim.y * gridDim.x] sum |psi_hat'|^2 partials
   double* partb;         // [gridDim.y][M*K] background sums over the chunk's frames
+  double* partb2;        // COL_DMBF: second set of background sums
+  float2* F;             // COL_DMBF(_LEAD): the DM iterate's spectra [N][M][K]
+  float2* Xo;            // COL_DMBF(_LEAD): second inverse output [N][M][K]
   int nframes, chunk;
   const float2* __restrict__ tw;
 };
@@ -314,15 +321,19 @@ template <int MODE> struct ColTraits {
   static constexpr bool fwd = MODE != COL_INV;
   static constexpr bool inv = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP ||
                               MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
-                              MODE == COL_DMB_UPD || MODE == COL_INV;
+                              MODE == COL_DMB_UPD || MODE == COL_INV || MODE == COL_DMBF_LEAD ||
+                              MODE == COL_DMBF;
   static constexpr bool store_fwd = MODE == COL_FWD;
-  static constexpr bool bsum = MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD;
+  static constexpr bool bsum = MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
+                               MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
+  static constexpr bool bsum2 = MODE == COL_DMBF;
   static constexpr bool err = MODE == COL_ERA1 || MODE == COL_ERR1 || MODE == COL_ERR7 ||
-                              MODE == COL_ERAB;
+                              MODE == COL_ERAB || MODE == COL_DMBF;
   // modes that keep amplitude / background values across the FFT need more
   // registers: tuned for 2 waves per SIMD
   static constexpr bool heavy = MODE == COL_ERR1 || MODE == COL_ERR7 || MODE == COL_ERAB ||
-                                MODE == COL_DMB_UPD || MODE == COL_DMB_LEAD;
+                                MODE == COL_DMB_UPD || MODE == COL_DMB_LEAD ||
+                                MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
 };
 
 // Launch shape of k_cols<L, MODE>.  Pmod_1 modes with at most 16 points per
@@ -338,13 +349,16 @@ template <int L, int MODE> struct ColShape {
   static constexpr int E = L / Plan<L>::T;
   static constexpr bool modulus = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_ERR1 ||
                                   MODE == COL_ERR7 || MODE == COL_ERAB ||
-                                  MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD;
+                                  MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
+                                  MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
   static constexpr bool hoist = modulus && E <= 16;
   // modes that also hold background values across the FFT fit 3 waves
   static constexpr bool bgvals = MODE == COL_ERR7 || MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
-                                 MODE == COL_DMB_UPD;
-  static constexpr int wpe = hoist ? (bgvals ? ((MODE == COL_DMB_UPD && L == 128) ? 2 : 3) : 4)
-                                   : ColTraits<MODE>::heavy ? 2 : 3;  // waves per SIMD
+                                 MODE == COL_DMB_UPD || MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
+  static constexpr int wpe =
+      MODE == COL_DMBF ? 2
+      : hoist ? (bgvals ? ((MODE == COL_DMB_UPD && L == 128) ? 2 : 3) : 4)
+              : ColTraits<MODE>::heavy ? 2 : 3;  // waves per SIMD
 };
 
 // Modulus-step arithmetic: the hardware square root and reciprocal (1 ulp;
@@ -390,7 +404,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
   // hoisted per-frame (or shared) background values; the frame mean bbar is
   // the same for every frame (an L2 hit) and is read where it is used, which
   // keeps DMB_UPD and ERR7 within their register budget
-  constexpr bool HBG = HOIST && (MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD);
+  constexpr bool HBG = HOIST && (MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
+                                 MODE == COL_DMBF_LEAD || MODE == COL_DMBF);
+  constexpr bool DMBF = MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
+  static_assert(MODE != COL_DMBF || HOIST, "the second step reads the hoisted amplitudes");
   constexpr int T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
   __shared__ float2 lds[CPW * W];
@@ -430,9 +447,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     // per-chunk background sums (at most 16 frames): FP32 registers in the
     // hoisted modes (relative rounding <= 16 * 2^-24), written out in FP64
     using BAcc = std::conditional_t<HOIST, float, double>;
-    BAcc bacc[TR::bsum ? E : 1];
+    BAcc bacc[TR::bsum ? E : 1], bacc2[TR::bsum2 ? E : 1];
 #pragma unroll
     for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0;
+#pragma unroll
+    for (int e = 0; e < (TR::bsum2 ? E : 1); ++e) bacc2[e] = 0;
 
     for (int n = n0; n < n1; ++n) {
       // opaque, wave-uniform frame index: keeps the frame base in SGPRs
@@ -440,6 +459,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       float2* Xn = a.X + (size_t)nu * L * K;
       float2 v[E];
       float amh[HOIST ? E : 1], bgh[HBG ? E : 1];
+      // COL_DMBF keeps b_new for its second step (re-reading it and the
+      // amplitudes from cache measured 7.24 vs 6.70 ms per DM iteration at
+      // 4096 x 256^2)
+      float bnh[MODE == COL_DMBF ? E : 1];
+      float2* Fn = DMBF ? a.F + (size_t)nu * L * K : nullptr;
       if constexpr (HOIST) {
         load(item, nu);
         const float* ampn = a.amp + (size_t)nu * a.amp_stride;
@@ -535,6 +559,47 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 *bp = bn;
                 bacc[e] += (BAcc)bn;
                 v[e] = z * (fac * a.scale);
+              } else if constexpr (MODE == COL_DMBF_LEAD) {
+                // z = F (the iterate's spectrum, stored as the state); the
+                // leading Pmod_7 of (f, b) goes to Xo, b itself is not changed
+                float b;
+                if constexpr (HBG) b = bgh[e];
+                else b = ((bgn + uo) + toff)[0];
+                (Fn + uo)[toff] = z;
+                const float mg = msqrt(z2 + b * b + a.alpha);
+                const float fac = m * am * mrcp(mg);
+                bacc[e] += (BAcc)(b * fac);
+                v[e] = z * (fac * a.scale);
+              } else if constexpr (MODE == COL_DMBF) {
+                // z = S, the spectrum of O P.  From the state (F, b): the
+                // leading Pmod_7 (f1, b1), the DM step y = 2 S - f1,
+                // b0 = 2 bbar - b1 (bbar = frame mean of b1), Pmod_7(y, b0),
+                // F_new = f1 - S + y', b_new = b1 - bbar + b0'; the step size
+                // sum |y' - S|^2 / (M K) = sum |f_new - f1|^2 in real space
+                float* bp = (bgn + uo) + toff;
+                float b;
+                if constexpr (HBG) b = bgh[e];
+                else b = *bp;
+                float2* fp = (Fn + uo) + toff;
+                const float2 f = *fp;
+                const float mg1 = msqrt(norm2(f) + b * b + a.alpha);
+                const float fac1 = m * am * mrcp(mg1);
+                const float b1 = b * fac1;
+                const float2 f1 = f * fac1;
+                const float bb = (a.bbar + uo)[toff];
+                const float2 y = make_float2(2.f * z.x - f1.x, 2.f * z.y - f1.y);
+                const float b0 = 2.f * bb - b1;
+                const float mg2 = msqrt(norm2(y) + b0 * b0 + a.alpha);
+                const float fac2 = m * am * mrcp(mg2);
+                const float2 yp = y * fac2;
+                const float2 fn = make_float2(f1.x - z.x + yp.x, f1.y - z.y + yp.y);
+                const float bn = (b1 - bb) + b0 * fac2;
+                acc[0] += (double)norm2(yp - z);
+                *fp = fn;
+                *bp = bn;
+                bnh[e] = bn;
+                bacc[e] += (BAcc)bn;
+                v[e] = fn * a.scale;
               }
             }
           }
@@ -554,9 +619,36 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
         }
         fft_inv<L>(v, mid, s_tw, t);
+        float2* dst = MODE == COL_DMBF_LEAD ? a.Xo + (size_t)nu * L * K : Xn;
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (Xn + (size_t)rowA<L>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) (dst + (size_t)rowA<L>(e) * K)[toff] = v[e];
+        }
+      }
+      if constexpr (MODE == COL_DMBF) {
+        // the next iteration's leading Pmod_7 of (F_new, b_new) into Xo; F_new
+        // is re-read from what this thread just stored
+        if (cv) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const size_t uo = (size_t)rowB<L>(e) * K;
+            const float2 fn = (Fn + uo)[toff];
+            const float m = ((mbits >> e) & 1u) ? 1.f : 0.f;
+            const float mg = msqrt(norm2(fn) + bnh[e] * bnh[e] + a.alpha);
+            const float fac = m * amh[e] * mrcp(mg);
+            bacc2[e] += (BAcc)(bnh[e] * fac);
+            v[e] = fn * (fac * a.scale);
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
+        }
+        lds_barrier();
+        fft_inv<L>(v, mid, s_tw, t);
+        float2* Xon = a.Xo + (size_t)nu * L * K;
+        if (cv) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) (Xon + (size_t)rowA<L>(e) * K)[toff] = v[e];
         }
       }
       lds_barrier();
@@ -569,8 +661,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         for (int e = 0; e < E; ++e) (pb + (size_t)rowB<L>(e) * K)[toff] = (double)bacc[e];
       }
     }
+    if constexpr (TR::bsum2) {
+      if (cv) {
+        double* pb = a.partb2 + (size_t)chunk * L * K;
+#pragma unroll
+        for (int e = 0; e < E; ++e) (pb + (size_t)rowB<L>(e) * K)[toff] = (double)bacc2[e];
+      }
+    }
   }
   if constexpr (TR::err) {
+    if constexpr (MODE == COL_DMBF) acc[0] *= (double)a.scale;   // sum |f_new - f1|^2
     block_sum<2>(acc, red);
     if (tid == 0) {
       if (a.part) a.part[blockIdx.x] = acc[0];

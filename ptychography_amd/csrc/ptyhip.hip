@@ -194,6 +194,16 @@ struct pty_ctx {
   float2* Dx = nullptr;
   double* pos_terms = nullptr;  // N x 6 per-frame position terms
 
+  // DM with background (method 4) carried in Fourier space: F holds the
+  // iterate's spectra FFT2(f), Xo the column-inverted spectra of the next
+  // leading Pmod_7 of (f, b), partb2 that step's background sums
+  float2* Xo = nullptr;
+  double* partb2 = nullptr;
+  bool f_spec = false;       // F holds spectra (else f in real space)
+  bool dmf_lead = false;     // Xo and partb2 belong to the current (F, b)
+  bool dmf_run = false;      // this run takes the Fourier-space DM step
+  float dmf_alpha = 0.f;     // alpha of the leading step in Xo
+
   // hipGraph replay of whole iterations
   struct GraphEntry {
     std::vector<uint64_t> key;
@@ -504,13 +514,13 @@ int rowg_attribute(int K) {
 // (H valid); one work-group per object row segment, partials per segment.  Fout, when
 // given, also receives the inverse-transformed rows (what rows_inv<DST_STORE> writes).
 int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, int* nparts,
-                float2* Fout = nullptr) {
+                float2* Fout = nullptr, float2* Oout = nullptr) {
   StageTimer tm(c, PTY_STAGE_GATHER);
   RowGatherArgs a{};
   a.Y = Y;
   a.P = c->P;
-  a.Oold = econ ? c->O : nullptr;
-  a.Oout = c->O;
+  a.Oout = Oout ? Oout : c->O;
+  a.Oold = econ ? a.Oout : nullptr;
   a.H = c->H;
   a.alpha = alpha;
   a.segs = c->rowg_segs;
@@ -639,11 +649,11 @@ int sum_region(pty_ctx* c, const double* p, int n, double* host_out) {
   return PTY_OK;
 }
 
-int bg_mean(pty_ctx* c, int nchunks, float* out) {
+int bg_mean(pty_ctx* c, int nchunks, float* out, const double* partb = nullptr) {
   StageTimer tm(c, PTY_STAGE_REDUCE);
   const int MK = (int)c->MK();
   hipLaunchKernelGGL(k_bg_mean, dim3((MK + WG - 1) / WG), dim3(WG), 0, c->stream,
-                     (const double*)c->partb, nchunks, MK, c->N, out);
+                     partb ? partb : (const double*)c->partb, nchunks, MK, c->N, out);
   PTY_LAUNCHED("k_bg_mean");
   return PTY_OK;
 }
@@ -789,6 +799,7 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     c->G_valid = true;
   } else if (rowg_op) {
     c->dm_ready = false;   // F now holds psi', not a DM iterate: a DM run must begin again
+    c->f_spec = c->dmf_lead = false;
     PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o.alpha, false, false));
     c->H_valid = true;
     PTY_TRY(rows_gather(c, c->X, o.alpha, true, false, &ngather, c->F));
@@ -1052,6 +1063,80 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   return PTY_OK;
 }
 
+// One DM iteration with background refinement (method 4, fixed probe) on the
+// iterate's spectra F (dm.py:169-294).  The reference's steps, with the
+// leading Pmod_7 of iteration i + 1 folded into iteration i's column pass:
+//   O  = Psup(f1)                 inverse row FFT of Xo fused into the gather
+//   S  = FFT2(O P)                row FFT, then (column stage, per pixel)
+//   (f1, b1) = Pmod_7(F, b)       recomputed from the state
+//   y = 2 S - f1, b0 = 2 bbar - b1, (y', b0') = Pmod_7(y, b0)
+//   F <- f1 - S + y', b <- b1 - bbar + b0'   (bbar = frame mean of b1)
+//   X  = col-IFFT(F), Xo = col-IFFT(first half of Pmod_7(F, b))
+//   Os = Psup(f) from X, eMod of Os P with the frame mean of b.
+// Per frame pixel: 8 (Xo) + 8 + 52 (rows forward, column pass) + 8 (X) + 20
+// (metric) = 96 bytes, against 156 for the same steps on f in real space.
+int dm_bg_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
+  const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
+  const int nrows = c->N * c->M;
+  const int nchunks = (c->N + c->col_chunk - 1) / c->col_chunk;
+  (void)it;
+  PTY_TRY(bg_mean(c, nchunks, c->bbar, c->partb2));   // frame mean of b1
+  if (!c->H_valid) {
+    PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o.alpha, false, false));
+    c->H_valid = true;
+  }
+  PTY_TRY(rows_gather(c, c->Xo, o.alpha, false, false, nullptr));
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
+  int nstep = 0;
+  {
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.amp = c->amp;
+    a.amp_stride = (long)MK;
+    a.alpha = o.alpha;
+    a.bg = c->bg;
+    a.bg_stride = (long)MK;
+    a.bbar = c->bbar;
+    a.partb = c->partb;
+    a.partb2 = c->partb2;
+    a.F = c->F;
+    a.Xo = c->Xo;
+    a.part = c->part_esa;
+    PTY_TRY(cols<COL_DMBF>(c, a, &nstep));
+  }
+  // metrics: Os = Psup(f), the frame mean of b, eMod of Os P (dm.py:231-271)
+  PTY_TRY(bg_mean(c, nchunks, c->bbar_m, c->partb));
+  PTY_TRY(rows_gather(c, c->X, o.alpha, false, false, nullptr, nullptr, c->Os));
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->P, nullptr, c->X, nrows));
+  int ncol = 0;
+  {
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.amp = c->amp;
+    a.amp_stride = (long)MK;
+    a.part = c->part_emod;
+    a.bbar = c->bbar_m;
+    PTY_TRY(cols<COL_ERR7>(c, a, &ncol));
+  }
+  Regions rg{};
+  rg.p[S_EMOD] = c->part_emod;
+  rg.n[S_EMOD] = ncol;
+  PTY_TRY(econ_pass(c, c->Os, c->Obak, UV, c->part_econO));
+  rg.p[S_ECON_ON] = c->part_econO;
+  rg.p[S_ECON_OD] = c->part_econO + ECON_BLOCKS;
+  rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
+  rg.p[S_ESUP_E] = c->part_esa;
+  rg.n[S_ESUP_E] = nstep;
+  PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, metrics_out));
+  {
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->Obak, c->Os, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->Ps, c->P, MK * sizeof(float2)));
+  }
+  c->have_metric_state = true;
+  return PTY_OK;
+}
+
 // Host decisions and buffer addresses that shape one iteration's launches.
 std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
   const int upd = (o.method - 1) % 3;
@@ -1068,7 +1153,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
           (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp,
-          (uint64_t)(uintptr_t)c->g.Pn};
+          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->dmf_run};
 }
 
 HostIterState save_state(const pty_ctx* c) {
@@ -1163,6 +1248,48 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
 }
 
 // validation and per-run set-up of pty_run (before any iteration)
+// Host-side switch for timing the real-space DM-with-background path:
+// PTY_DM_FOURIER=0.  Read at every run.
+bool dmf_disabled() {
+  const char* e = getenv("PTY_DM_FOURIER");
+  return e && atoi(e) == 0;
+}
+
+// F (spectra) -> f in real space, through X
+int dm_f_to_real(pty_ctx* c) {
+  PTY_TRY(copy_d2d(c, c->X, c->F, c->NMK() * sizeof(float2)));
+  ColArgs a = col_args(c);
+  a.X = c->X;
+  a.nframes = c->N;
+  a.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+  PTY_TRY(cols<COL_INV>(c, a));
+  PTY_TRY(rows_inv<DST_STORE>(c, c->X, c->F, nullptr, nullptr, 1.0f, nullptr, c->N * c->M));
+  c->f_spec = false;
+  c->dmf_lead = false;
+  return PTY_OK;
+}
+
+// f in real space -> F = its spectra, with the leading Pmod_7 of (f, b)
+// column-inverted into Xo and its background sums in partb2 (b unchanged)
+int dm_lead(pty_ctx* c, float alpha) {
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->F, c->X, c->N * c->M));
+  ColArgs a = col_args(c);
+  a.X = c->X;
+  a.amp = c->amp;
+  a.amp_stride = (long)c->MK();
+  a.alpha = alpha;
+  a.bg = c->bg;
+  a.bg_stride = (long)c->MK();
+  a.partb = c->partb2;
+  a.F = c->F;
+  a.Xo = c->Xo;
+  PTY_TRY(cols<COL_DMBF_LEAD>(c, a));
+  c->f_spec = true;
+  c->dmf_lead = true;
+  c->dmf_alpha = alpha;
+  return PTY_OK;
+}
+
 // dst[n] = IFFT(FFT(P) ramp(d_n)) (deriv 0) or its derivative in d_n0 / d_n1
 // (deriv 1 / 2), from the spectrum of P in Xp
 int probe_stack(pty_ctx* c, float2* dst, int deriv) {
@@ -1231,9 +1358,26 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
                          (const float2*)c->P, c->F);
       PTY_LAUNCHED("k_make_exits");
       c->dm_ready = true;
+      c->f_spec = false;
+      c->dmf_lead = false;
     }
   }
   if (o->algorithm == 1 && !c->dm_ready) return fail(PTY_ESTATE, "DM run needs begin=1 first");
+  // DM with background refinement and a fixed probe (method 4, no far-field
+  // probe constraint in this run) carries its iterate as spectra when the
+  // fused row gather is available; everything else in real space
+  c->dmf_run = o->algorithm == 1 && o->method == 4 && o->pmod_probe_until <= iter0 && c->rowg &&
+               !dmf_disabled();
+  if (o->algorithm == 1) {
+    if (c->dmf_run) {
+      if (!c->Xo) PTY_TRY(dev_alloc(&c->Xo, c->NMK()));
+      if (!c->partb2) PTY_TRY(dev_alloc(&c->partb2, (size_t)c->col_chunks * c->MK()));
+      if (c->f_spec && (!c->dmf_lead || c->dmf_alpha != o->alpha)) PTY_TRY(dm_f_to_real(c));
+      if (!c->f_spec) PTY_TRY(dm_lead(c, o->alpha));
+    } else if (c->f_spec) {
+      PTY_TRY(dm_f_to_real(c));
+    }
+  }
   // ERA's object + probe steps keep psi' in F when the fused row gather runs
   // them (allocated here, never inside a capture)
   if (o->algorithm == 0 && (o->method == 3 || o->method == 6) && c->rowg) PTY_TRY(ensure_F(c));
@@ -1242,8 +1386,11 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
   return PTY_OK;
 }
 
+int dm_bg_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out);
+
 int run_one(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_dev) {
-  return o.algorithm == 0 ? era_iter(c, o, it, metrics_dev) : dm_iter(c, o, it, metrics_dev);
+  if (o.algorithm == 0) return era_iter(c, o, it, metrics_dev);
+  return c->dmf_run ? dm_bg_fourier_iter(c, o, it, metrics_dev) : dm_iter(c, o, it, metrics_dev);
 }
 
 int run_finish(pty_ctx* c, int32_t iters, double* metrics_host) {
@@ -1556,6 +1703,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->pnum); dev_free(c->pg);
   dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
   dev_free(c->sub_d); dev_free(c->Pn); dev_free(c->Dy); dev_free(c->Dx); dev_free(c->pos_terms);
+  dev_free(c->Xo); dev_free(c->partb2);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1589,6 +1737,7 @@ int pty_set_data(pty_ctx* c, const float* amp, const uint8_t* mask, double I_nor
   c->amp = amp;
   c->mask = mask;
   c->I_norm = I_norm;
+  c->dmf_lead = false;   // the leading Pmod_7 in Xo used the previous data
   if (mask) {
     const int T = threads_per_line(c->M);
     if (!c->maskbits) PTY_TRY(dev_alloc(&c->maskbits, (size_t)T * c->K));
@@ -1698,6 +1847,7 @@ int pty_set_background(pty_ctx* c, const float* b, int32_t per_frame) {
   }
   c->bg_set = true;
   c->era_bg_shared = false;
+  c->dmf_lead = false;   // the leading Pmod_7 in Xo used the previous background
   return PTY_OK;
 }
 
@@ -2146,6 +2296,7 @@ int pty_set_iterate(pty_ctx* c, const pty_cfloat* f, const pty_cfloat* O_ref, co
     PTY_TRY(ensure_F(c));
     PTY_TRY(copy_d2d(c, c->F, f, c->NMK() * sizeof(float2)));
     c->dm_ready = true;
+    c->f_spec = c->dmf_lead = false;
   }
   const size_t UV = (size_t)c->U * c->V;
   PTY_TRY(copy_d2d(c, c->Obak, O_ref ? (const void*)O_ref : (const void*)c->O, UV * sizeof(float2)));
@@ -2160,7 +2311,17 @@ int pty_get_iterate(pty_ctx* c, pty_cfloat* f, float* b) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (f) {
     if (!c->F || !c->dm_ready) return fail(PTY_ESTATE, "no DM iterate (run DM or pty_set_iterate first)");
-    PTY_TRY(copy_d2d(c, f, c->F, c->NMK() * sizeof(float2)));
+    float2* fo = reinterpret_cast<float2*>(f);
+    PTY_TRY(copy_d2d(c, fo, c->F, c->NMK() * sizeof(float2)));
+    if (c->f_spec) {
+      // the iterate is held as spectra: inverse 2-D transform in the caller's buffer
+      ColArgs a = col_args(c);
+      a.X = fo;
+      a.nframes = c->N;
+      a.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+      PTY_TRY(cols<COL_INV>(c, a));
+      PTY_TRY(rows_inv<DST_STORE>(c, fo, fo, nullptr, nullptr, 1.0f, nullptr, c->N * c->M));
+    }
   }
   if (b) {
     if (!c->bg_set) return fail(PTY_ESTATE, "no background state");

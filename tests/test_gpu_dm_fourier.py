@@ -1,0 +1,116 @@
+"""DM with background refinement (method 4) with its iterate held as spectra.
+
+With the probe fixed and no far-field probe constraint, the engine carries
+DM's iterate f as F = FFT2(f) and folds the next iteration's leading Pmod_7
+into the column pass of the DM step (``dm_bg_fourier_iter``); the real-space
+path (PTY_DM_FOURIER=0, or any other DM method) stores f itself.  The two
+are the same algorithm and differ by rounding.  DM is not a contraction: a
+one-ulp difference grows about 1.5x per iteration (the background state, of
+small values, faster), so the one-step results must agree to 1e-6 (rel L2)
+and after 3 iterations the object, f (through ``pty_get_iterate``, which
+transforms it back) and the background to the row-gather tests' 1e-5; eMod
+within 1e-5 relative.  Oracle parity of the spectral
+path is covered by the DM method-4 cases of test_gpu_solvers.py and
+test_gpu_step_parity.py, which take it by default.
+"""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from ptychography_amd.engine import Engine
+from tests.scans import broadband_start, rel_l2, small_scan
+
+pytestmark = pytest.mark.gpu
+
+
+class _Env:
+    def __init__(self, real):
+        self.real = real
+
+    def __enter__(self):
+        self.old = os.environ.get("PTY_DM_FOURIER")
+        if self.real:
+            os.environ["PTY_DM_FOURIER"] = "0"
+        else:
+            os.environ.pop("PTY_DM_FOURIER", None)
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            os.environ.pop("PTY_DM_FOURIER", None)
+        else:
+            os.environ["PTY_DM_FOURIER"] = self.old
+
+
+def _setup(s, dev, bg):
+    N, M, K = s.I.shape
+    U, V = s.O.shape
+    eng = Engine(N, M, K, U, V, s.R.astype(np.int32), device=dev)
+    amp, I_norm = eng.prepare_amp(torch.from_numpy(s.I.astype(np.float32)), None)
+    eng.set_data(amp, None, I_norm)
+    O_t = torch.from_numpy(broadband_start((U, V)).astype(np.complex64)).to(dev)
+    P_t = torch.from_numpy(s.P.astype(np.complex64)).to(dev)
+    eng.set_state(O_t, P_t)
+    eng.set_background(torch.from_numpy(bg.astype(np.float32)), per_frame=False)
+    return eng, O_t, amp
+
+
+def _run(s, dev, bg, plan):
+    """plan: list of (iterations, real-space path?) runs, the first with begin."""
+    eng, O_t, amp = _setup(s, dev, bg)
+    try:
+        rows, Os, it = [], [], 0
+        for k, (n, real) in enumerate(plan):
+            with _Env(real):
+                rows.append(eng.run(1, 4, n, iter0=it, begin=(k == 0)))
+            it += n
+            Os.append(O_t.cpu().numpy())
+        f, b = eng.get_iterate(f=True, background=True)
+        return np.concatenate(rows), Os, f.cpu().numpy(), b.cpu().numpy()
+    finally:
+        eng.close()
+
+
+CASES = [(64, (5, 5), 12), (256, (3, 3), 48)]
+
+
+@pytest.mark.parametrize("frame,grid,step", CASES)
+def test_spectral_and_real_space_paths_agree(frame, grid, step, gpu_device):
+    print(f"DM method 4 spectra vs real space, frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=grid, step=step, jitter=2, background_frac=0.02)
+    bg = np.random.default_rng(4).random((frame, frame)) * 0.5
+    ma, Oa, fa, ba = _run(s, gpu_device, bg, [(1, False), (2, False)])
+    mb, Ob, fb, bb = _run(s, gpu_device, bg, [(1, True), (2, True)])
+    errs = [rel_l2(x, y) for x, y in zip(Oa, Ob)]
+    print("object rel L2 after 1 / 3:", " ".join(f"{e:.1e}" for e in errs),
+          f"f {rel_l2(fa, fb):.1e} b {rel_l2(ba, bb):.1e}")
+    assert errs[0] < 1e-6 and errs[1] < 1e-5
+    assert rel_l2(fa, fb) < 1e-5 and rel_l2(ba, bb) < 1e-5
+    assert np.allclose(ma[:, 0], mb[:, 0], rtol=1e-5)
+    assert np.allclose(ma[:, 2], mb[:, 2], rtol=1e-4)
+
+
+def test_split_runs_equal_one_run(gpu_device):
+    """The spectra, the folded leading step and its background sums carry
+    from one pty_run to the next: 2 + 4 iterations = 6, bit for bit."""
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=2, background_frac=0.02)
+    bg = np.random.default_rng(5).random((64, 64)) * 0.5
+    ma, Oa, fa, ba = _run(s, gpu_device, bg, [(6, False)])
+    mb, Ob, fb, bb = _run(s, gpu_device, bg, [(2, False), (4, False)])
+    assert np.array_equal(Oa[-1], Ob[-1])
+    assert np.array_equal(fa, fb) and np.array_equal(ba, bb)
+    assert np.array_equal(ma, mb)
+
+
+def test_switching_paths_mid_run(gpu_device):
+    """2 iterations on spectra, then 1 in real space (the engine transforms
+    F back), against 3 in real space."""
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=2, background_frac=0.02)
+    bg = np.random.default_rng(6).random((64, 64)) * 0.5
+    ma, Oa, fa, ba = _run(s, gpu_device, bg, [(2, False), (1, True)])
+    mb, Ob, fb, bb = _run(s, gpu_device, bg, [(2, True), (1, True)])
+    print(f"object {rel_l2(Oa[-1], Ob[-1]):.1e} f {rel_l2(fa, fb):.1e}")
+    assert rel_l2(Oa[-1], Ob[-1]) < 1e-5 and rel_l2(fa, fb) < 1e-5
+    assert np.allclose(ma[:, 0], mb[:, 0], rtol=1e-5)
