@@ -293,6 +293,9 @@ enum ColMode {
   COL_DMBF_LEAD,  // F = spectrum of the input; Xo = col-IFFT(Pmod_7(F, b)); sums b'
   COL_DMBF,       // one DM step on (F, b) from the spectra of O P; X = col-IFFT(F_new),
                   // Xo = col-IFFT(Pmod_7(F_new, b_new)); sums b_new and b'
+  COL_DM1F,       // one DM step on F (no background) from the spectra S of O P;
+                  // X = col-IFFT(F_new); the model error of S (the previous
+                  // iteration's metric, O being its metric object) when part is set
 };
 
 struct ColArgs {
@@ -322,13 +325,13 @@ template <int MODE> struct ColTraits {
   static constexpr bool inv = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP ||
                               MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
                               MODE == COL_DMB_UPD || MODE == COL_INV || MODE == COL_DMBF_LEAD ||
-                              MODE == COL_DMBF;
+                              MODE == COL_DMBF || MODE == COL_DM1F;
   static constexpr bool store_fwd = MODE == COL_FWD;
   static constexpr bool bsum = MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
                                MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
   static constexpr bool bsum2 = MODE == COL_DMBF;
   static constexpr bool err = MODE == COL_ERA1 || MODE == COL_ERR1 || MODE == COL_ERR7 ||
-                              MODE == COL_ERAB || MODE == COL_DMBF;
+                              MODE == COL_ERAB || MODE == COL_DMBF || MODE == COL_DM1F;
   // modes that keep amplitude / background values across the FFT need more
   // registers: tuned for 2 waves per SIMD
   static constexpr bool heavy = MODE == COL_ERR1 || MODE == COL_ERR7 || MODE == COL_ERAB ||
@@ -350,7 +353,7 @@ template <int L, int MODE> struct ColShape {
   static constexpr bool modulus = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_ERR1 ||
                                   MODE == COL_ERR7 || MODE == COL_ERAB ||
                                   MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
-                                  MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
+                                  MODE == COL_DMBF_LEAD || MODE == COL_DMBF || MODE == COL_DM1F;
   static constexpr bool hoist = modulus && E <= 16;
   // modes that also hold background values across the FFT fit 3 waves
   static constexpr bool bgvals = MODE == COL_ERR7 || MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
@@ -406,7 +409,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
   // keeps DMB_UPD and ERR7 within their register budget
   constexpr bool HBG = HOIST && (MODE == COL_ERAB || MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
                                  MODE == COL_DMBF_LEAD || MODE == COL_DMBF);
-  constexpr bool DMBF = MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
+  constexpr bool DMBF = MODE == COL_DMBF_LEAD || MODE == COL_DMBF || MODE == COL_DM1F;
   static_assert(MODE != COL_DMBF || HOIST, "the second step reads the hoisted amplitudes");
   constexpr int T = Plan<L>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<L>::words;
@@ -600,6 +603,24 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 bnh[e] = bn;
                 bacc[e] += (BAcc)bn;
                 v[e] = fn * a.scale;
+              } else if constexpr (MODE == COL_DM1F) {
+                // z = S: the model error of S (dm.py:337-341) when asked for;
+                // y = 2 S - f, y' = Pmod_1(y) (masked pixels keep y),
+                // F_new = f - S + y', step size sum |y' - S|^2 / (M K)
+                if (a.part) {
+                  const float d = msqrt(z2) - am;
+                  acc[0] += (double)(m * d * d);
+                }
+                float2* fp = (Fn + uo) + toff;
+                const float2 f = *fp;
+                const float2 y = make_float2(2.f * z.x - f.x, 2.f * z.y - f.y);
+                const float mg = msqrt(norm2(y)) + a.alpha;
+                const float fac = (m != 0.f) ? am * mrcp(mg) : 1.f;
+                const float2 yp = y * fac;
+                const float2 fn = make_float2(f.x - z.x + yp.x, f.y - z.y + yp.y);
+                acc[1] += (double)norm2(yp - z);
+                *fp = fn;
+                v[e] = fn * a.scale;
               }
             }
           }
@@ -671,6 +692,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
   }
   if constexpr (TR::err) {
     if constexpr (MODE == COL_DMBF) acc[0] *= (double)a.scale;   // sum |f_new - f1|^2
+    if constexpr (MODE == COL_DM1F) acc[1] *= (double)a.scale;   // sum |f_new - f|^2
     block_sum<2>(acc, red);
     if (tid == 0) {
       if (a.part) a.part[blockIdx.x] = acc[0];

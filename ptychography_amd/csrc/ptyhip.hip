@@ -93,6 +93,9 @@ struct HostIterState {
   bool H_valid, G_valid, era_bg_shared, have_metric_state, os_next;
   float2* F;
   float2* X;
+  bool dm1_pending;
+  int dm1_par;
+  bool f_spec;
 };
 
 struct pty_ctx {
@@ -203,6 +206,13 @@ struct pty_ctx {
   bool dmf_lead = false;     // Xo and partb2 belong to the current (F, b)
   bool dmf_run = false;      // this run takes the Fourier-space DM step
   float dmf_alpha = 0.f;     // alpha of the leading step in Xo
+  // DM method 1 on spectra: the metric of iteration i is the model error of
+  // O_{i+1} P, formed in iteration i + 1's column pass (or by the run's tail)
+  bool dm1_run = false;      // DM method 1 runs on spectra from iteration pmod_probe_until on
+  bool dm1_pending = false;  // Os awaits its model error; its metrics are not yet written
+  int dm1_par = 0;           // which step-size partial buffer the last step wrote
+  int dm1_nstep = 0;         // partials per step
+  double* dm1_esup[2] = {nullptr, nullptr};
 
   // hipGraph replay of whole iterations
   struct GraphEntry {
@@ -1137,6 +1147,94 @@ int dm_bg_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metric
   return PTY_OK;
 }
 
+// One DM method-1 iteration on the iterate's spectra F (dm.py:45-160):
+//   O = Os of the previous iteration (= Psup(f), dm.py:85)
+//   S = FFT2(O P): row FFT, then (column stage, per pixel) the previous
+//     iteration's model error of S when it is pending, y = 2 S - f,
+//     y' = Pmod_1(y), F <- f - S + y', step size sum |y' - S|^2 / (M K),
+//     X = col-IFFT(F)
+//   Os = Psup(f) from X (the metric object, dm.py:104-137), eCon against the
+//     previous Os; iteration i's metrics are written once iteration i + 1
+//     (or the run's tail, dm1_tail) has formed the model error of Os P.
+// Per frame pixel: 8 + 36 (rows forward, column pass) + 8 (gather) = 52
+// bytes, against 88 on f in real space.
+int dm1_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
+  const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
+  const int nrows = c->N * c->M;
+  (void)it;
+  if (c->dm1_pending) {
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->O, c->Os, UV * sizeof(float2)));
+  }
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
+  const int par = c->dm1_pending ? 1 - c->dm1_par : c->dm1_par;
+  int ncol = 0;
+  {
+    ColArgs a = col_args(c);
+    a.X = c->X;
+    a.amp = c->amp;
+    a.amp_stride = (long)MK;
+    a.alpha = o.alpha;
+    a.F = c->F;
+    a.part = c->dm1_pending ? c->part_emod : nullptr;
+    a.part2 = c->dm1_esup[par];
+    PTY_TRY(cols<COL_DM1F>(c, a, &ncol));
+  }
+  if (c->dm1_pending) {
+    // the previous iteration's metrics: eMod of its Os (= this O), its step
+    // size and eCon
+    Regions rg{};
+    rg.p[S_EMOD] = c->part_emod;
+    rg.n[S_EMOD] = ncol;
+    rg.p[S_ECON_ON] = c->part_econO;
+    rg.p[S_ECON_OD] = c->part_econO + ECON_BLOCKS;
+    rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
+    rg.p[S_ESUP_E] = c->dm1_esup[c->dm1_par];
+    rg.n[S_ESUP_E] = c->dm1_nstep;
+    PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, c->use_slot ? metrics_out : metrics_out - 3));
+  }
+  c->dm1_par = par;
+  c->dm1_nstep = ncol;
+  if (!c->H_valid) {
+    PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o.alpha, false, false));
+    c->H_valid = true;
+  }
+  PTY_TRY(rows_gather(c, c->X, o.alpha, false, false, nullptr, nullptr, c->Os));
+  PTY_TRY(econ_pass(c, c->Os, c->Obak, UV, c->part_econO));
+  {
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->Obak, c->Os, UV * sizeof(float2)));
+  }
+  c->dm1_pending = true;
+  c->have_metric_state = true;
+  return PTY_OK;
+}
+
+// The last iteration's metrics: the model error of its Os P (dm.py:337-341).
+int dm1_tail(pty_ctx* c, double* metrics_out) {
+  if (!c->dm1_pending) return PTY_OK;
+  const size_t MK = c->MK();
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->P, nullptr, c->X, c->N * c->M));
+  int ncol = 0;
+  ColArgs a = col_args(c);
+  a.X = c->X;
+  a.amp = c->amp;
+  a.amp_stride = (long)MK;
+  a.part = c->part_emod;
+  PTY_TRY(cols<COL_ERR1>(c, a, &ncol));
+  Regions rg{};
+  rg.p[S_EMOD] = c->part_emod;
+  rg.n[S_EMOD] = ncol;
+  rg.p[S_ECON_ON] = c->part_econO;
+  rg.p[S_ECON_OD] = c->part_econO + ECON_BLOCKS;
+  rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
+  rg.p[S_ESUP_E] = c->dm1_esup[c->dm1_par];
+  rg.n[S_ESUP_E] = c->dm1_nstep;
+  PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, metrics_out));
+  c->dm1_pending = false;
+  return PTY_OK;
+}
+
 // Host decisions and buffer addresses that shape one iteration's launches.
 std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
   const int upd = (o.method - 1) % 3;
@@ -1153,11 +1251,13 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
           (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp,
-          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->dmf_run};
+          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->dmf_run, (uint64_t)c->dm1_run,
+          (uint64_t)c->dm1_pending, (uint64_t)c->dm1_par, (uint64_t)c->f_spec};
 }
 
 HostIterState save_state(const pty_ctx* c) {
-  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->os_next, c->F, c->X};
+  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->os_next, c->F, c->X,
+          c->dm1_pending, c->dm1_par, c->f_spec};
 }
 void load_state(pty_ctx* c, const HostIterState& st) {
   c->H_valid = st.H_valid;
@@ -1167,6 +1267,9 @@ void load_state(pty_ctx* c, const HostIterState& st) {
   c->os_next = st.os_next;
   c->F = st.F;
   c->X = st.X;
+  c->dm1_pending = st.dm1_pending;
+  c->dm1_par = st.dm1_par;
+  c->f_spec = st.f_spec;
 }
 
 void drop_graphs(pty_ctx* c) {
@@ -1253,6 +1356,23 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
 bool dmf_disabled() {
   const char* e = getenv("PTY_DM_FOURIER");
   return e && atoi(e) == 0;
+}
+
+// DM method 1 entering the spectral path from f in real space: the
+// iteration's O = Psup(f) (dm.py:85), then F = FFT2(f) in place
+int dm1_to_spectra(pty_ctx* c, float alpha) {
+  PTY_TRY(gather(c, c->F, c->P, nullptr, c->O, c->H, !c->H_valid, 1, alpha, false, false));
+  c->H_valid = true;
+  PTY_TRY(copy_d2d(c, c->Ps, c->P, c->MK() * sizeof(float2)));
+  PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->F, c->F, c->N * c->M));
+  ColArgs a = col_args(c);
+  a.X = c->F;
+  a.nframes = c->N;
+  a.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+  PTY_TRY(cols<COL_FWD>(c, a));
+  c->f_spec = true;
+  c->dmf_lead = false;
+  return PTY_OK;
 }
 
 // F (spectra) -> f in real space, through X
@@ -1368,8 +1488,38 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
   // fused row gather is available; everything else in real space
   c->dmf_run = o->algorithm == 1 && o->method == 4 && o->pmod_probe_until <= iter0 && c->rowg &&
                !dmf_disabled();
+  // (iterations inside a Pmod_probe window take the real-space path; the
+  // first iteration after it switches, so the split of iterations into runs
+  // does not change the result)
+  c->dm1_run = o->algorithm == 1 && o->method == 1 && c->rowg && !dmf_disabled() && !c->dmf_run &&
+               iters > 0 && o->pmod_probe_until < iter0 + iters;
+  c->dm1_pending = false;
+  if (c->dm1_run) {
+    for (int k = 0; k < 2; ++k)
+      if (!c->dm1_esup[k]) PTY_TRY(dev_alloc(&c->dm1_esup[k], (size_t)c->cap_col * MAX_CHUNKS));
+  }
   if (o->algorithm == 1) {
-    if (c->dmf_run) {
+    if (c->dm1_run && o->pmod_probe_until <= iter0) {
+      PTY_TRY(copy_d2d(c, c->Ps, c->P, c->MK() * sizeof(float2)));
+      ColArgs a = col_args(c);
+      a.nframes = c->N;
+      a.chunk = std::max(1, std::min(16, (int)((long)c->N * c->col_blocks / 2048)));
+      // the first iteration's O = Psup(f) (dm.py:85)
+      if (c->f_spec) {
+        // from the spectra, through the kernels an iteration's metric
+        // projection takes (so split runs equal one run bit for bit)
+        PTY_TRY(copy_d2d(c, c->X, c->F, c->NMK() * sizeof(float2)));
+        a.X = c->X;
+        PTY_TRY(cols<COL_INV>(c, a));
+        if (!c->H_valid) {
+          PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o->alpha, false, false));
+          c->H_valid = true;
+        }
+        PTY_TRY(rows_gather(c, c->X, o->alpha, false, false, nullptr));
+      } else {
+        PTY_TRY(dm1_to_spectra(c, o->alpha));
+      }
+    } else if (c->dmf_run) {
       if (!c->Xo) PTY_TRY(dev_alloc(&c->Xo, c->NMK()));
       if (!c->partb2) PTY_TRY(dev_alloc(&c->partb2, (size_t)c->col_chunks * c->MK()));
       if (c->f_spec && (!c->dmf_lead || c->dmf_alpha != o->alpha)) PTY_TRY(dm_f_to_real(c));
@@ -1388,9 +1538,22 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
 
 int dm_bg_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out);
 
+int dm1_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out);
+int dm1_tail(pty_ctx* c, double* metrics_out);
+
 int run_one(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_dev) {
   if (o.algorithm == 0) return era_iter(c, o, it, metrics_dev);
+  if (c->dm1_run && it >= o.pmod_probe_until) {
+    if (!c->f_spec) PTY_TRY(dm1_to_spectra(c, o.alpha));   // leaving a Pmod_probe window
+    return dm1_fourier_iter(c, o, it, metrics_dev);
+  }
   return c->dmf_run ? dm_bg_fourier_iter(c, o, it, metrics_dev) : dm_iter(c, o, it, metrics_dev);
+}
+
+// after a run's iterations: the metrics a spectral DM method-1 run still owes
+int run_tail(pty_ctx* c, int32_t iters) {
+  if (!c->dm1_pending || iters <= 0) return PTY_OK;
+  return dm1_tail(c, c->use_slot ? c->metrics : c->metrics + 3 * (size_t)(iters - 1));
 }
 
 int run_finish(pty_ctx* c, int32_t iters, double* metrics_host) {
@@ -1703,7 +1866,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->pnum); dev_free(c->pg);
   dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
   dev_free(c->sub_d); dev_free(c->Pn); dev_free(c->Dy); dev_free(c->Dx); dev_free(c->pos_terms);
-  dev_free(c->Xo); dev_free(c->partb2);
+  dev_free(c->Xo); dev_free(c->partb2); dev_free(c->dm1_esup[0]); dev_free(c->dm1_esup[1]);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -1878,6 +2041,7 @@ int pty_run(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t
   } else {
     for (int32_t k = 0; k < iters; ++k) PTY_TRY(run_one(c, *o, iter0 + k, c->metrics + 3 * (size_t)k));
   }
+  PTY_TRY(run_tail(c, iters));
   return run_finish(c, iters, metrics_host);
 }
 
@@ -2203,6 +2367,7 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
     }
     PTY_HIP(hipStreamSynchronize(b->origin));
   }
+  for (int s = 0; s < n; ++s) PTY_TRY(run_tail(b->ctx[s], iters));
   for (int s = 0; s < n; ++s) {
     pty_ctx* c = b->ctx[s];
     if (metrics_host && metrics_host[s] && iters > 0)

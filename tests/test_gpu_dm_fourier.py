@@ -1,9 +1,12 @@
-"""DM with background refinement (method 4) with its iterate held as spectra.
+"""DM methods 1 and 4 with the iterate held as spectra.
 
 With the probe fixed and no far-field probe constraint, the engine carries
-DM's iterate f as F = FFT2(f) and folds the next iteration's leading Pmod_7
-into the column pass of the DM step (``dm_bg_fourier_iter``); the real-space
-path (PTY_DM_FOURIER=0, or any other DM method) stores f itself.  The two
+DM's iterate f as F = FFT2(f): with background refinement (method 4) it folds
+the next iteration's leading Pmod_7 into the column pass of the DM step
+(``dm_bg_fourier_iter``); without (method 1) it forms each iteration's metric
+in the next iteration's column pass (``dm1_fourier_iter``, the last one in
+the run's tail).  The real-space path (PTY_DM_FOURIER=0, or any other DM
+method) stores f itself.  The two
 are the same algorithm and differ by rounding.  DM is not a contraction: a
 one-ulp difference grows about 1.5x per iteration (the background state, of
 small values, faster), so the one-step results must agree to 1e-6 (rel L2)
@@ -44,7 +47,7 @@ class _Env:
             os.environ["PTY_DM_FOURIER"] = self.old
 
 
-def _setup(s, dev, bg):
+def _setup(s, dev, bg, graphs=False):
     N, M, K = s.I.shape
     U, V = s.O.shape
     eng = Engine(N, M, K, U, V, s.R.astype(np.int32), device=dev)
@@ -53,22 +56,25 @@ def _setup(s, dev, bg):
     O_t = torch.from_numpy(broadband_start((U, V)).astype(np.complex64)).to(dev)
     P_t = torch.from_numpy(s.P.astype(np.complex64)).to(dev)
     eng.set_state(O_t, P_t)
-    eng.set_background(torch.from_numpy(bg.astype(np.float32)), per_frame=False)
+    if bg is not None:
+        eng.set_background(torch.from_numpy(bg.astype(np.float32)), per_frame=False)
+    eng.set_graphs(graphs)
     return eng, O_t, amp
 
 
-def _run(s, dev, bg, plan):
+def _run(s, dev, bg, plan, method=4, graphs=False):
     """plan: list of (iterations, real-space path?) runs, the first with begin."""
-    eng, O_t, amp = _setup(s, dev, bg)
+    eng, O_t, amp = _setup(s, dev, bg if method == 4 else None, graphs)
     try:
         rows, Os, it = [], [], 0
         for k, (n, real) in enumerate(plan):
             with _Env(real):
-                rows.append(eng.run(1, 4, n, iter0=it, begin=(k == 0)))
+                rows.append(eng.run(1, method, n, iter0=it, begin=(k == 0)))
             it += n
             Os.append(O_t.cpu().numpy())
-        f, b = eng.get_iterate(f=True, background=True)
-        return np.concatenate(rows), Os, f.cpu().numpy(), b.cpu().numpy()
+        f, b = eng.get_iterate(f=True, background=method == 4)
+        return (np.concatenate(rows), Os, f.cpu().numpy(),
+                None if b is None else b.cpu().numpy())
     finally:
         eng.close()
 
@@ -114,3 +120,59 @@ def test_switching_paths_mid_run(gpu_device):
     print(f"object {rel_l2(Oa[-1], Ob[-1]):.1e} f {rel_l2(fa, fb):.1e}")
     assert rel_l2(Oa[-1], Ob[-1]) < 1e-5 and rel_l2(fa, fb) < 1e-5
     assert np.allclose(ma[:, 0], mb[:, 0], rtol=1e-5)
+
+
+# ---- method 1 ---------------------------------------------------------------
+
+@pytest.mark.parametrize("frame,grid,step", CASES)
+def test_method1_spectral_and_real_space_paths_agree(frame, grid, step, gpu_device):
+    """Every iteration's metrics (the model error formed one iteration later
+    or in the tail) and the objects after 1 and 2 iterations on the
+    illuminated region (sum |P|^2 >= 1e-6 of its maximum).  Bar per
+    iteration: max(1e-5, 10 x the difference between the complex64 and the
+    complex128 oracle runs from the same start) -- the rounding two complex64
+    evaluations of the same step may differ by; without background the
+    weakly lit border divides round-off by a small heatmap."""
+    import oracle
+    print(f"DM method 1 spectra vs real space, frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=grid, step=step, jitter=2)
+    H = oracle.make_P_heatmap(s.P, s.R - s.R.max(axis=0), s.O.shape)
+    region = H >= 1e-6 * H.max()
+    O0 = broadband_start(s.O.shape).astype(np.complex64)
+    ref = {}
+    for dt in ("double", "single"):
+        tr = []
+        oracle.DM(s.I, s.R.copy(), s.P.copy(), O0.astype(np.complex128), 2, method=1, dtype=dt,
+                  callback=lambda i, O_, P_: tr.append(O_.copy()))
+        ref[dt] = tr
+    bound = [max(1e-5, 10 * rel_l2(a[region], b[region]))
+             for a, b in zip(ref["single"], ref["double"])]
+    ma, Oa, fa, _ = _run(s, gpu_device, None, [(1, False), (1, False)], method=1)
+    mb, Ob, fb, _ = _run(s, gpu_device, None, [(1, True), (1, True)], method=1)
+    errs = [rel_l2(x[region], y[region]) for x, y in zip(Oa, Ob)]
+    print("object rel L2 after 1 / 2:", " ".join(f"{e:.1e}" for e in errs),
+          "bound", " ".join(f"{e:.1e}" for e in bound), f"f {rel_l2(fa, fb):.1e}")
+    assert errs[0] < 1e-6 and errs[1] < bound[1] and rel_l2(fa, fb) < 1e-5
+    assert np.allclose(ma[:, 0], mb[:, 0], rtol=1e-5)
+    assert np.allclose(ma[:, 1], mb[:, 1], rtol=1e-4)
+    assert np.allclose(ma[:, 2], mb[:, 2], rtol=1e-4)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_method1_split_runs_equal_one_run(graphs, gpu_device):
+    """The run's tail writes the last metrics; the next run rebuilds O from
+    the spectra through the same kernels: 2 + 4 iterations = 6 bit for bit,
+    eager and with graph replay."""
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=2)
+    ma, Oa, fa, _ = _run(s, gpu_device, None, [(6, False)], method=1, graphs=graphs)
+    mb, Ob, fb, _ = _run(s, gpu_device, None, [(2, False), (4, False)], method=1, graphs=graphs)
+    assert np.array_equal(Oa[-1], Ob[-1])
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(ma, mb)
+
+
+def test_method1_graphs_equal_eager(gpu_device):
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=2)
+    ma, Oa, fa, _ = _run(s, gpu_device, None, [(5, False)], method=1, graphs=False)
+    mb, Ob, fb, _ = _run(s, gpu_device, None, [(5, False)], method=1, graphs=True)
+    assert np.array_equal(Oa[-1], Ob[-1]) and np.array_equal(fa, fb) and np.array_equal(ma, mb)
