@@ -434,7 +434,7 @@ def main():
 
     # ---- difference map at the same shape: with background refinement
     # (method 4, the workload's own algorithm), object-only (method 1) and
-    # object + probe with OP_iters (1, 3) (method 3) ----
+    # object + probe (method 3) with OP_iters (1, 1) and (1, 3) ----
     def dm_timing(method, OP_iters=(1, 1)):
         eng = Engine(N, M, M, U, V, scan["R"], device=dev)
         if args.path != "default" or args.frame_threads or args.frame_slots:
@@ -482,12 +482,14 @@ def main():
         per4, e4 = dm_timing(4)
         per1, e1 = dm_timing(1)
         per3, e3 = dm_timing(3, (1, 3))
+        per31, e31 = dm_timing(3, (1, 1))
         dm = {"dm_bg_iters_per_s": 1.0 / per4, "dm_bg_ms_per_step": 1e3 * per4,
               "dm_bg_roofline_frac": (DM4_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per4 / (HBM_PEAK_GBS * 1e9),
               "dm_bg_final_eMod": e4,
               "dm1_iters_per_s": 1.0 / per1, "dm1_ms_per_step": 1e3 * per1,
               "dm1_roofline_frac": (DM1_BYTES_PX * px + OBJ_BYTES_PX * U * V) / per1 / (HBM_PEAK_GBS * 1e9),
               "dm3_op13_iters_per_s": 1.0 / per3, "dm3_op13_ms_per_step": 1e3 * per3,
+              "dm3_iters_per_s": 1.0 / per31, "dm3_ms_per_step": 1e3 * per31,
               "repeats": max(1, args.repeats), "timing": "median of the repeats"}
         dm.update(era3 or {})
 

@@ -90,7 +90,7 @@ constexpr int ESUP_BLOCKS = 1024;
 // host-side state an iteration changes; a graph replay restores what its
 // capture left
 struct HostIterState {
-  bool H_valid, G_valid, era_bg_shared, have_metric_state, os_next;
+  bool H_valid, G_valid, era_bg_shared, have_metric_state, os_next, os_next_op;
   float2* F;
   float2* X;
   bool dm1_pending;
@@ -232,6 +232,10 @@ struct pty_ctx {
   // Os holds the next object-only DM projection of f (the last iteration's
   // metric projection, with the same f, P and heatmap); cleared by every run
   bool os_next = false;
+  // Os, Ps hold the next object + probe projection (OP_iters (1, 1), no
+  // centring, no Pmod_P: the metric projection with the iterate f the next
+  // iteration projects, from the same O, P; its heatmaps were stored)
+  bool os_next_op = false;
   bool have_metric_state = false;
   int last_algo = -1;
 
@@ -910,7 +914,7 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
 // one DM iteration (dm.py:45-160 and :169-294)
 // --------------------------------------------------------------------------
 int dm_project(pty_ctx* c, const pty_opts& o, int64_t it, int upd, bool op_step, float2* Ob,
-               float2* Pb, bool metric) {
+               float2* Pb, bool metric, bool store = false) {
   if (upd == 0) {
     PTY_TRY(gather(c, c->F, Pb, nullptr, Ob, c->H, !c->H_valid, 1, o.alpha, false, false));
     c->H_valid = true;
@@ -918,10 +922,13 @@ int dm_project(pty_ctx* c, const pty_opts& o, int64_t it, int upd, bool op_step,
     PTY_TRY(probe_update(c, c->F, Ob, nullptr, Pb, c->G, !c->G_valid, 1, o.alpha, false));
     c->G_valid = true;
   } else if (op_step) {
+    // (a metric projection that the next iteration reuses stores its heatmaps
+    // as that iteration's own projection would)
+    const int st = (metric && !store) ? 0 : 1;
     for (int r = 0; r < o.op_n; ++r) {
-      PTY_TRY(gather(c, c->F, Pb, nullptr, Ob, c->H, 1, metric ? 0 : 1, o.alpha, false, false));
-      PTY_TRY(probe_update(c, c->F, Ob, nullptr, Pb, c->G, 1, metric ? 0 : 1, o.alpha, false));
-      if (!metric) c->H_valid = c->G_valid = true;
+      PTY_TRY(gather(c, c->F, Pb, nullptr, Ob, c->H, 1, st, o.alpha, false, false));
+      PTY_TRY(probe_update(c, c->F, Ob, nullptr, Pb, c->G, 1, st, o.alpha, false));
+      if (st) c->H_valid = c->G_valid = true;
     }
     if (o.probe_centering) {
       PTY_TRY(centre_probe(c, Pb, Ob, !metric));
@@ -971,12 +978,22 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
 
   // object-only projections: upd 0, or upd 2 away from its OP iterations
   const bool plain_proj = upd == 0 || (upd == 2 && !op_step);
+  // an object + probe projection that the next iteration would repeat: one
+  // pass per iteration, no centring, no Pmod_P on this iteration's metric probe
+  const bool op_reuse = !bgm && upd == 2 && o.op_every == 1 && o.op_n == 1 && !o.probe_centering;
   if (!bgm && plain_proj && c->os_next) {
     // dm.py:85 / :130: the projection of this f with P and the heatmap is the
     // one the previous iteration's metric step computed into Os (f, P and the
     // heatmap have not changed since)
     StageTimer tm(c, PTY_STAGE_MISC);
     PTY_TRY(copy_d2d(c, c->O, c->Os, UV * sizeof(float2)));
+  } else if (op_reuse && op_step && c->os_next_op) {
+    // dm.py:65-82: the object + probe projection of this f from this O, P is
+    // the previous iteration's metric projection (dm.py:104-130), heatmaps
+    // included
+    StageTimer tm(c, PTY_STAGE_MISC);
+    PTY_TRY(copy_d2d(c, c->O, c->Os, UV * sizeof(float2)));
+    PTY_TRY(copy_d2d(c, c->P, c->Ps, MK * sizeof(float2)));
   } else if (!fused_proj) {
     PTY_TRY(dm_project(c, o, it, upd, op_step, c->O, c->P, false));
   }
@@ -1019,10 +1036,11 @@ int dm_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     PTY_TRY(copy_d2d(c, c->Os, c->O, UV * sizeof(float2)));
     PTY_TRY(copy_d2d(c, c->Ps, c->P, MK * sizeof(float2)));
   }
-  PTY_TRY(dm_project(c, o, it, upd, op_step, c->Os, c->Ps, true));
+  PTY_TRY(dm_project(c, o, it, upd, op_step, c->Os, c->Ps, true, op_reuse && op_step && !pmodp));
   // a plain metric projection used P (Ps == P) and the heatmap; pmod_probe
   // below changes only Ps, and P changes only on OP iterations
   c->os_next = !bgm && plain_proj;
+  c->os_next_op = op_reuse && op_step && !pmodp;
   if (pmodp) PTY_TRY(pmod_probe(c, c->Ps, o.alpha));
   if (bgm) PTY_TRY(bg_mean(c, nchunks, c->bbar_m));
   int ncol = 0;
@@ -1243,7 +1261,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(upd == 2 && (it % o.op_every) == 0), (uint64_t)(it < o.pmod_probe_until),
           (uint64_t)o.probe_centering, bits((double)o.alpha), bits(c->I_norm),
           (uint64_t)c->H_valid, (uint64_t)c->G_valid, (uint64_t)c->era_bg_shared,
-          (uint64_t)c->os_next,
+          (uint64_t)c->os_next, (uint64_t)c->os_next_op,
           (uint64_t)c->use_frame, (uint64_t)c->frame_nt, (uint64_t)c->frame_slots,
           (uint64_t)c->rows_mult, (uint64_t)c->cols_mult, (uint64_t)c->cols_wg,
           (uint64_t)c->chunk_frames, (uint64_t)c->rowg,
@@ -1256,7 +1274,8 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
 }
 
 HostIterState save_state(const pty_ctx* c) {
-  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->os_next, c->F, c->X,
+  return {c->H_valid, c->G_valid, c->era_bg_shared, c->have_metric_state, c->os_next,
+          c->os_next_op, c->F, c->X,
           c->dm1_pending, c->dm1_par, c->f_spec};
 }
 void load_state(pty_ctx* c, const HostIterState& st) {
@@ -1265,6 +1284,7 @@ void load_state(pty_ctx* c, const HostIterState& st) {
   c->era_bg_shared = st.era_bg_shared;
   c->have_metric_state = st.have_metric_state;
   c->os_next = st.os_next;
+  c->os_next_op = st.os_next_op;
   c->F = st.F;
   c->X = st.X;
   c->dm1_pending = st.dm1_pending;
@@ -1459,6 +1479,7 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
     if (begin || !c->Pn_valid) PTY_TRY(build_subpixel_probes(c));
   }
   c->os_next = false;   // the caller may have changed O, P or the data since
+  c->os_next_op = false;
   if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
   if (iters > c->metrics_cap) {
     dev_free(c->metrics);
@@ -1975,6 +1996,7 @@ int pty_position_terms(pty_ctx* c, double* terms_host) {
                          hipMemcpyDeviceToHost, c->stream));
   PTY_HIP(hipStreamSynchronize(c->stream));
   c->os_next = false;   // X no longer holds what a DM projection reuse expects
+  c->os_next_op = false;
   return PTY_OK;
 }
 
@@ -2447,7 +2469,7 @@ int pty_set_heatmap(pty_ctx* c, const double* H, const double* G) {
   if (G) PTY_TRY(copy_d2d(c, c->G, G, c->MK() * sizeof(double)));
   c->H_valid = H != nullptr;
   c->G_valid = G != nullptr;
-  c->os_next = false;
+  c->os_next = c->os_next_op = false;
   PTY_HIP(hipStreamSynchronize(c->stream));
   return PTY_OK;
 }
@@ -2467,7 +2489,7 @@ int pty_set_iterate(pty_ctx* c, const pty_cfloat* f, const pty_cfloat* O_ref, co
   PTY_TRY(copy_d2d(c, c->Obak, O_ref ? (const void*)O_ref : (const void*)c->O, UV * sizeof(float2)));
   PTY_TRY(copy_d2d(c, c->Pbak, P_ref ? (const void*)P_ref : (const void*)c->P, c->MK() * sizeof(float2)));
   c->have_metric_state = false;
-  c->os_next = false;
+  c->os_next = c->os_next_op = false;
   PTY_HIP(hipStreamSynchronize(c->stream));
   return PTY_OK;
 }

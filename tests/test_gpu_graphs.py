@@ -107,7 +107,11 @@ def test_non_finite_data_raise(gpu_device):
     # two non-OP iterations in a row (the reuse happens), with the centring
     # of the OP iterations resetting the heatmaps and the far-field probe
     # constraint changing the metric probe only
-    (3, (1, 3), True, 5), (3, (2, 3), True, 5), (1, (1, 1), False, 5)])
+    (3, (1, 3), True, 5), (3, (2, 3), True, 5), (1, (1, 1), False, 5),
+    # object + probe every iteration: each OP projection is the previous
+    # iteration's metric projection (heatmaps included), also around the end
+    # of a Pmod_probe window
+    (3, (1, 1), False, 0), (3, (1, 1), False, 4)])
 def test_dm_projection_reuse_is_exact(method, op, centre, pmodp, graphs, gpu_device):
     # one run of 8 iterations starts each object-only projection from the
     # previous iteration's metric projection; 8 runs of one iteration never do
