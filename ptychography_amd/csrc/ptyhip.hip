@@ -208,7 +208,9 @@ struct pty_ctx {
   float dmf_alpha = 0.f;     // alpha of the leading step in Xo
   // DM method 1 on spectra: the metric of iteration i is the model error of
   // O_{i+1} P, formed in iteration i + 1's column pass (or by the run's tail)
-  bool dm1_run = false;      // DM method 1 runs on spectra from iteration pmod_probe_until on
+  bool dm1_run = false;      // DM method 1 (or 3, see dm1_op) runs on spectra from iteration
+                             // pmod_probe_until on
+  bool dm1_op = false;       // ... method 3, object + probe every iteration
   bool dm1_pending = false;  // Os awaits its model error; its metrics are not yet written
   int dm1_par = 0;           // which step-size partial buffer the last step wrote
   int dm1_nstep = 0;         // partials per step
@@ -1175,14 +1177,19 @@ int dm_bg_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metric
 //     previous Os; iteration i's metrics are written once iteration i + 1
 //     (or the run's tail, dm1_tail) has formed the model error of Os P.
 // Per frame pixel: 8 + 36 (rows forward, column pass) + 8 (gather) = 52
-// bytes, against 88 on f in real space.
+// bytes, against 88 on f in real space.  Method 3 with an object + probe
+// step every iteration (dm1_op): O, P = the previous Os, Ps (the metric
+// projection is the next projection); the gather also writes f in real
+// space (8) for the probe update with Os (8): 68 bytes against 96.
 int dm1_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
   const size_t MK = c->MK(), UV = (size_t)c->U * c->V;
   const int nrows = c->N * c->M;
   (void)it;
+  const bool op = c->dm1_op;
   if (c->dm1_pending) {
     StageTimer tm(c, PTY_STAGE_MISC);
     PTY_TRY(copy_d2d(c, c->O, c->Os, UV * sizeof(float2)));
+    if (op) PTY_TRY(copy_d2d(c, c->P, c->Ps, MK * sizeof(float2)));
   }
   PTY_TRY(rows_fwd<SRC_OP>(c, c->O, c->P, nullptr, c->X, nrows));
   const int par = c->dm1_pending ? 1 - c->dm1_par : c->dm1_par;
@@ -1209,19 +1216,33 @@ int dm1_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_
     rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
     rg.p[S_ESUP_E] = c->dm1_esup[c->dm1_par];
     rg.n[S_ESUP_E] = c->dm1_nstep;
+    if (op) {
+      rg.p[S_ECON_PN] = c->part_econP;
+      rg.p[S_ECON_PD] = c->part_econP + ECON_BLOCKS;
+      rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = ECON_BLOCKS;
+    }
     PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, c->use_slot ? metrics_out : metrics_out - 3));
   }
   c->dm1_par = par;
   c->dm1_nstep = ncol;
-  if (!c->H_valid) {
+  if (!c->H_valid || op) {
+    // (object + probe: the heatmap of this iteration's probe, dm.py:104-130)
     PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o.alpha, false, false));
     c->H_valid = true;
   }
-  PTY_TRY(rows_gather(c, c->X, o.alpha, false, false, nullptr, nullptr, c->Os));
+  // Os = Psup(f); object + probe: the gather also writes f in real space for
+  // the probe update with Os, Ps (dm.py:104-130)
+  PTY_TRY(rows_gather(c, c->X, o.alpha, false, false, nullptr, op ? c->Xo : nullptr, c->Os));
+  if (op) {
+    PTY_TRY(probe_update(c, c->Xo, c->Os, nullptr, c->Ps, c->G, 1, 1, o.alpha, false));
+    c->G_valid = true;
+  }
   PTY_TRY(econ_pass(c, c->Os, c->Obak, UV, c->part_econO));
+  if (op) PTY_TRY(econ_pass(c, c->Ps, c->Pbak, MK, c->part_econP));
   {
     StageTimer tm(c, PTY_STAGE_MISC);
     PTY_TRY(copy_d2d(c, c->Obak, c->Os, UV * sizeof(float2)));
+    if (op) PTY_TRY(copy_d2d(c, c->Pbak, c->Ps, MK * sizeof(float2)));
   }
   c->dm1_pending = true;
   c->have_metric_state = true;
@@ -1232,7 +1253,7 @@ int dm1_fourier_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_
 int dm1_tail(pty_ctx* c, double* metrics_out) {
   if (!c->dm1_pending) return PTY_OK;
   const size_t MK = c->MK();
-  PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->P, nullptr, c->X, c->N * c->M));
+  PTY_TRY(rows_fwd<SRC_OP>(c, c->Os, c->dm1_op ? c->Ps : c->P, nullptr, c->X, c->N * c->M));
   int ncol = 0;
   ColArgs a = col_args(c);
   a.X = c->X;
@@ -1248,6 +1269,11 @@ int dm1_tail(pty_ctx* c, double* metrics_out) {
   rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ECON_BLOCKS;
   rg.p[S_ESUP_E] = c->dm1_esup[c->dm1_par];
   rg.n[S_ESUP_E] = c->dm1_nstep;
+  if (c->dm1_op) {
+    rg.p[S_ECON_PN] = c->part_econP;
+    rg.p[S_ECON_PD] = c->part_econP + ECON_BLOCKS;
+    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = ECON_BLOCKS;
+  }
   PTY_TRY(metrics_launch(c, rg, ESUP_EXPLICIT, metrics_out));
   c->dm1_pending = false;
   return PTY_OK;
@@ -1270,7 +1296,8 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
           (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp,
           (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->dmf_run, (uint64_t)c->dm1_run,
-          (uint64_t)c->dm1_pending, (uint64_t)c->dm1_par, (uint64_t)c->f_spec};
+          (uint64_t)c->dm1_pending, (uint64_t)c->dm1_par, (uint64_t)c->f_spec,
+          (uint64_t)c->dm1_op};
 }
 
 HostIterState save_state(const pty_ctx* c) {
@@ -1381,8 +1408,15 @@ bool dmf_disabled() {
 // DM method 1 entering the spectral path from f in real space: the
 // iteration's O = Psup(f) (dm.py:85), then F = FFT2(f) in place
 int dm1_to_spectra(pty_ctx* c, float alpha) {
-  PTY_TRY(gather(c, c->F, c->P, nullptr, c->O, c->H, !c->H_valid, 1, alpha, false, false));
-  c->H_valid = true;
+  if (c->dm1_op) {
+    // dm.py:65-82: object with the current probe, then probe with the new object
+    PTY_TRY(gather(c, c->F, c->P, nullptr, c->O, c->H, 1, 1, alpha, false, false));
+    PTY_TRY(probe_update(c, c->F, c->O, nullptr, c->P, c->G, 1, 1, alpha, false));
+    c->H_valid = c->G_valid = true;
+  } else {
+    PTY_TRY(gather(c, c->F, c->P, nullptr, c->O, c->H, !c->H_valid, 1, alpha, false, false));
+    c->H_valid = true;
+  }
   PTY_TRY(copy_d2d(c, c->Ps, c->P, c->MK() * sizeof(float2)));
   PTY_TRY(rows_fwd<SRC_F>(c, nullptr, nullptr, c->F, c->F, c->N * c->M));
   ColArgs a = col_args(c);
@@ -1512,12 +1546,14 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
   // (iterations inside a Pmod_probe window take the real-space path; the
   // first iteration after it switches, so the split of iterations into runs
   // does not change the result)
-  c->dm1_run = o->algorithm == 1 && o->method == 1 && c->rowg && !dmf_disabled() && !c->dmf_run &&
-               iters > 0 && o->pmod_probe_until < iter0 + iters;
+  c->dm1_op = o->method == 3 && o->op_every == 1 && o->op_n == 1 && !o->probe_centering;
+  c->dm1_run = o->algorithm == 1 && (o->method == 1 || c->dm1_op) && c->rowg && !dmf_disabled() &&
+               !c->dmf_run && iters > 0 && o->pmod_probe_until < iter0 + iters;
   c->dm1_pending = false;
   if (c->dm1_run) {
     for (int k = 0; k < 2; ++k)
       if (!c->dm1_esup[k]) PTY_TRY(dev_alloc(&c->dm1_esup[k], (size_t)c->cap_col * MAX_CHUNKS));
+    if (c->dm1_op && !c->Xo) PTY_TRY(dev_alloc(&c->Xo, c->NMK()));   // f in real space
   }
   if (o->algorithm == 1) {
     if (c->dm1_run && o->pmod_probe_until <= iter0) {
@@ -1532,11 +1568,15 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
         PTY_TRY(copy_d2d(c, c->X, c->F, c->NMK() * sizeof(float2)));
         a.X = c->X;
         PTY_TRY(cols<COL_INV>(c, a));
-        if (!c->H_valid) {
+        if (!c->H_valid || c->dm1_op) {
           PTY_TRY(gather(c, nullptr, c->P, nullptr, nullptr, c->H, 1, 1, o->alpha, false, false));
           c->H_valid = true;
         }
-        PTY_TRY(rows_gather(c, c->X, o->alpha, false, false, nullptr));
+        PTY_TRY(rows_gather(c, c->X, o->alpha, false, false, nullptr, c->dm1_op ? c->Xo : nullptr));
+        if (c->dm1_op) {
+          PTY_TRY(probe_update(c, c->Xo, c->O, nullptr, c->P, c->G, 1, 1, o->alpha, false));
+          c->G_valid = true;
+        }
       } else {
         PTY_TRY(dm1_to_spectra(c, o->alpha));
       }

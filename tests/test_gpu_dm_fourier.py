@@ -47,24 +47,25 @@ class _Env:
             os.environ["PTY_DM_FOURIER"] = self.old
 
 
-def _setup(s, dev, bg, graphs=False):
+def _setup(s, dev, bg, graphs=False, P0=None):
     N, M, K = s.I.shape
     U, V = s.O.shape
     eng = Engine(N, M, K, U, V, s.R.astype(np.int32), device=dev)
     amp, I_norm = eng.prepare_amp(torch.from_numpy(s.I.astype(np.float32)), None)
     eng.set_data(amp, None, I_norm)
     O_t = torch.from_numpy(broadband_start((U, V)).astype(np.complex64)).to(dev)
-    P_t = torch.from_numpy(s.P.astype(np.complex64)).to(dev)
+    P_t = torch.from_numpy((s.P if P0 is None else P0).astype(np.complex64)).to(dev)
     eng.set_state(O_t, P_t)
     if bg is not None:
         eng.set_background(torch.from_numpy(bg.astype(np.float32)), per_frame=False)
     eng.set_graphs(graphs)
-    return eng, O_t, amp
+    return eng, O_t, P_t, amp
 
 
-def _run(s, dev, bg, plan, method=4, graphs=False):
-    """plan: list of (iterations, real-space path?) runs, the first with begin."""
-    eng, O_t, amp = _setup(s, dev, bg if method == 4 else None, graphs)
+def _run(s, dev, bg, plan, method=4, graphs=False, P0=None, Ps=None):
+    """plan: list of (iterations, real-space path?) runs, the first with begin;
+    the probe after each run goes to the list Ps when given."""
+    eng, O_t, P_t, amp = _setup(s, dev, bg if method == 4 else None, graphs, P0)
     try:
         rows, Os, it = [], [], 0
         for k, (n, real) in enumerate(plan):
@@ -72,6 +73,8 @@ def _run(s, dev, bg, plan, method=4, graphs=False):
                 rows.append(eng.run(1, method, n, iter0=it, begin=(k == 0)))
             it += n
             Os.append(O_t.cpu().numpy())
+            if Ps is not None:
+                Ps.append(P_t.cpu().numpy())
         f, b = eng.get_iterate(f=True, background=method == 4)
         return (np.concatenate(rows), Os, f.cpu().numpy(),
                 None if b is None else b.cpu().numpy())
@@ -176,3 +179,56 @@ def test_method1_graphs_equal_eager(gpu_device):
     ma, Oa, fa, _ = _run(s, gpu_device, None, [(5, False)], method=1, graphs=False)
     mb, Ob, fb, _ = _run(s, gpu_device, None, [(5, False)], method=1, graphs=True)
     assert np.array_equal(Oa[-1], Ob[-1]) and np.array_equal(fa, fb) and np.array_equal(ma, mb)
+
+
+# ---- method 3 (object + probe every iteration) --------------------------------
+
+def _probe_start(s):
+    rng = np.random.default_rng(17)
+    return s.P * (1.0 + 0.2 * rng.standard_normal(s.P.shape))
+
+
+@pytest.mark.parametrize("frame,grid,step", CASES)
+def test_method3_spectral_and_real_space_paths_agree(frame, grid, step, gpu_device):
+    """As for method 1, with the probe (whole array) under the same bar."""
+    import oracle
+    print(f"DM method 3 spectra vs real space, frame {frame}", flush=True)
+    s = small_scan(frame=frame, grid=grid, step=step, jitter=2)
+    P0 = _probe_start(s)
+    H = oracle.make_P_heatmap(s.P, s.R - s.R.max(axis=0), s.O.shape)
+    region = H >= 1e-6 * H.max()
+    O0 = broadband_start(s.O.shape).astype(np.complex64)
+    ref = {}
+    for dt in ("double", "single"):
+        tr = []
+        oracle.DM(s.I, s.R.copy(), P0.astype(np.complex64).astype(np.complex128),
+                  O0.astype(np.complex128), 2, method=3, dtype=dt,
+                  callback=lambda i, O_, P_: tr.append((O_.copy(), P_.copy())))
+        ref[dt] = tr
+    bO = [max(1e-5, 10 * rel_l2(a[0][region], b[0][region])) for a, b in zip(ref["single"], ref["double"])]
+    bP = [max(1e-5, 10 * rel_l2(a[1], b[1])) for a, b in zip(ref["single"], ref["double"])]
+    Pa, Pb = [], []
+    ma, Oa, fa, _ = _run(s, gpu_device, None, [(1, False), (1, False)], method=3, P0=P0, Ps=Pa)
+    mb, Ob, fb, _ = _run(s, gpu_device, None, [(1, True), (1, True)], method=3, P0=P0, Ps=Pb)
+    eO = [rel_l2(x[region], y[region]) for x, y in zip(Oa, Ob)]
+    eP = [rel_l2(x, y) for x, y in zip(Pa, Pb)]
+    print("object", " ".join(f"{e:.1e}" for e in eO), "bound", " ".join(f"{e:.1e}" for e in bO))
+    print("probe ", " ".join(f"{e:.1e}" for e in eP), "bound", " ".join(f"{e:.1e}" for e in bP),
+          f"f {rel_l2(fa, fb):.1e}")
+    assert eO[0] < 1e-6 and eP[0] < 1e-6
+    assert eO[1] < bO[1] and eP[1] < bP[1] and rel_l2(fa, fb) < 1e-5
+    assert np.allclose(ma[:, 0], mb[:, 0], rtol=1e-5)
+    assert np.allclose(ma[:, 1:], mb[:, 1:], rtol=1e-4)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_method3_split_runs_equal_one_run(graphs, gpu_device):
+    s = small_scan(frame=64, grid=(5, 5), step=12, jitter=2)
+    P0 = _probe_start(s)
+    Pa, Pb = [], []
+    ma, Oa, fa, _ = _run(s, gpu_device, None, [(6, False)], method=3, graphs=graphs, P0=P0, Ps=Pa)
+    mb, Ob, fb, _ = _run(s, gpu_device, None, [(2, False), (4, False)], method=3, graphs=graphs,
+                         P0=P0, Ps=Pb)
+    assert np.array_equal(Oa[-1], Ob[-1]) and np.array_equal(Pa[-1], Pb[-1])
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(ma, mb)

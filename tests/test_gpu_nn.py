@@ -29,8 +29,9 @@ def bf(x, dev):
 
 
 def rel(a, b):
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a, b = a.astype(dt), b.astype(dt)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
