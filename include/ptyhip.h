@@ -217,7 +217,9 @@ int pty_tune(pty_ctx* ctx, const pty_opts* opts, int32_t trial_iters, pty_tune_r
 /* ---- several scans at once (no counterpart in the reference) ----
  * The contexts (prepared with pty_set_data / pty_set_state / ...) stay owned
  * by the caller and must outlive the batch.  Each scan's kernels run on one
- * of up to 8 lane streams so scans of different sizes overlap; with graphs
+ * of up to 4 lane streams (GPU_MAX_HW_QUEUES, the process's hardware queues,
+ * when set; PTY_BATCH_LANES overrides, at most 8) so scans of different
+ * sizes overlap; with graphs
  * on, one iteration of every scan is captured as a single hipGraph (fork /
  * join over the lanes) per distinct iteration kind and replayed.  Per scan,
  * results are identical to pty_run on that context alone. */

@@ -1708,7 +1708,7 @@ int noalloc_setup(int dev, NoallocDevice** out) {
 }  // namespace
 
 // Several scans advanced together: each scan's kernels go to one of up to
-// eight lane streams (default priority), so small scans fill the device
+// four lane streams (default priority), so small scans fill the device
 // side by side; with graphs on, one iteration of every scan is captured as one
 // hipGraph (fork / join through events) and replayed.
 struct pty_batch {
@@ -2280,7 +2280,12 @@ int pty_batch_create(pty_batch** out, int32_t n, pty_ctx* const* ctxs) {
   }
   pty_batch* b = new pty_batch();
   b->ctx.assign(ctxs, ctxs + n);
-  const int nl = std::min(n, 8);
+  // lanes: as many as the process's hardware queues (GPU_MAX_HW_QUEUES,
+  // 4 by default), more would only alias onto them; PTY_BATCH_LANES overrides
+  int maxl = 4;
+  if (const char* e_q = getenv("GPU_MAX_HW_QUEUES")) maxl = std::max(1, std::min(8, atoi(e_q)));
+  if (const char* e_l = getenv("PTY_BATCH_LANES")) maxl = std::max(1, std::min(8, atoi(e_l)));
+  const int nl = std::min(n, maxl);
   b->lanes.resize(nl, nullptr);
   b->join.resize(nl, nullptr);
   hipError_t e = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking);
