@@ -228,9 +228,31 @@ int pty_batch_create(pty_batch** out, int32_t n, pty_ctx* const* ctxs);
 int pty_batch_destroy(pty_batch* batch);
 int pty_batch_set_graphs(pty_batch* batch, int32_t enable);
 int pty_batch_get_graphs(pty_batch* batch, int32_t* enabled, int32_t* fell_back);
-/* opts: n entries; metrics_host: n pointers to 3 * iters doubles (or NULL). */
+/* opts: n entries; metrics_host: n pointers to 3 * S * iters doubles (S = the
+ * context's packed scans, else 1; or NULL). */
 int pty_batch_run(pty_batch* batch, const pty_opts* opts, int64_t iter0, int32_t iters,
                   int32_t begin, double* const* metrics_host);
+
+/* Packed scans (no counterpart in the reference): S scans of one frame size
+ * in one context, so that each stage of an iteration is one launch over all
+ * their frames.  The context is created over the concatenated frames (scan s
+ * owns frames [frame_start[s], frame_start[s + 1])) and a packed object whose
+ * row band [row_start[s], row_start[s + 1]) is scan s's object (bands start
+ * on multiples of 16 rows, row_start[S] = U; each frame's window lies in its
+ * scan's band).  Call it before pty_prepare_amp / pty_set_data (it unbinds
+ * the data); after it, mask arguments are [S][M][K], the probe bound by
+ * pty_set_state is [S][M][K] (scan s's probe at s M K), pty_prepare_amp
+ * writes S values of I_norm, and pty_run writes 3 * S doubles per iteration
+ * (iteration-major, eMod / eCon / eSup of each scan).  Runs take ERA methods
+ * 1-3 without background, centring or Pmod_probe, method 3 with OP_iters
+ * (1, 1), integer positions, staged kernels; anything else is PTY_EUNSUP.
+ * I_norm (S values) normalises each scan's metrics; NULL: pty_prepare_amp
+ * computes them.  Per scan the object and probe equal a context of that scan
+ * alone bit for bit (the same sums in the same order); the metrics agree to
+ * FP64 rounding.  S = 1 returns the context to a single scan (with I_norm[0]
+ * when given).  Synchronous. */
+int pty_set_scans(pty_ctx* ctx, int32_t S, const int32_t* frame_start_host,
+                  const int32_t* row_start_host, const double* I_norm_host);
 
 /* ---- per-stage timing (HIP events on the context stream) ---- */
 enum {

@@ -55,6 +55,7 @@ SIGNATURES = {
     "pty_set_background": (ctypes.c_int, [_vp, _vp, _i32]),
     "pty_set_pamp": (ctypes.c_int, [_vp, _vp]),
     "pty_set_subpixel": (ctypes.c_int, [_vp, _vp]),
+    "pty_set_scans": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp]),
     "pty_position_terms": (ctypes.c_int, [_vp, _vp]),
     "pty_run": (ctypes.c_int, [_vp, ctypes.POINTER(PtyOpts), _i64, _i32, _i32, _vp]),
     "pty_intensity": (ctypes.c_int, [_vp, _vp]),

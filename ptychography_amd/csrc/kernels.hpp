@@ -32,11 +32,21 @@ struct Geom {
   // probe Fourier-shifted by the position's fractional part) in place of the
   // shared P; null for integer positions
   const float2* Pn;
+  // packed scans: frame n belongs to scan fscan[n] and sees that scan's probe
+  // P + fscan[n] M K (and its mask); null for a single scan
+  const int* fscan;
 };
+
+// probe of frame n: the shared probe, or its scan's probe when scans are packed
+__device__ __forceinline__ const float2* frame_probe(const Geom& g, const float2* P, int n) {
+  return g.fscan ? P + (size_t)g.fscan[n] * g.M * g.K : P;
+}
 
 // the probe row frame row `row` (= n M + i) sees
 __device__ __forceinline__ const float2* probe_row(const Geom& g, const float2* P, int row, int i) {
-  return g.Pn ? g.Pn + (size_t)row * g.K : P + (size_t)i * g.K;
+  if (g.Pn) return g.Pn + (size_t)row * g.K;
+  if (g.fscan) P = frame_probe(g, P, row / g.M);
+  return P + (size_t)i * g.K;
 }
 
 // ---------------------------------------------------------------------------
@@ -317,6 +327,9 @@ struct ColArgs {
   float2* F;             // COL_DMBF(_LEAD): the DM iterate's spectra [N][M][K]
   float2* Xo;            // COL_DMBF(_LEAD): second inverse output [N][M][K]
   int nframes, chunk;
+  int item_parts;        // 1: one partial per item (at the item's index; the
+                         // launch then runs one item per work-group), so the
+                         // partials of packed scans are contiguous per scan
   const float2* __restrict__ tw;
 };
 
@@ -446,7 +459,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     const uint32_t toff = (uint32_t)(t * K + c);
     const int n0 = chunk * a.chunk;
     const int n1 = min(a.nframes, n0 + a.chunk);
-    const uint32_t mbits = (HOIST && cv && a.maskbits) ? a.maskbits[t * K + c] : 0xffffffffu;
+    // packed scans: the mask of the item's scan (an item's frames share one:
+    // chunks of several frames only occur in background modes, never packed)
+    const size_t mscan = g.fscan ? (size_t)g.fscan[n0] : 0;
+    const uint32_t mbits =
+        (HOIST && cv && a.maskbits) ? a.maskbits[mscan * T * K + t * K + c] : 0xffffffffu;
     // per-chunk background sums (at most 16 frames): FP32 registers in the
     // hoisted modes (relative rounding <= 16 * 2^-24), written out in FP64
     using BAcc = std::conditional_t<HOIST, float, double>;
@@ -512,7 +529,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 m = ((mbits >> e) & 1u) ? 1.f : 0.f;
                 am = amh[e];
               } else {
-                m = a.mask ? (float)(a.mask + uo)[toff] : 1.f;
+                m = a.mask ? (float)(a.mask + mscan * L * K + uo)[toff] : 1.f;
                 am = (ampn + uo)[toff];
               }
               if constexpr (MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_PMODP) {
@@ -695,8 +712,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     if constexpr (MODE == COL_DM1F) acc[1] *= (double)a.scale;   // sum |f_new - f|^2
     block_sum<2>(acc, red);
     if (tid == 0) {
-      if (a.part) a.part[blockIdx.x] = acc[0];
-      if (a.part2) a.part2[blockIdx.x] = acc[1];
+      const int pi = a.item_parts ? wg : (int)blockIdx.x;
+      if (a.part) a.part[pi] = acc[0];
+      if (a.part2) a.part2[pi] = acc[1];
     }
   }
 }
@@ -764,7 +782,7 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
         const int j = v - o.y;
         const bool jin = b0f + f < cnt && (unsigned)j < (unsigned)K;
         const float2* Sn = a.S ? a.S + (size_t)n * M * K + (jin ? j : 0) : nullptr;
-        const float2* Pj = (g.Pn ? g.Pn + (size_t)n * M * K : a.P) + (jin ? j : 0);
+        const float2* Pj = (g.Pn ? g.Pn + (size_t)n * M * K : frame_probe(g, a.P, n)) + (jin ? j : 0);
 #pragma unroll
         for (int k = 0; k < PY; ++k) {
           const int i = u0 + ty + k * (WG / TILE_W) - o.x;
@@ -1049,6 +1067,8 @@ struct ProbeArgs {
   double* pg;        // [nchunks][M*K]  sum |O_n|^2 (fresh only)
   int fresh;
   int chunk;
+  const int* cstart;  // packed scans: chunk y takes frames [cstart[y], cstart[y + 1])
+                      // (chunks never straddle scans); null: frames of y * chunk
 };
 
 __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
@@ -1056,7 +1076,8 @@ __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
   const int p = blockIdx.x * WG + threadIdx.x;
   if (p >= MK) return;
   const int i = p / g.K, j = p % g.K;
-  const int n0 = blockIdx.y * a.chunk, n1 = min(g.N, n0 + a.chunk);
+  const int n0 = a.cstart ? a.cstart[blockIdx.y] : blockIdx.y * a.chunk;
+  const int n1 = a.cstart ? a.cstart[blockIdx.y + 1] : min(g.N, n0 + a.chunk);
   double nx = 0.0, ny = 0.0, gg = 0.0;
   // four frames' loads in flight per step; the sums take the frames in order
   constexpr int FB = 4;
@@ -1094,6 +1115,8 @@ struct ProbeFinalArgs {
   float2* Pout;
   double* part_econ;  // [2][nblocks]
   double* part_esup;  // [2][nblocks]  sum Re(conj(P) pnum), sum |P|^2 G (null = skip)
+  const int* sc;      // packed scans: scan y = blockIdx.y sums chunks [sc[y], sc[y + 1]) into
+                      // its own probe (Pold, Pout, G + y M K) and partials (+ y 2 nblocks)
 };
 
 // With part_esup, also the two probe-side terms of the eSup identity
@@ -1103,22 +1126,32 @@ __global__ __launch_bounds__(WG) void k_probe_final(Geom g, ProbeFinalArgs a) {
   __shared__ double red[16];
   const int MK = g.M * g.K;
   const int p = blockIdx.x * WG + threadIdx.x;
+  const int sy = blockIdx.y;
+  const int c0 = a.sc ? a.sc[sy] : 0, c1 = a.sc ? a.sc[sy + 1] : a.nchunks;
+  if (a.sc) {
+    const size_t so = (size_t)sy * MK;
+    if (a.G) a.G += so;
+    if (a.Pold) a.Pold += so;
+    a.Pout += so;
+    if (a.part_econ) a.part_econ += (size_t)sy * 2 * gridDim.x;
+    if (a.part_esup) a.part_esup += (size_t)sy * 2 * gridDim.x;
+  }
   double r[4] = {0.0, 0.0, 0.0, 0.0};
   if (p < MK) {
     double nx = 0.0, ny = 0.0, gg = 0.0;
     constexpr int CB = 4;  // chunks' loads in flight per step, summed in order
-    for (int cb = 0; cb < a.nchunks; cb += CB) {
+    for (int cb = c0; cb < c1; cb += CB) {
       double2 q[CB];
       double pgv[CB];
 #pragma unroll
       for (int f = 0; f < CB; ++f) {
-        const int c = min(cb + f, a.nchunks - 1);
+        const int c = min(cb + f, c1 - 1);
         q[f] = a.pnum[(size_t)c * MK + p];
         pgv[f] = a.fresh ? a.pg[(size_t)c * MK + p] : 0.0;
       }
 #pragma unroll
       for (int f = 0; f < CB; ++f) {
-        if (cb + f < a.nchunks) {
+        if (cb + f < c1) {
           nx += q[f].x;
           ny += q[f].y;
           if (a.fresh) gg += pgv[f];
@@ -1169,7 +1202,7 @@ __global__ __launch_bounds__(WG) void k_make_exits(Geom g, const float2* O, cons
     const int n = (int)(idx / MK);
     const int p = (int)(idx % MK), i = p / g.K, j = p % g.K;
     const int2 o = g.off[n];
-    out[idx] = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], g.Pn ? g.Pn[idx] : P[p]);
+    out[idx] = cmul(O[(size_t)(o.x + i) * g.V + o.y + j], g.Pn ? g.Pn[idx] : frame_probe(g, P, n)[p]);
   }
 }
 
@@ -1472,6 +1505,73 @@ __global__ __launch_bounds__(1024) void k_metrics(Regions rg, double I_norm, dou
     }
     metrics[2] = es;
   }
+}
+
+// Packed scans: where scan s's partials of one slot lie in p[k].
+enum RegionKind {
+  RK_COLS = 0,   // one per (frame, column block) item: [fstart[s] ncb, fstart[s + 1] ncb)
+  RK_TILES,      // one per 16 x 64 object tile: the tile rows of the scan's object rows
+  RK_ROWS,       // one per (object row, column segment): the scan's object rows x segs
+  RK_PROBE,      // probe reductions: [2 pb s, 2 pb s + pb) (p[k] already holds + pb for the
+                 // second half)
+  RK_ESUP,       // support-error pass per scan: [esup_blocks s, esup_blocks (s + 1))
+};
+
+struct ScanRegions {
+  const double* p[NSLOT];
+  int kind[NSLOT];
+  const int* fstart;      // [S + 1] first frame of each scan
+  const int* rstart;      // [S + 1] first object row of each scan (multiples of TILE_H)
+  const double* I_norm;   // [S] sum mask I of each scan
+  int S, ncb, tiles_x, segs, pb, esup_blocks;
+};
+
+// k_metrics for packed scans: one work-group takes the scans in turn and
+// writes metrics[3 (slot S + s) ..] (slot = the iteration's metrics slot).
+__global__ __launch_bounds__(1024) void k_metrics_scans(ScanRegions rg, double MK, int esup_mode,
+                                                        double* metrics, int* slot) {
+  __shared__ double red[16 * NSLOT];
+  int base = 0;
+  if (slot) base = *slot;   // read by every thread before thread 0 advances it below
+  for (int s = 0; s < rg.S; ++s) {
+    double v[NSLOT];
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+      v[k] = 0.0;
+      if (!rg.p[k]) continue;
+      int b = 0, n = 0;
+      switch (rg.kind[k]) {
+        case RK_COLS: b = rg.fstart[s] * rg.ncb; n = rg.fstart[s + 1] * rg.ncb - b; break;
+        case RK_TILES:
+          // (rounded up: only the last scan's end may not be a tile boundary)
+          b = (rg.rstart[s] + TILE_H - 1) / TILE_H * rg.tiles_x;
+          n = (rg.rstart[s + 1] + TILE_H - 1) / TILE_H * rg.tiles_x - b;
+          break;
+        case RK_ROWS: b = rg.rstart[s] * rg.segs; n = rg.rstart[s + 1] * rg.segs - b; break;
+        case RK_PROBE: b = 2 * rg.pb * s; n = rg.pb; break;
+        default: b = rg.esup_blocks * s; n = rg.esup_blocks; break;
+      }
+      v[k] = strided_sum(rg.p[k] + b, n);
+    }
+    block_sum<NSLOT>(v, red);
+    if (threadIdx.x == 0) {
+      double* m = metrics + 3 * ((size_t)base * rg.S + s);
+      const double In = rg.I_norm[s];
+      m[0] = sqrt(v[S_EMOD] / In);
+      const double den = v[S_ECON_OD] + v[S_ECON_PD];
+      m[1] = sqrt((v[S_ECON_ON] + v[S_ECON_PN]) / den);
+      double es = __builtin_nan("");
+      if (esup_mode == ESUP_EXPLICIT) {
+        es = sqrt(MK * v[S_ESUP_E] / In);
+      } else if (esup_mode == ESUP_IDENTITY) {
+        const double q = v[S_ESUP_A] - MK * (2.0 * v[S_ESUP_X] - v[S_ESUP_H]);
+        es = sqrt((q > 0.0 ? q : 0.0) / In);
+      }
+      m[2] = es;
+    }
+  }
+  __syncthreads();
+  if (slot && threadIdx.x == 0) *slot = base + 1;
 }
 
 // plain sum of one region into out[0]

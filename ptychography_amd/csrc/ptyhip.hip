@@ -84,6 +84,10 @@ void dev_free(T*& p) {
 
 constexpr int ECON_BLOCKS = 512;
 constexpr int ESUP_BLOCKS = 1024;
+// The staged ERA modulus step can run over at most MAX_CHUNKS frame blocks
+// (the column kernels' error partials are appended per block); the partial
+// buffers hold MAX_CHUNKS x the per-launch capacity.
+constexpr int MAX_CHUNKS = 16;
 
 }  // namespace
 
@@ -241,6 +245,26 @@ struct pty_ctx {
   bool have_metric_state = false;
   int last_algo = -1;
 
+  // packed scans (pty_set_scans): scans of one frame size whose frames form
+  // this context's stack in scan order and whose objects occupy consecutive
+  // row bands of its object; every stage then runs once over all of them
+  int nscan = 1;
+  std::vector<int2> h_off;        // window offsets (host copy)
+  std::vector<int> h_fstart, h_rstart;
+  int* fscan = nullptr;           // [N] scan of each frame (Geom::fscan while packed)
+  int* sc_fstart = nullptr;       // [S + 1]
+  int* sc_rstart = nullptr;       // [S + 1]
+  int* sc_pchunk = nullptr;       // probe chunks: first frame of each, [nchunks + 1]
+  int* sc_pscan = nullptr;        // first probe chunk of each scan, [S + 1]
+  int pk_chunks = 0;              // probe chunks over all scans
+  double* sc_inorm = nullptr;     // [S] I_norm of each scan
+  bool sc_norm_set = false;       // sc_inorm given (pty_set_scans) or computed (pty_prepare_amp)
+  size_t cap_pchunks = 0;         // probe chunks pnum / pg hold
+  size_t cap_G = 0;               // probes G holds (M K each)
+  size_t cap_econP_all = 0;       // doubles part_econP holds
+  size_t cap_esup_all = 0;        // doubles part_esup holds
+  size_t cap_maskbits = 0;        // words maskbits holds
+
   size_t MK() const { return (size_t)M * K; }
   size_t NMK() const { return (size_t)N * M * K; }
 };
@@ -289,6 +313,9 @@ void prof_collect(pty_ctx* c) {
 // --------------------------------------------------------------------------
 // launch helpers
 // --------------------------------------------------------------------------
+
+// an iteration's metrics (3 doubles per scan) in the run's device buffer
+double* metrics_at(pty_ctx* c, int64_t k) { return c->metrics + 3 * (size_t)c->nscan * k; }
 
 // grid of a persistent launch: all items if few, else `cap` rounded down to a
 // multiple of the XCD count (so the XCD-aware renumbering applies)
@@ -382,8 +409,17 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   // 1.17 ms for a persistent grid of 2 per CU, intermediate sizes in between);
   // the others as many work-groups per CU as the register budget keeps resident
   const int mult = c->cols_mult ? c->cols_mult : hoist ? 128 : (wide ? std::max(1, wpe / 2) : wpe);
-  const int grid = persistent_grid(items, (long)mult * c->cus);
-  if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
+  int grid;
+  if (c->nscan > 1 && ColTraits<MODE>::err) {
+    // packed scans: one item per work-group and one partial per item, so each
+    // scan's partials are one contiguous range
+    if (items > (long)c->cap_col * MAX_CHUNKS) return fail(PTY_EINVAL, "column partials exceed capacity");
+    grid = (int)items;
+    a.item_parts = 1;
+  } else {
+    grid = persistent_grid(items, (long)mult * c->cus);
+    if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
+  }
   if (wide) {
     hipLaunchKernelGGL((k_cols<256, MODE, 512>), dim3(grid), dim3(512), 0, c->stream, c->g, a);
   } else {
@@ -574,7 +610,9 @@ int probe_update(pty_ctx* c, const float2* S, const float2* O, const float2* Pol
   a.pg = c->pg;
   a.fresh = fresh;
   a.chunk = c->probe_chunk;
-  dim3 g1((MK + WG - 1) / WG, c->probe_chunks);
+  const bool pk = c->nscan > 1;   // packed scans: per-scan chunk tables, one probe per scan
+  if (pk) a.cstart = c->sc_pchunk;
+  dim3 g1((MK + WG - 1) / WG, pk ? c->pk_chunks : c->probe_chunks);
   hipLaunchKernelGGL(k_probe_partial, g1, dim3(WG), 0, c->stream, c->g, a);
   PTY_LAUNCHED("k_probe_partial");
   ProbeFinalArgs f{};
@@ -589,7 +627,9 @@ int probe_update(pty_ctx* c, const float2* S, const float2* O, const float2* Pol
   f.Pout = Pout;
   f.part_econ = econ ? c->part_econP : nullptr;
   f.part_esup = esup ? c->part_esup : nullptr;
-  hipLaunchKernelGGL(k_probe_final, dim3((MK + WG - 1) / WG), dim3(WG), 0, c->stream, c->g, f);
+  if (pk) f.sc = c->sc_pscan;
+  hipLaunchKernelGGL(k_probe_final, dim3((MK + WG - 1) / WG, pk ? c->nscan : 1), dim3(WG), 0,
+                     c->stream, c->g, f);
   PTY_LAUNCHED("k_probe_final");
   return PTY_OK;
 }
@@ -692,9 +732,7 @@ int check_bound(pty_ctx* c, bool need_bg) {
 // The staged modulus step (row FFTs -> column FFTs + modulus -> inverse row
 // FFTs) can run over blocks of frames small enough that the frame spectra stay
 // in the 256 MB Infinity Cache between its three kernels instead of going
-// through HBM.  At most MAX_CHUNKS blocks (the column kernels' error partials
-// are appended per block).
-constexpr int MAX_CHUNKS = 16;
+// through HBM.  At most MAX_CHUNKS blocks.
 
 int chunk_frames(const pty_ctx* c) {
   if (c->chunk_frames <= 0 || c->chunk_frames >= c->N) return c->N;
@@ -723,6 +761,7 @@ struct ChunkView {
   ChunkView(pty_ctx* c_, int n0, int n) : c(c_), saved(c_->g) {
     c->g.off = saved.off + n0;
     if (saved.Pn) c->g.Pn = saved.Pn + (size_t)n0 * c->MK();
+    if (saved.fscan) c->g.fscan = saved.fscan + n0;
     c->g.N = n;
   }
   ~ChunkView() { c->g = saved; }
@@ -848,28 +887,35 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     c->era_bg_shared = true;
   }
 
-  // metrics
+  // metrics (packed scans: `kind` says how each scan's partials lie in a region)
   Regions rg{};
+  int kind[NSLOT] = {};
+  const int okind = (rowg || rowg_op) ? RK_ROWS : RK_TILES;   // object-update partials
   rg.p[S_EMOD] = c->part_emod;
   rg.n[S_EMOD] = ncol;
+  kind[S_EMOD] = RK_COLS;
+  const int pb = (int)((MK + WG - 1) / WG);
   if (fusedO) {
     rg.p[S_ECON_ON] = c->part_econO;
     rg.p[S_ECON_OD] = c->part_econO + ngather;
     rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ngather;
+    kind[S_ECON_ON] = kind[S_ECON_OD] = okind;
   } else if (fusedP) {
-    const int nb = (int)((MK + WG - 1) / WG);
     rg.p[S_ECON_PN] = c->part_econP;
-    rg.p[S_ECON_PD] = c->part_econP + nb;
-    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = nb;
+    rg.p[S_ECON_PD] = c->part_econP + pb;
+    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = pb;
+    kind[S_ECON_PN] = kind[S_ECON_PD] = RK_PROBE;
   } else if (fusedOP) {
-    const int nb = (int)((MK + WG - 1) / WG);
     rg.p[S_ECON_ON] = c->part_econO;
     rg.p[S_ECON_OD] = c->part_econO + ngather;
     rg.n[S_ECON_ON] = rg.n[S_ECON_OD] = ngather;
+    kind[S_ECON_ON] = kind[S_ECON_OD] = okind;
     rg.p[S_ECON_PN] = c->part_econP;
-    rg.p[S_ECON_PD] = c->part_econP + nb;
-    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = nb;
+    rg.p[S_ECON_PD] = c->part_econP + pb;
+    rg.n[S_ECON_PN] = rg.n[S_ECON_PD] = pb;
+    kind[S_ECON_PN] = kind[S_ECON_PD] = RK_PROBE;
   } else {
+    if (c->nscan > 1) return fail(PTY_EUNSUP, "packed scans: unfused update step");
     if (upd != 1) {
       PTY_TRY(econ_pass(c, c->O, c->Obak, UV, c->part_econO));
       rg.p[S_ECON_ON] = c->part_econO;
@@ -888,26 +934,62 @@ int era_iter(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_out) {
     esup_mode = ESUP_IDENTITY;
     rg.p[S_ESUP_A] = c->part_esa;
     rg.n[S_ESUP_A] = ncol;
+    kind[S_ESUP_A] = RK_COLS;
     if (esup_probe) {
-      const int nb = (int)((MK + WG - 1) / WG);
       rg.p[S_ESUP_X] = c->part_esup;
-      rg.p[S_ESUP_H] = c->part_esup + nb;
-      rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = nb;
+      rg.p[S_ESUP_H] = c->part_esup + pb;
+      rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = pb;
+      kind[S_ESUP_X] = kind[S_ESUP_H] = RK_PROBE;
     } else {
       rg.p[S_ESUP_X] = c->part_esxh;
       rg.p[S_ESUP_H] = c->part_esxh + ngather;
       rg.n[S_ESUP_X] = rg.n[S_ESUP_H] = ngather;
+      kind[S_ESUP_X] = kind[S_ESUP_H] = okind;
     }
   } else {
     esup_mode = ESUP_EXPLICIT;
     StageTimer tm(c, PTY_STAGE_REDUCE);
     // a whole number of frame rows per work-group, every thread busy
     const int esup_threads = (WG / c->K) * c->K;
-    hipLaunchKernelGGL(k_esup, dim3(ESUP_BLOCKS), dim3(esup_threads), 0, c->stream, c->g, (const float2*)c->X,
-                       (const float2*)c->O, (const float2*)c->P, c->part_esup);
-    PTY_LAUNCHED("k_esup");
+    if (c->nscan > 1) {
+      // one pass per scan, each into its own partials
+      for (int sc = 0; sc < c->nscan; ++sc) {
+        const int f0 = c->h_fstart[sc], nf = c->h_fstart[sc + 1] - f0;
+        ChunkView view(c, f0, nf);
+        hipLaunchKernelGGL(k_esup, dim3(ESUP_BLOCKS), dim3(esup_threads), 0, c->stream, c->g,
+                           (const float2*)(c->X + (size_t)f0 * MK), (const float2*)c->O,
+                           (const float2*)c->P, c->part_esup + (size_t)sc * ESUP_BLOCKS);
+        PTY_LAUNCHED("k_esup");
+      }
+    } else {
+      hipLaunchKernelGGL(k_esup, dim3(ESUP_BLOCKS), dim3(esup_threads), 0, c->stream, c->g, (const float2*)c->X,
+                         (const float2*)c->O, (const float2*)c->P, c->part_esup);
+      PTY_LAUNCHED("k_esup");
+    }
     rg.p[S_ESUP_E] = c->part_esup;
     rg.n[S_ESUP_E] = ESUP_BLOCKS;
+    kind[S_ESUP_E] = RK_ESUP;
+  }
+  if (c->nscan > 1) {
+    ScanRegions sr{};
+    for (int k = 0; k < NSLOT; ++k) {
+      sr.p[k] = rg.p[k];
+      sr.kind[k] = kind[k];
+    }
+    sr.fstart = c->sc_fstart;
+    sr.rstart = c->sc_rstart;
+    sr.I_norm = c->sc_inorm;
+    sr.S = c->nscan;
+    sr.ncb = ncol / c->N;
+    sr.tiles_x = c->tiles_x;
+    sr.segs = c->rowg_segs;
+    sr.pb = pb;
+    sr.esup_blocks = ESUP_BLOCKS;
+    StageTimer tm(c, PTY_STAGE_REDUCE);
+    hipLaunchKernelGGL(k_metrics_scans, dim3(1), dim3(1024), 0, c->stream, sr, (double)MK, esup_mode,
+                       metrics_out, c->use_slot ? c->slot : nullptr);
+    PTY_LAUNCHED("k_metrics_scans");
+    return PTY_OK;
   }
   return metrics_launch(c, rg, esup_mode, metrics_out);
 }
@@ -1295,7 +1377,7 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
           (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp,
-          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->dmf_run, (uint64_t)c->dm1_run,
+          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->nscan, (uint64_t)c->dmf_run, (uint64_t)c->dm1_run,
           (uint64_t)c->dm1_pending, (uint64_t)c->dm1_par, (uint64_t)c->f_spec,
           (uint64_t)c->dm1_op};
 }
@@ -1353,7 +1435,7 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
   for (int32_t k = 0; k < iters; ++k) {
     const int64_t it = iter0 + k;
     if (!c->graphs) {   // a capture failed earlier in this run: continue eagerly
-      PTY_TRY(run_one(c, o, it, c->metrics + 3 * (size_t)k));
+      PTY_TRY(run_one(c, o, it, metrics_at(c, k)));
       continue;
     }
     std::vector<uint64_t> key = iter_key(c, o, it);
@@ -1383,7 +1465,7 @@ int run_graphs(pty_ctx* c, const pty_opts& o, int64_t iter0, int32_t iters) {
         c->graphs = false;
         c->graphs_fell_back = true;
         c->use_slot = false;
-        PTY_TRY(run_one(c, o, it, c->metrics + 3 * (size_t)k));
+        PTY_TRY(run_one(c, o, it, metrics_at(c, k)));
         continue;
       }
       c->gcache.push_back({key, exec, save_state(c)});
@@ -1512,13 +1594,27 @@ int run_prepare(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int
       return fail(PTY_EUNSUP, "sub-pixel positions are modelled without the far-field probe constraint");
     if (begin || !c->Pn_valid) PTY_TRY(build_subpixel_probes(c));
   }
+  if (c->nscan > 1) {
+    // packed scans: the fused ERA steps whose partials are kept per scan
+    if (o->algorithm != 0) return fail(PTY_EUNSUP, "packed scans run ERA");
+    if (o->method > 3) return fail(PTY_EUNSUP, "packed scans: methods 1-3 (no background)");
+    if (o->probe_centering) return fail(PTY_EUNSUP, "packed scans: no probe centring");
+    if (o->pmod_probe_until > iter0)
+      return fail(PTY_EUNSUP, "packed scans: no far-field probe constraint");
+    if (o->method == 3 && (o->op_n != 1 || o->op_every != 1))
+      return fail(PTY_EUNSUP, "packed scans: method 3 with OP_iters (1, 1)");
+    if (c->use_frame) return fail(PTY_EUNSUP, "packed scans take the staged kernels");
+    if (c->chunk_frames > 0 && c->chunk_frames < c->N)
+      return fail(PTY_EUNSUP, "packed scans: modulus step over the whole stack");
+    if (!c->sc_norm_set) return fail(PTY_ESTATE, "packed scans: no I_norm (pty_prepare_amp)");
+  }
   c->os_next = false;   // the caller may have changed O, P or the data since
   c->os_next_op = false;
   if (o->pmod_probe_until > iter0 && !c->pamp_set) PTY_TRY(pty_set_pamp(c, nullptr));
-  if (iters > c->metrics_cap) {
+  if ((long)iters * c->nscan > c->metrics_cap) {
     dev_free(c->metrics);
-    PTY_TRY(dev_alloc(&c->metrics, 3 * (size_t)iters));
-    c->metrics_cap = iters;
+    PTY_TRY(dev_alloc(&c->metrics, 3 * (size_t)iters * c->nscan));
+    c->metrics_cap = iters * c->nscan;
   }
   if (begin) {
     c->H_valid = c->G_valid = false;
@@ -1614,12 +1710,12 @@ int run_one(pty_ctx* c, const pty_opts& o, int64_t it, double* metrics_dev) {
 // after a run's iterations: the metrics a spectral DM method-1 run still owes
 int run_tail(pty_ctx* c, int32_t iters) {
   if (!c->dm1_pending || iters <= 0) return PTY_OK;
-  return dm1_tail(c, c->use_slot ? c->metrics : c->metrics + 3 * (size_t)(iters - 1));
+  return dm1_tail(c, c->use_slot ? c->metrics : metrics_at(c, iters - 1));
 }
 
 int run_finish(pty_ctx* c, int32_t iters, double* metrics_host) {
   if (metrics_host && iters > 0)
-    PTY_HIP(hipMemcpyAsync(metrics_host, c->metrics, 3 * (size_t)iters * sizeof(double),
+    PTY_HIP(hipMemcpyAsync(metrics_host, c->metrics, 3 * (size_t)c->nscan * iters * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
   PTY_HIP(hipStreamSynchronize(c->stream));
   if (c->prof_on) prof_collect(c);
@@ -1769,6 +1865,7 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   }
   pty_ctx* c = new pty_ctx();
   c->N = N; c->M = M; c->K = K; c->U = U; c->V = V;
+  c->h_off = off;
   auto cleanup = [&](int code) { pty_destroy(c); return code; };
 
   // object tiles -> ascending frame lists (CSR)
@@ -1892,6 +1989,11 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->rowg = !(e_rg && atoi(e_rg) == 0);
   }
 
+  c->cap_pchunks = (size_t)c->probe_chunks;
+  c->cap_G = 1;
+  c->cap_econP_all = 2 * (size_t)c->cap_econP;
+  c->cap_esup_all = (size_t)c->cap_esup;
+
   std::vector<float2> tw;
   hipError_t e = hipSuccess;
   twiddles(M, tw);
@@ -1930,6 +2032,8 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->Xo); dev_free(c->partb2); dev_free(c->dm1_esup[0]); dev_free(c->dm1_esup[1]);
   dev_free(c->part_emod); dev_free(c->part_esa); dev_free(c->part_econO);
   dev_free(c->part_econP); dev_free(c->part_esup); dev_free(c->part_esxh);
+  dev_free(c->fscan); dev_free(c->sc_fstart); dev_free(c->sc_rstart); dev_free(c->sc_pchunk);
+  dev_free(c->sc_pscan); dev_free(c->sc_inorm);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   drop_graphs(c);
   if (c->gstream) (void)hipStreamDestroy(c->gstream);
@@ -1947,6 +2051,27 @@ int pty_set_stream(pty_ctx* c, void* stream) {
 int pty_prepare_amp(pty_ctx* c, const float* I, const uint8_t* mask_centred, float* amp,
                     double* I_norm_out) {
   if (!c || !I || !amp) return fail(PTY_EINVAL, "NULL argument");
+  if (c->nscan > 1) {
+    // packed scans: each scan's frames with its own mask ([S][M][K]), one I_norm per scan
+    const size_t MK = c->MK();
+    std::vector<double> norms(c->nscan);
+    for (int sc = 0; sc < c->nscan; ++sc) {
+      const int f0 = c->h_fstart[sc], nf = c->h_fstart[sc + 1] - f0;
+      hipLaunchKernelGGL(k_prepare_amp, dim3(ESUP_BLOCKS), dim3(WG), 0, c->stream,
+                         I + (size_t)f0 * MK, nf, c->M, c->K,
+                         mask_centred ? mask_centred + (size_t)sc * MK : nullptr,
+                         amp + (size_t)f0 * MK, c->part_esup);
+      PTY_LAUNCHED("k_prepare_amp");
+      double sv = 0.0;
+      PTY_TRY(sum_region(c, c->part_esup, ESUP_BLOCKS, &sv));
+      norms[sc] = sv;
+      if (I_norm_out) I_norm_out[sc] = sv;
+    }
+    // the scans' metrics normalise by these sums
+    PTY_HIP(hipMemcpy(c->sc_inorm, norms.data(), c->nscan * sizeof(double), hipMemcpyHostToDevice));
+    c->sc_norm_set = true;
+    return PTY_OK;
+  }
   hipLaunchKernelGGL(k_prepare_amp, dim3(ESUP_BLOCKS), dim3(WG), 0, c->stream, I, c->N, c->M, c->K,
                      mask_centred, amp, c->part_esup);
   PTY_LAUNCHED("k_prepare_amp");
@@ -1963,21 +2088,138 @@ int pty_set_data(pty_ctx* c, const float* amp, const uint8_t* mask, double I_nor
   c->I_norm = I_norm;
   c->dmf_lead = false;   // the leading Pmod_7 in Xo used the previous data
   if (mask) {
+    // one packed mask per scan (a packed context's mask is [S][M][K])
     const int T = threads_per_line(c->M);
-    if (!c->maskbits) PTY_TRY(dev_alloc(&c->maskbits, (size_t)T * c->K));
+    const size_t words = (size_t)T * c->K * c->nscan;
+    if (c->cap_maskbits < words) {
+      dev_free(c->maskbits);
+      c->cap_maskbits = 0;
+      PTY_TRY(dev_alloc(&c->maskbits, words));
+      c->cap_maskbits = words;
+    }
     const int grid = (T * c->K + WG - 1) / WG;
-    switch (c->M) {
+    for (int sc = 0; sc < c->nscan; ++sc) {
+      const uint8_t* ms = mask + (size_t)sc * c->MK();
+      uint32_t* mb = c->maskbits + (size_t)sc * T * c->K;
+      switch (c->M) {
 #define C(Lv)                                                                                  \
   case Lv:                                                                                     \
-    hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c->stream, mask, c->K,        \
-                       c->maskbits);                                                           \
+    hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c->stream, ms, c->K, mb);     \
     break;
-      PTY_SIZES(C)
+        PTY_SIZES(C)
 #undef C
-      default: return fail(PTY_EUNSUP, "unsupported M");
+        default: return fail(PTY_EUNSUP, "unsupported M");
+      }
+      PTY_LAUNCHED("k_pack_mask");
     }
-    PTY_LAUNCHED("k_pack_mask");
   }
+  return PTY_OK;
+}
+
+int pty_set_scans(pty_ctx* c, int32_t S, const int32_t* frame_start, const int32_t* row_start,
+                  const double* I_norm) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (S < 1) return fail(PTY_EINVAL, "S must be >= 1");
+  if (S > 1 && (!frame_start || !row_start)) return fail(PTY_EINVAL, "NULL argument");
+  if (S > 1 && c->sub_set) return fail(PTY_EUNSUP, "packed scans take integer positions");
+  if (S > 1) {
+    if (frame_start[0] != 0 || frame_start[S] != c->N)
+      return fail(PTY_EINVAL, "frame_start must run from 0 to N");
+    if (row_start[0] != 0 || row_start[S] != c->U)
+      return fail(PTY_EINVAL, "row_start must run from 0 to U");
+    for (int sc = 0; sc < S; ++sc) {
+      if (frame_start[sc + 1] <= frame_start[sc]) return fail(PTY_EINVAL, "every scan needs frames");
+      if (row_start[sc + 1] <= row_start[sc]) return fail(PTY_EINVAL, "every scan needs object rows");
+      if (row_start[sc] % TILE_H)
+        return fail(PTY_EINVAL, "scan object bands must start on multiples of 16 rows");
+      for (int n = frame_start[sc]; n < frame_start[sc + 1]; ++n) {
+        const int y = c->h_off[n].x;
+        if (y < row_start[sc] || y + c->M > row_start[sc + 1])
+          return fail(PTY_EINVAL, "frame " + std::to_string(n) + " lies outside scan " +
+                                      std::to_string(sc) + "'s object rows");
+      }
+    }
+  }
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  // a new packing rebinds data and state
+  c->amp = nullptr;
+  c->mask = nullptr;
+  c->H_valid = c->G_valid = false;
+  c->os_next = c->os_next_op = false;
+  dev_free(c->fscan); dev_free(c->sc_fstart); dev_free(c->sc_rstart);
+  dev_free(c->sc_pchunk); dev_free(c->sc_pscan); dev_free(c->sc_inorm);
+  c->h_fstart.clear();
+  c->h_rstart.clear();
+  c->nscan = 1;
+  c->g.fscan = nullptr;
+  c->sc_norm_set = false;
+  if (S == 1) {
+    if (I_norm) c->I_norm = I_norm[0];
+    return PTY_OK;
+  }
+
+  const int MK = (int)c->MK();
+  const int pblocks = (MK + WG - 1) / WG;
+  std::vector<int> fscan(c->N), pchunk, pscan(S + 1);
+  for (int sc = 0; sc < S; ++sc) {
+    const int f0 = frame_start[sc], nf = frame_start[sc + 1] - f0;
+    for (int n = f0; n < f0 + nf; ++n) fscan[n] = sc;
+    // probe chunks of a scan as a context of its own would cut them
+    int chunks = std::max(1, std::min(nf, std::max(1, 4096 / pblocks)));
+    const int chunk = (nf + chunks - 1) / chunks;
+    chunks = (nf + chunk - 1) / chunk;
+    pscan[sc] = (int)pchunk.size();
+    for (int k = 0; k < chunks; ++k) pchunk.push_back(f0 + k * chunk);
+  }
+  pscan[S] = (int)pchunk.size();
+  pchunk.push_back(c->N);
+  const size_t nch = pchunk.size() - 1;
+  // workspace per scan: probe heatmaps, probe-chunk sums, probe / eSup partials
+  if (c->cap_pchunks < nch) {
+    dev_free(c->pnum); dev_free(c->pg);
+    c->cap_pchunks = 0;
+    PTY_TRY(dev_alloc(&c->pnum, nch * MK));
+    PTY_TRY(dev_alloc(&c->pg, nch * MK));
+    c->cap_pchunks = nch;
+  }
+  if (c->cap_G < (size_t)S) {
+    dev_free(c->G);
+    c->cap_G = 0;
+    PTY_TRY(dev_alloc(&c->G, (size_t)S * MK));
+    c->cap_G = S;
+  }
+  if (c->cap_econP_all < 2 * (size_t)S * pblocks) {
+    dev_free(c->part_econP);
+    c->cap_econP_all = 0;
+    PTY_TRY(dev_alloc(&c->part_econP, 2 * (size_t)S * pblocks));
+    c->cap_econP_all = 2 * (size_t)S * pblocks;
+  }
+  const size_t esup_need = std::max(2 * (size_t)S * pblocks, (size_t)S * ESUP_BLOCKS);
+  if (c->cap_esup_all < esup_need) {
+    dev_free(c->part_esup);
+    c->cap_esup_all = 0;
+    PTY_TRY(dev_alloc(&c->part_esup, esup_need));
+    c->cap_esup_all = esup_need;
+  }
+  PTY_TRY(dev_alloc(&c->fscan, (size_t)c->N));
+  PTY_TRY(dev_alloc(&c->sc_fstart, (size_t)S + 1));
+  PTY_TRY(dev_alloc(&c->sc_rstart, (size_t)S + 1));
+  PTY_TRY(dev_alloc(&c->sc_pchunk, pchunk.size()));
+  PTY_TRY(dev_alloc(&c->sc_pscan, pscan.size()));
+  PTY_TRY(dev_alloc(&c->sc_inorm, (size_t)S));
+  PTY_HIP(hipMemcpy(c->fscan, fscan.data(), fscan.size() * sizeof(int), hipMemcpyHostToDevice));
+  PTY_HIP(hipMemcpy(c->sc_fstart, frame_start, (S + 1) * sizeof(int), hipMemcpyHostToDevice));
+  PTY_HIP(hipMemcpy(c->sc_rstart, row_start, (S + 1) * sizeof(int), hipMemcpyHostToDevice));
+  PTY_HIP(hipMemcpy(c->sc_pchunk, pchunk.data(), pchunk.size() * sizeof(int), hipMemcpyHostToDevice));
+  PTY_HIP(hipMemcpy(c->sc_pscan, pscan.data(), pscan.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (I_norm) PTY_HIP(hipMemcpy(c->sc_inorm, I_norm, S * sizeof(double), hipMemcpyHostToDevice));
+  c->sc_norm_set = I_norm != nullptr;
+  c->h_fstart.assign(frame_start, frame_start + S + 1);
+  c->h_rstart.assign(row_start, row_start + S + 1);
+  c->pk_chunks = (int)nch;
+  c->nscan = S;
+  c->g.fscan = c->fscan;
   return PTY_OK;
 }
 
@@ -2050,6 +2292,10 @@ int pty_set_subpixel(pty_ctx* c, const double* d_host) {
     c->sub_set = false;
     return PTY_OK;
   }
+  if (c->nscan > 1) {
+    c->sub_set = false;
+    return fail(PTY_EUNSUP, "packed scans take integer positions");
+  }
   for (int k = 0; k < 2 * c->N; ++k)
     if (!std::isfinite(d_host[k]) || std::fabs(d_host[k]) > 1.0)
       return fail(PTY_EINVAL, "sub-pixel shifts must be finite and within [-1, 1]");
@@ -2061,6 +2307,7 @@ int pty_set_subpixel(pty_ctx* c, const double* d_host) {
 }
 
 int pty_set_background(pty_ctx* c, const float* b, int32_t per_frame) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_set_background: not available on packed scans");
   if (!c || !b) return fail(PTY_EINVAL, "NULL argument");
   if (!c->bg) PTY_TRY(dev_alloc(&c->bg, c->NMK()));
   if (!c->partb) PTY_TRY(dev_alloc(&c->partb, (size_t)c->col_chunks * c->MK()));
@@ -2077,6 +2324,7 @@ int pty_set_background(pty_ctx* c, const float* b, int32_t per_frame) {
 }
 
 int pty_set_pamp(pty_ctx* c, const float* pamp) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_set_pamp: not available on packed scans");
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (pamp) {
     PTY_TRY(copy_d2d(c, c->pamp, pamp, c->MK() * sizeof(float)));
@@ -2101,7 +2349,7 @@ int pty_run(pty_ctx* c, const pty_opts* o, int64_t iter0, int32_t iters, int32_t
   if (c->graphs) {
     PTY_TRY(run_graphs(c, *o, iter0, iters));
   } else {
-    for (int32_t k = 0; k < iters; ++k) PTY_TRY(run_one(c, *o, iter0 + k, c->metrics + 3 * (size_t)k));
+    for (int32_t k = 0; k < iters; ++k) PTY_TRY(run_one(c, *o, iter0 + k, metrics_at(c, k)));
   }
   PTY_TRY(run_tail(c, iters));
   return run_finish(c, iters, metrics_host);
@@ -2181,6 +2429,7 @@ int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t col
 // O, P, the DM exits and the background are restored afterwards, so the
 // caller's next pty_run(begin = 1) starts from the state it set.
 int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result* out) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_tune: not available on packed scans");
   if (!c || !o || !out || trial_iters < 1) return fail(PTY_EINVAL, "bad pty_tune arguments");
   PTY_TRY(check_bound(c, o->method >= 4));
   PTY_HIP(hipStreamSynchronize(c->stream));
@@ -2359,7 +2608,7 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
   if (!b->graphs) {
     for (int32_t k = 0; k < iters; ++k)
       for (int s = 0; s < n; ++s)
-        PTY_TRY(run_one(b->ctx[s], opts[s], iter0 + k, b->ctx[s]->metrics + 3 * (size_t)k));
+        PTY_TRY(run_one(b->ctx[s], opts[s], iter0 + k, metrics_at(b->ctx[s], k)));
   } else {
     const int zero = 0;
     for (int s = 0; s < n; ++s) {
@@ -2372,7 +2621,7 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
       const int64_t it = iter0 + k;
       if (!b->graphs) {   // a capture failed earlier in this run: continue eagerly
         for (int s = 0; s < n; ++s)
-          PTY_TRY(run_one(b->ctx[s], opts[s], it, b->ctx[s]->metrics + 3 * (size_t)k));
+          PTY_TRY(run_one(b->ctx[s], opts[s], it, metrics_at(b->ctx[s], k)));
         continue;
       }
       std::vector<uint64_t> key;
@@ -2420,7 +2669,7 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
           b->graphs = false;
           b->fell_back = true;
           for (int s = 0; s < n; ++s)
-            PTY_TRY(run_one(b->ctx[s], opts[s], it, b->ctx[s]->metrics + 3 * (size_t)k));
+            PTY_TRY(run_one(b->ctx[s], opts[s], it, metrics_at(b->ctx[s], k)));
           continue;
         }
         pty_batch::Entry e{key, exec, {}};
@@ -2438,7 +2687,7 @@ int pty_batch_run(pty_batch* b, const pty_opts* opts, int64_t iter0, int32_t ite
   for (int s = 0; s < n; ++s) {
     pty_ctx* c = b->ctx[s];
     if (metrics_host && metrics_host[s] && iters > 0)
-      PTY_HIP(hipMemcpyAsync(metrics_host[s], c->metrics, 3 * (size_t)iters * sizeof(double),
+      PTY_HIP(hipMemcpyAsync(metrics_host[s], c->metrics, 3 * (size_t)c->nscan * iters * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
   }
   for (int l = 0; l < nl; ++l) PTY_HIP(hipStreamSynchronize(b->lanes[l]));
@@ -2509,6 +2758,7 @@ int pty_background(pty_ctx* c, float* b) {
 }
 
 int pty_set_heatmap(pty_ctx* c, const double* H, const double* G) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_set_heatmap: not available on packed scans");
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (H) PTY_TRY(copy_d2d(c, c->H, H, (size_t)c->U * c->V * sizeof(double)));
   if (G) PTY_TRY(copy_d2d(c, c->G, G, c->MK() * sizeof(double)));
@@ -2520,6 +2770,7 @@ int pty_set_heatmap(pty_ctx* c, const double* H, const double* G) {
 }
 
 int pty_set_iterate(pty_ctx* c, const pty_cfloat* f, const pty_cfloat* O_ref, const pty_cfloat* P_ref) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_set_iterate: not available on packed scans");
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (!c->O || !c->P) return fail(PTY_ESTATE, "pty_set_state has not been called");
   if ((O_ref == nullptr) != (P_ref == nullptr))
@@ -2540,6 +2791,7 @@ int pty_set_iterate(pty_ctx* c, const pty_cfloat* f, const pty_cfloat* O_ref, co
 }
 
 int pty_get_iterate(pty_ctx* c, pty_cfloat* f, float* b) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_get_iterate: not available on packed scans");
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
   if (f) {
     if (!c->F || !c->dm_ready) return fail(PTY_ESTATE, "no DM iterate (run DM or pty_set_iterate first)");
@@ -2591,6 +2843,7 @@ int pty_fft2(pty_ctx* c, pty_cfloat* data, int32_t nframes, int32_t inverse) {
 }
 
 int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfloat* ex) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_make_exits: not available on packed scans");
   if (!c || !O || !P || !ex) return fail(PTY_EINVAL, "NULL argument");
   SharedProbe shared(c);
   hipLaunchKernelGGL(k_make_exits, dim3(4096), dim3(WG), 0, c->stream, c->g,
@@ -2603,6 +2856,7 @@ int pty_make_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, pty_cfl
 
 int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, const pty_cfloat* O,
                     const pty_cfloat* P, float w, float beta, float alpha, double* emod) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_hio_modulus: not available on packed scans");
   if (!c || !f || !m || !O || !P) return fail(PTY_EINVAL, "NULL argument");
   if (!std::isfinite(w)) return fail(PTY_EINVAL, "w must be finite");
   if (f == m) return fail(PTY_EINVAL, "f and m must be different buffers");
@@ -2635,6 +2889,7 @@ int pty_hio_modulus(pty_ctx* c, pty_cfloat* f, pty_cfloat* m, const pty_cfloat* 
 }
 
 int pty_add_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, float scale, pty_cfloat* f) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_add_exits: not available on packed scans");
   if (!c || !O || !P || !f) return fail(PTY_EINVAL, "NULL argument");
   SharedProbe shared(c);
   hipLaunchKernelGGL(k_add_exits, dim3(4096), dim3(WG), 0, c->stream, c->g,
@@ -2646,6 +2901,7 @@ int pty_add_exits(pty_ctx* c, const pty_cfloat* O, const pty_cfloat* P, float sc
 }
 
 int pty_pmod1(pty_ctx* c, pty_cfloat* exits, float alpha, double* emod) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_pmod1: not available on packed scans");
   if (!c || !exits) return fail(PTY_EINVAL, "NULL argument");
   if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
   float2* d = reinterpret_cast<float2*>(exits);
@@ -2730,6 +2986,7 @@ int pty_pmod1_noalloc(pty_cfloat* exits, const float* amp, const uint8_t* mask, 
 
 int pty_psup_O(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* P, pty_cfloat* O_out, double* H,
                int32_t fresh, float alpha) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_psup_O: not available on packed scans");
   if (!c || !ex || !P || !O_out || !H) return fail(PTY_EINVAL, "NULL argument");
   SharedProbe shared(c);
   PTY_TRY(gather(c, reinterpret_cast<const float2*>(ex), reinterpret_cast<const float2*>(P), nullptr,
@@ -2740,6 +2997,7 @@ int pty_psup_O(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* P, pty_cfloat
 
 int pty_psup_P(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* O, pty_cfloat* P_out, double* G,
                int32_t fresh, float alpha) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_psup_P: not available on packed scans");
   if (!c || !ex || !O || !P_out || !G) return fail(PTY_EINVAL, "NULL argument");
   PTY_TRY(probe_update(c, reinterpret_cast<const float2*>(ex), reinterpret_cast<const float2*>(O),
                        nullptr, reinterpret_cast<float2*>(P_out), G, fresh ? 1 : 0, 1, alpha, false));
@@ -2753,6 +3011,7 @@ int pty_psup_P(pty_ctx* c, const pty_cfloat* ex, const pty_cfloat* O, pty_cfloat
 // (N x M x K, optional) receives the exit waves.
 int pty_back_projection(pty_ctx* c, const pty_cfloat* P, pty_cfloat* exits_out, pty_cfloat* O_out,
                         double* H, float alpha) {
+  if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_back_projection: not available on packed scans");
   if (!c || !P || !O_out || !H) return fail(PTY_EINVAL, "NULL argument");
   if (!c->amp) return fail(PTY_ESTATE, "pty_set_data has not been called");
   SharedProbe shared(c);

@@ -59,6 +59,7 @@ class Engine:
             stream = torch.cuda.current_stream(self.device)
             call("pty_set_stream", self._ctx, ctypes.c_void_p(stream.cuda_stream))
         self._keep = {}
+        self.S = 1
         if FUSED is not None:
             self.set_path(FUSED)
 
@@ -84,17 +85,37 @@ class Engine:
         torch.cuda.current_stream(self.device).synchronize()
 
     # -- binding ------------------------------------------------------------
+    def set_scans(self, frame_start, row_start):
+        """Pack scans: scan s owns frames [frame_start[s], frame_start[s+1]) and
+        the object rows [row_start[s], row_start[s+1]) (``pty_set_scans``).
+        Afterwards masks and the probe are per scan ([S, M, K]) and runs
+        return (iters, S, 3) metrics.  Unbinds the data."""
+        fs = np.ascontiguousarray(frame_start, dtype=np.int32)
+        rs = np.ascontiguousarray(row_start, dtype=np.int32)
+        if fs.shape != rs.shape or fs.ndim != 1 or len(fs) < 2:
+            raise ValueError("frame_start and row_start: S + 1 entries each")
+        self._sync_torch()
+        call("pty_set_scans", self._ctx, len(fs) - 1, fs.ctypes.data_as(ctypes.c_void_p),
+             rs.ctypes.data_as(ctypes.c_void_p), None)
+        self.S = len(fs) - 1
+        self._keep.pop("amp", None)
+        self._keep.pop("mask", None)
+
     def prepare_amp(self, I, mask_centred=None):
-        """amp = ifftshift(sqrt(I)), I_norm = sum(mask * I) (ptychography/era.py:495-498)."""
+        """amp = ifftshift(sqrt(I)), I_norm = sum(mask * I) (ptychography/era.py:495-498).
+        Packed scans: mask_centred is [S, M, K] and I_norm a list of S sums."""
         I = I.to(self.device, torch.float32).contiguous()
         amp = torch.empty_like(I)
         m = None
         if mask_centred is not None:
             m = mask_centred.to(self.device, torch.uint8).contiguous()
         self._sync_torch()
-        out = ctypes.c_double()
-        call("pty_prepare_amp", self._ctx, _ptr(I), _ptr(m), _ptr(amp), ctypes.byref(out))
-        return amp, float(out.value)
+        out = (ctypes.c_double * self.S)()
+        call("pty_prepare_amp", self._ctx, _ptr(I), _ptr(m), _ptr(amp),
+             ctypes.cast(out, ctypes.POINTER(ctypes.c_double)))
+        if self.S > 1:
+            return amp, [float(v) for v in out]
+        return amp, float(out[0])
 
     def set_data(self, amp, mask_shifted, I_norm):
         self._keep["amp"] = amp
@@ -104,7 +125,8 @@ class Engine:
     def set_state(self, O, P):
         assert O.dtype == torch.complex64 and P.dtype == torch.complex64
         assert O.is_contiguous() and P.is_contiguous()
-        assert tuple(O.shape) == (self.U, self.V) and tuple(P.shape) == (self.M, self.K)
+        pshape = (self.M, self.K) if self.S == 1 else (self.S, self.M, self.K)
+        assert tuple(O.shape) == (self.U, self.V) and tuple(P.shape) == pshape
         self._keep["O"], self._keep["P"] = O, P
         self._sync_torch()
         call("pty_set_state", self._ctx, _ptr(O), _ptr(P))
@@ -151,9 +173,11 @@ class Engine:
 
     def run(self, algorithm, method, iters, iter0=0, begin=False, OP_iters=(1, 1),
             Pmod_probe_until=0, probe_centering=False, alpha=1e-10, metrics=True):
-        """Run ``iters`` iterations; returns an (iters, 3) array of eMod, eCon, eSup."""
+        """Run ``iters`` iterations; returns an (iters, 3) array of eMod, eCon, eSup
+        ((iters, S, 3) for packed scans)."""
         o = self.opts(algorithm, method, OP_iters, Pmod_probe_until, probe_centering, alpha)
-        out = np.zeros((max(iters, 0), 3), dtype=np.float64) if metrics else None
+        shape = (max(iters, 0), 3) if self.S == 1 else (max(iters, 0), self.S, 3)
+        out = np.zeros(shape, dtype=np.float64) if metrics else None
         ptr = out.ctypes.data_as(ctypes.c_void_p) if (metrics and iters > 0) else None
         call("pty_run", self._ctx, ctypes.byref(o), int(iter0), int(iters), int(bool(begin)), ptr)
         return out

@@ -67,6 +67,48 @@ def _random(rng, shape):
     return np.random.random(shape) if rng is None else rng.random(shape)
 
 
+def result_dtypes(I, dtype):
+    """(real, complex) numpy dtypes of the returns: the reference's precision
+    rule (float32 intensities -> complex64, else complex128) or ``dtype``."""
+    if dtype is None:
+        single = (I.dtype == torch.float32) if isinstance(I, torch.Tensor) else \
+            (np.asarray(I).dtype == np.float32)
+        dtype = 'single' if single else 'double'
+    if dtype == 'single':
+        return np.float32, np.complex64
+    if dtype == 'double':
+        return np.float64, np.complex128
+    raise ValueError("dtype must be None, 'single' or 'double'")
+
+
+def start_state(R, P, O, M, K, c_dtype, rng):
+    """Starting O and P as the preamble makes them (era.py:475-493): a flat
+    object covering every window, a random probe."""
+    if O is None:
+        shape = (M + int(np.ceil(R[:, 0].max() - R[:, 0].min())),
+                 K + int(np.ceil(R[:, 1].max() - R[:, 1].min())))
+        O = np.ones(shape, dtype=c_dtype)
+    if P is None:
+        P = _random(rng, (M, K)) + 1J * _random(rng, (M, K))
+    O = _as_numpy(O).astype(c_dtype)
+    P = _as_numpy(P).astype(c_dtype)
+    if P.shape != (M, K):
+        raise ValueError(f"P has shape {P.shape}, frames are {(M, K)}")
+    return O, P
+
+
+def mask_array(mask, M, K):
+    """The centred good-pixel mask as uint8 (None for the reference's mask=1)."""
+    if np.isscalar(mask) and mask == 1:
+        return None
+    m = _as_numpy(mask)
+    if m.shape != (M, K):
+        raise ValueError(f"mask must be 1 or an array of shape {(M, K)}")
+    if not np.all((m == 0) | (m == 1)):
+        raise ValueError("mask values must be 0/1 (True = good pixel)")
+    return m.astype(np.uint8)
+
+
 class Job:
     """One prepared scan: the engine with data and state bound, and what the
     returns need (``prepare`` / ``finish``)."""
@@ -104,15 +146,7 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
 
     I_is_tensor = isinstance(I, torch.Tensor)
     N, M, K = (int(s) for s in I.shape)
-    if dtype is None:
-        single = (I.dtype == torch.float32) if I_is_tensor else (np.asarray(I).dtype == np.float32)
-        dtype = 'single' if single else 'double'
-    if dtype == 'single':
-        r_dtype, c_dtype = np.float32, np.complex64
-    elif dtype == 'double':
-        r_dtype, c_dtype = np.float64, np.complex128
-    else:
-        raise ValueError("dtype must be None, 'single' or 'double'")
+    r_dtype, c_dtype = result_dtypes(I, dtype)
 
     if not isinstance(R, np.ndarray):
         raise TypeError("R must be a numpy array (it is shifted in place, era.py:504-505)")
@@ -130,16 +164,7 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
         raise ValueError(f"{what}: modelled without the far-field probe constraint")
     if frac and init == 'learned':
         raise ValueError(f"{what}: init='learned' stitches patches at integer positions only")
-    if O is None:
-        shape = (M + int(np.ceil(R[:, 0].max() - R[:, 0].min())),
-                 K + int(np.ceil(R[:, 1].max() - R[:, 1].min())))
-        O = np.ones(shape, dtype=c_dtype)
-    if P is None:
-        P = _random(rng, (M, K)) + 1J * _random(rng, (M, K))
-    O = _as_numpy(O).astype(c_dtype)
-    P = _as_numpy(P).astype(c_dtype)
-    if P.shape != (M, K):
-        raise ValueError(f"P has shape {P.shape}, frames are {(M, K)}")
+    O, P = start_state(R, P, O, M, K, c_dtype, rng)
 
     # in-place shift of the caller's R (era.py:504-505)
     R[:, 0] -= R[:, 0].max()
@@ -151,13 +176,8 @@ def prepare(algorithm, I, R, P, O, iters, OP_iters=1, mask=1, background=None, m
     try:
         mask_c = None
         mask_s = None
-        if not (np.isscalar(mask) and mask == 1):
-            m = _as_numpy(mask)
-            if m.shape != (M, K):
-                raise ValueError(f"mask must be 1 or an array of shape {(M, K)}")
-            if not np.all((m == 0) | (m == 1)):
-                raise ValueError("mask values must be 0/1 (True = good pixel)")
-            mc = m.astype(np.uint8)
+        mc = mask_array(mask, M, K)
+        if mc is not None:
             mask_c = torch.from_numpy(np.ascontiguousarray(mc)).to(dev)
             mask_s = torch.from_numpy(np.ascontiguousarray(np.fft.ifftshift(mc))).to(dev)
         I_t = I.to(dev) if I_is_tensor else torch.from_numpy(np.ascontiguousarray(I, dtype=np.float32))
