@@ -200,19 +200,35 @@ int pty_set_row_gather(pty_ctx* ctx, int32_t enable);
 int pty_get_row_gather(pty_ctx* ctx, int32_t* supported, int32_t* enabled);
 typedef struct pty_tune_result {
   int32_t fused, frame_threads, frame_slots, rows_mult, cols_mult, cols_wg, chunk_frames;  /* chosen */
-  int32_t row_gather;                                               /* chosen */
+  int32_t row_gather, fft_plan;                   /* chosen (fft_plan: pty_set_plan's mask) */
   int32_t candidates;                                               /* configurations timed */
   float best_ms, default_ms;                                        /* per iteration */
 } pty_tune_result;
 /* Times `trial_iters` iterations of opts for every launch configuration
  * (staged: rows_mult {2,3,4} x cols_mult {default,2,4}, and at 256^2 512-thread
  * column work-groups x cols_mult {default,1,3}; the fused inverse-row object
- * update on / off; ERA frame blocks of 0 and 1/4..2x the default; fused
- * per-frame path
- * where the frame shape allows it: 512 / 1024 threads x CUs / 2*CUs slots),
- * keeps the fastest and restores O, P and the background; the next pty_run
- * must pass begin = 1. */
+ * update on / off; every other line-FFT plan mask (pty_set_plan); ERA frame
+ * blocks of 0 and 1/4..2x the default; fused per-frame path where the frame
+ * shape allows it: 512 / 1024 threads x CUs / 2*CUs slots), keeps the fastest
+ * and restores O, P and the background; the next pty_run must pass begin = 1. */
 int pty_tune(pty_ctx* ctx, const pty_opts* opts, int32_t trial_iters, pty_tune_result* out);
+
+/* Line-FFT plans of the kernel stages (no counterpart in the reference,
+ * whose transforms are numpy.fft).  `mask` sets, per stage, the frame side's
+ * alternative radix split instead of its default: PTY_PLAN_ROWS (row
+ * kernels), PTY_PLAN_COLS (column kernels), PTY_PLAN_ROWG (the fused inverse
+ * row FFT + object update).  Alternatives: 64: 4x4x4 on 16 threads per line
+ * instead of 8x8 on 8; 96: 4x4x6 instead of 6x4x4; 128: 4x8x4 on 16 threads
+ * instead of 16x8 on 8; 192: 4x4x12 instead of 12x4x4; 256: 8x4x8 on 32
+ * threads instead of 16x16 on 16; sides 16 and 32 keep their plan.  Stages
+ * exchange frames in natural order, so plans mix freely; results agree to
+ * float32 round-off between plans (the same transform, another order of
+ * operations).  The default mask is the library's per-side choice (or the
+ * environment's PTY_FFT_PLAN); pty_tune tries every mask.  pty_get_plan
+ * reports the mask and whether the frame has alternatives. */
+enum { PTY_PLAN_ROWS = 1, PTY_PLAN_COLS = 2, PTY_PLAN_ROWG = 4 };
+int pty_set_plan(pty_ctx* ctx, int32_t mask);
+int pty_get_plan(pty_ctx* ctx, int32_t* mask, int32_t* has_alternative);
 
 /* ---- several scans at once (no counterpart in the reference) ----
  * The contexts (prepared with pty_set_data / pty_set_state / ...) stay owned

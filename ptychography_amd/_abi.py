@@ -32,7 +32,7 @@ class PtyTuneResult(ctypes.Structure):
                 ("frame_slots", ctypes.c_int32), ("rows_mult", ctypes.c_int32),
                 ("cols_mult", ctypes.c_int32), ("cols_wg", ctypes.c_int32),
                 ("chunk_frames", ctypes.c_int32),
-                ("row_gather", ctypes.c_int32),
+                ("row_gather", ctypes.c_int32), ("fft_plan", ctypes.c_int32),
                 ("candidates", ctypes.c_int32),
                 ("best_ms", ctypes.c_float), ("default_ms", ctypes.c_float)]
 
@@ -56,6 +56,8 @@ SIGNATURES = {
     "pty_set_pamp": (ctypes.c_int, [_vp, _vp]),
     "pty_set_subpixel": (ctypes.c_int, [_vp, _vp]),
     "pty_set_scans": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp]),
+    "pty_set_plan": (ctypes.c_int, [_vp, _i32]),
+    "pty_get_plan": (ctypes.c_int, [_vp, _vp, _vp]),
     "pty_position_terms": (ctypes.c_int, [_vp, _vp]),
     "pty_run": (ctypes.c_int, [_vp, ctypes.POINTER(PtyOpts), _i64, _i32, _i32, _vp]),
     "pty_intensity": (ctypes.c_int, [_vp, _vp]),

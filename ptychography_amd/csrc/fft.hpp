@@ -171,11 +171,20 @@ __device__ __forceinline__ void dft(float2* v) {
   }
 }
 
-// Line-FFT plan for one length: L = R1 * R2 (* R3 for three passes), T =
-// threads per line.  96 and 192 take three passes so that a thread holds 12
-// points instead of 24 (the two-pass radix-24 / radix-12 x 8 plans needed
-// ~250 VGPRs: 2 waves per SIMD).
-template <int L> struct Plan;
+// Line-FFT plans.  A plan key PK names a length L = plan_len(PK) and one
+// factorisation of it: L = R1 * R2 (* R3 for three passes), T = threads per
+// line, E = L / T points per thread.  Key L itself is the default plan of
+// length L; plan_key(L, 1) an alternative the launch-shape tuner may pick
+// (pty_set_plan).  96 and 192 take three passes by default so that a thread
+// holds 12 points instead of 24 (the two-pass radix-24 / radix-12 x 8 plans
+// needed ~250 VGPRs: 2 waves per SIMD).  The alternatives reverse the radix
+// order (96, 192) or split the line over more threads with three passes
+// (64: 4 x 4 x 4 on 16 threads, 128: 4 x 8 x 4 on 16, 256: 8 x 4 x 8 on 32),
+// halving the points a thread holds for one more LDS exchange per transform.
+constexpr int PLAN_STRIDE = 1024;
+constexpr int plan_len(int PK) { return PK % PLAN_STRIDE; }
+constexpr int plan_key(int L, int variant) { return L + PLAN_STRIDE * variant; }
+template <int PK> struct Plan;
 template <> struct Plan<16>  { static constexpr int R1 = 4,  R2 = 4,  R3 = 1, T = 4; };
 template <> struct Plan<32>  { static constexpr int R1 = 8,  R2 = 4,  R3 = 1, T = 4; };
 template <> struct Plan<48>  { static constexpr int R1 = 12, R2 = 4,  R3 = 1, T = 4; };
@@ -184,6 +193,11 @@ template <> struct Plan<96>  { static constexpr int R1 = 6,  R2 = 4,  R3 = 4, T 
 template <> struct Plan<128> { static constexpr int R1 = 16, R2 = 8,  R3 = 1, T = 8; };
 template <> struct Plan<192> { static constexpr int R1 = 12, R2 = 4,  R3 = 4, T = 16; };
 template <> struct Plan<256> { static constexpr int R1 = 16, R2 = 16, R3 = 1, T = 16; };
+template <> struct Plan<plan_key(64, 1)>  { static constexpr int R1 = 4, R2 = 4,  R3 = 4,  T = 16; };
+template <> struct Plan<plan_key(96, 1)>  { static constexpr int R1 = 4, R2 = 4,  R3 = 6,  T = 8; };
+template <> struct Plan<plan_key(128, 1)> { static constexpr int R1 = 4, R2 = 8,  R3 = 4,  T = 16; };
+template <> struct Plan<plan_key(192, 1)> { static constexpr int R1 = 4, R2 = 4,  R3 = 12, T = 16; };
+template <> struct Plan<plan_key(256, 1)> { static constexpr int R1 = 8, R2 = 4,  R3 = 8,  T = 32; };
 
 // Element index held in register slot e by thread t, for layoutA / layoutB.
 template <int Ra, int Rb, int T>
@@ -192,9 +206,9 @@ template <int Ra, int Rb, int T>
 __device__ __forceinline__ int idxB(int e, int t) { return (t + (e / Rb) * T) + (e % Rb) * Ra; }
 
 // LDS words (float2) one line needs for its exchange buffer (both directions).
-template <int L>
+template <int PK>
 struct LineLds {
-  static constexpr int R1 = Plan<L>::R1, R2 = Plan<L>::R2, R3 = Plan<L>::R3;
+  static constexpr int R1 = Plan<PK>::R1, R2 = Plan<PK>::R2, R3 = Plan<PK>::R3;
   static constexpr int a = R3 > 1 ? R1 * (R2 * R3 + 1) : R2 * (R1 + 1);
   static constexpr int b = R3 > 1 ? R3 * (R1 * R2 + 1) : R1 * (R2 + 1);
   static constexpr int words = (a > b ? a : b) | 1;  // odd stride between lines
@@ -297,48 +311,52 @@ __device__ __forceinline__ void line_fft3(float2* v, float2* mid, const float2* 
 // (forward input, inverse output): slot e of thread t holds element
 // t + (e / R1) T + (e % R1) (L / R1); layout B (forward output, inverse input):
 // the same with the last radix.  Both are "thread index + per-slot constant".
-template <int L>
+template <int PK>
 __device__ __forceinline__ void fft_fwd(float2* v, float2* mid, const float2* __restrict__ tw, int t) {
-  using Pl = Plan<L>;
+  using Pl = Plan<PK>;
+  constexpr int L = plan_len(PK);
   if constexpr (Pl::R3 > 1) line_fft3<L, Pl::R1, Pl::R2, Pl::R3, Pl::T, false>(v, mid, tw, t);
   else line_fft<L, Pl::R1, Pl::R2, Pl::T, false>(v, mid, tw, t);
 }
-template <int L>
+template <int PK>
 __device__ __forceinline__ void fft_inv(float2* v, float2* mid, const float2* __restrict__ tw, int t) {
-  using Pl = Plan<L>;
+  using Pl = Plan<PK>;
+  constexpr int L = plan_len(PK);
   if constexpr (Pl::R3 > 1) line_fft3<L, Pl::R3, Pl::R2, Pl::R1, Pl::T, true>(v, mid, tw, t);
   else line_fft<L, Pl::R2, Pl::R1, Pl::T, true>(v, mid, tw, t);
 }
-template <int L> struct Lay {
-  static constexpr int T = Plan<L>::T, RA = Plan<L>::R1;
-  static constexpr int RB = Plan<L>::R3 > 1 ? Plan<L>::R3 : Plan<L>::R2;
+template <int PK> struct Lay {
+  static constexpr int T = Plan<PK>::T, RA = Plan<PK>::R1;
+  static constexpr int RB = Plan<PK>::R3 > 1 ? Plan<PK>::R3 : Plan<PK>::R2;
 };
-template <int L>
+template <int PK>
 __host__ __device__ __forceinline__ constexpr int rowA(int e) {
-  return (e / Lay<L>::RA) * Lay<L>::T + (e % Lay<L>::RA) * (L / Lay<L>::RA);
+  return (e / Lay<PK>::RA) * Lay<PK>::T + (e % Lay<PK>::RA) * (plan_len(PK) / Lay<PK>::RA);
 }
-template <int L>
+template <int PK>
 __host__ __device__ __forceinline__ constexpr int rowB(int e) {
-  return (e / Lay<L>::RB) * Lay<L>::T + (e % Lay<L>::RB) * (L / Lay<L>::RB);
+  return (e / Lay<PK>::RB) * Lay<PK>::T + (e % Lay<PK>::RB) * (plan_len(PK) / Lay<PK>::RB);
 }
-template <int L>
-__host__ __device__ __forceinline__ constexpr int layA(int e, int t) { return t + rowA<L>(e); }
-template <int L>
-__host__ __device__ __forceinline__ constexpr int layB(int e, int t) { return t + rowB<L>(e); }
+template <int PK>
+__host__ __device__ __forceinline__ constexpr int layA(int e, int t) { return t + rowA<PK>(e); }
+template <int PK>
+__host__ __device__ __forceinline__ constexpr int layB(int e, int t) { return t + rowB<PK>(e); }
 
 // Compile-time checks: for two-pass plans the layouts are the two-pass
 // Stockham ones (idxA / idxB of R1, R2), and every layout maps the T x E
 // (thread, slot) pairs one-to-one onto 0 .. L-1.
-template <int L>
+template <int PK>
 constexpr bool layouts_ok() {
-  constexpr int T = Plan<L>::T, E = L / T, R1 = Plan<L>::R1, R2 = Plan<L>::R2;
+  constexpr int L = plan_len(PK);
+  constexpr int T = Plan<PK>::T, E = L / T, R1 = Plan<PK>::R1, R2 = Plan<PK>::R2;
+  static_assert(Plan<PK>::R1 * Plan<PK>::R2 * Plan<PK>::R3 == L, "plan factorisation");
   bool seenA[L] = {}, seenB[L] = {};
   for (int t = 0; t < T; ++t)
     for (int e = 0; e < E; ++e) {
-      const int a = layA<L>(e, t), b = layB<L>(e, t);
+      const int a = layA<PK>(e, t), b = layB<PK>(e, t);
       if (a < 0 || a >= L || b < 0 || b >= L || seenA[a] || seenB[b]) return false;
       seenA[a] = seenB[b] = true;
-      if (Plan<L>::R3 == 1) {
+      if (Plan<PK>::R3 == 1) {
         if (a != (t + (e / R1) * T) + (e % R1) * R2) return false;
         if (b != (t + (e / R2) * T) + (e % R2) * R1) return false;
       }
@@ -348,5 +366,9 @@ constexpr bool layouts_ok() {
 static_assert(layouts_ok<16>() && layouts_ok<32>() && layouts_ok<48>() && layouts_ok<64>() &&
                   layouts_ok<96>() && layouts_ok<128>() && layouts_ok<192>() && layouts_ok<256>(),
               "line layouts");
+static_assert(layouts_ok<plan_key(64, 1)>() && layouts_ok<plan_key(96, 1)>() &&
+                  layouts_ok<plan_key(128, 1)>() && layouts_ok<plan_key(192, 1)>() &&
+                  layouts_ok<plan_key(256, 1)>(),
+              "alternative line layouts");
 
 }  // namespace pty

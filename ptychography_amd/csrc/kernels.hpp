@@ -111,14 +111,15 @@ enum RowSrc { SRC_OP = 0, SRC_DM = 1, SRC_F = 2, SRC_HIO = 3 };
 // registers: at three work-groups per CU they spilled 88 bytes per lane; at
 // two they do not (DM with background at 4096 x 256^2 frames: 9.61 vs
 // 9.65-9.70 ms per iteration).
-template <int L, int SRC>
+template <int PK, int SRC>
 __global__ __launch_bounds__(WG, (SRC == SRC_DM || SRC == SRC_HIO) ? 2 : 3)
 void k_rows_fwd(Geom g, const float2* O, const float2* P,
                                                  const float2* F, float2* X,
                                                  const float2* __restrict__ tw, int nrows,
                                                  float w) {
-  constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
-  constexpr int W = LineLds<L>::words;
+  constexpr int L = plan_len(PK);
+  constexpr int T = Plan<PK>::T, E = L / T, LPW = WG / T;
+  constexpr int W = LineLds<PK>::words;
   constexpr bool HAS_F = (SRC == SRC_DM || SRC == SRC_HIO);
   constexpr int E2 = HAS_F ? E : 1;  // f values (DM, HIO)
   __shared__ float2 lds[LPW * W];
@@ -135,14 +136,14 @@ void k_rows_fwd(Geom g, const float2* O, const float2* P,
     const size_t rbase = (size_t)row * L;
     if constexpr (SRC == SRC_F) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) n0[e] = load_stream(&F[rbase + layA<L>(e, t)]);
+      for (int e = 0; e < E; ++e) n0[e] = load_stream(&F[rbase + layA<PK>(e, t)]);
     } else {
       const int n = row / M, i = row % M;
       const int2 o = g.off[n];
       const float2* Orow = O + (size_t)(o.x + i) * g.V + o.y;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const int c = layA<L>(e, t);
+        const int c = layA<PK>(e, t);
         n0[e] = Orow[c];
         if constexpr (HAS_F) n2[e] = load_stream(&F[rbase + c]);
       }
@@ -160,19 +161,19 @@ void k_rows_fwd(Geom g, const float2* O, const float2* P,
       const float2* Prow = probe_row(g, P, row, row % M);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        float2 psi = cmul(n0[e], Prow[layA<L>(e, t)]);
+        float2 psi = cmul(n0[e], Prow[layA<PK>(e, t)]);
         if constexpr (SRC == SRC_DM) psi = make_float2(2.f * psi.x - n2[e].x, 2.f * psi.y - n2[e].y);
         if constexpr (SRC == SRC_HIO) psi = make_float2(n2[e].x + w * psi.x, n2[e].y + w * psi.y);
         v[e] = psi;
       }
     }
     if (grp + (int)gridDim.x < ngroups) load(grp + gridDim.x);
-    fft_fwd<L>(v, lds + line * W, s_tw, t);
+    fft_fwd<PK>(v, lds + line * W, s_tw, t);
     const int row = grp * LPW + line;
     if (row < nrows) {
       const size_t rbase = (size_t)row * L;
 #pragma unroll
-      for (int e = 0; e < E; ++e) store_stream(&X[rbase + layB<L>(e, t)], v[e]);
+      for (int e = 0; e < E; ++e) store_stream(&X[rbase + layB<PK>(e, t)], v[e]);
     }
     lds_barrier();
   }
@@ -199,13 +200,14 @@ struct HioMix {
 //   DST_DMUPD: Out[n, i, c] += y - O[..] P[i, c]        (DM: f - Ps f + Pm(..))
 //              partial[block] = sum |y - O P|^2          (DM step size)
 //   DST_HIO  : f = F + w O[..] P[i, c]; Out = a y - b f, F = f - beta y (HioMix)
-template <int L, int DST>
+template <int PK, int DST>
 __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, float2* Out,
                                                  const float2* O, const float2* P, float scale,
                                                  double* partial, const float2* __restrict__ tw,
                                                  int nrows, HioMix hm) {
-  constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
-  constexpr int W = LineLds<L>::words;
+  constexpr int L = plan_len(PK);
+  constexpr int T = Plan<PK>::T, E = L / T, LPW = WG / T;
+  constexpr int W = LineLds<PK>::words;
   __shared__ float2 lds[LPW * W];
   __shared__ float2 s_tw[L];
   __shared__ double red[4];
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
     const int row = min(grp * LPW + line, nrows - 1);
     const size_t rbase = (size_t)row * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nx[e] = load_stream(&Y[rbase + layB<L>(e, t)]);
+    for (int e = 0; e < E; ++e) nx[e] = load_stream(&Y[rbase + layB<PK>(e, t)]);
   };
   double acc[1] = {0.0};
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -235,14 +237,14 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
     if constexpr (DST == DST_HIO) {
       const size_t rb = (size_t)min(row, nrows - 1) * L;
 #pragma unroll
-      for (int e = 0; e < E; ++e) fpre[e] = load_stream(&hm.F[rb + layA<L>(e, t)]);
+      for (int e = 0; e < E; ++e) fpre[e] = load_stream(&hm.F[rb + layA<PK>(e, t)]);
     }
-    fft_inv<L>(v, lds + line * W, s_tw, t);
+    fft_inv<PK>(v, lds + line * W, s_tw, t);
     if (row < nrows) {
       const size_t rbase = (size_t)row * L;
       if constexpr (DST == DST_STORE) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) store_stream(&Out[rbase + layA<L>(e, t)], v[e] * scale);
+        for (int e = 0; e < E; ++e) store_stream(&Out[rbase + layA<PK>(e, t)], v[e] * scale);
       } else if constexpr (DST == DST_HIO) {
         const int M = g.M, n = row / M, i = row % M;
         const int2 o = g.off[n];
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
         const float2* Prow = hm.P + (size_t)i * L;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const int c = layA<L>(e, t);
+          const int c = layA<PK>(e, t);
           const float2 m = v[e] * scale;
           float2 fv = fpre[e];
           if (hm.w != 0.f) {
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_inv(Geom g, const float2* Y, flo
         const float2* Prow = probe_row(g, P, row, i);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-          const int c = layA<L>(e, t);
+          const int c = layA<PK>(e, t);
           const float2 ex = cmul(Orow[c], Prow[c]);
           const float2 y = v[e] * scale;
           const float2 d = y - ex;
@@ -352,7 +354,7 @@ template <int MODE> struct ColTraits {
                                 MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
 };
 
-// Launch shape of k_cols<L, MODE>.  Pmod_1 modes with at most 16 points per
+// Launch shape of k_cols<PK, MODE>.  Pmod_1 modes with at most 16 points per
 // thread load each frame's spectra and amplitudes at the top of the frame step
 // (amplitudes in flight across the forward FFT) instead of prefetching the
 // next frame's spectra into registers, and take their mask packed one word per
@@ -361,8 +363,9 @@ template <int MODE> struct ColTraits {
 // loads, 3 waves) -> 1.28 (packed mask, 4 waves) -> 1.18 ms with 512-thread
 // work-groups (32-column blocks).  With 24 points per thread (96, 192) the
 // hoisted amplitudes would cost 2 waves per SIMD; those keep the prefetch.
-template <int L, int MODE> struct ColShape {
-  static constexpr int E = L / Plan<L>::T;
+template <int PK, int MODE> struct ColShape {
+  static constexpr int L = plan_len(PK);
+  static constexpr int E = L / Plan<PK>::T;
   static constexpr bool modulus = MODE == COL_ERA1 || MODE == COL_DM1 || MODE == COL_ERR1 ||
                                   MODE == COL_ERR7 || MODE == COL_ERAB ||
                                   MODE == COL_DMB_LEAD || MODE == COL_DMB_UPD ||
@@ -371,9 +374,11 @@ template <int L, int MODE> struct ColShape {
   // modes that also hold background values across the FFT fit 3 waves
   static constexpr bool bgvals = MODE == COL_ERR7 || MODE == COL_ERAB || MODE == COL_DMB_LEAD ||
                                  MODE == COL_DMB_UPD || MODE == COL_DMBF_LEAD || MODE == COL_DMBF;
+  // (plans with at most 8 points per thread hold half the values: one or two
+  // waves more; COL_DMBF spills at 4)
   static constexpr int wpe =
-      MODE == COL_DMBF ? 2
-      : hoist ? (bgvals ? ((MODE == COL_DMB_UPD && L == 128) ? 2 : 3) : 4)
+      MODE == COL_DMBF ? (E <= 8 ? 3 : 2)
+      : hoist ? (bgvals ? (E <= 8 ? 4 : (MODE == COL_DMB_UPD && L == 128) ? 2 : 3) : 4)
               : ColTraits<MODE>::heavy ? 2 : 3;  // waves per SIMD
 };
 
@@ -397,10 +402,11 @@ __device__ __forceinline__ constexpr int urowB(int e) { return (e / Rb) * T + (e
 // mask of the pixel that column-thread t keeps in register slot e of column c
 // (row t + urowB(e)), so a Pmod_1 column step reads one word per column
 // instead of E bytes per frame.
-template <int L>
+template <int PK>
 __global__ __launch_bounds__(WG) void k_pack_mask(const uint8_t* __restrict__ mask, int K,
                                                   uint32_t* __restrict__ bits) {
-  constexpr int T = Plan<L>::T, E = L / T;
+  constexpr int L = plan_len(PK);
+  constexpr int T = Plan<PK>::T, E = L / T;
   static_assert(E <= 32, "one bit per register slot");
   const int idx = blockIdx.x * WG + threadIdx.x;
   if (idx >= T * K) return;
@@ -408,15 +414,16 @@ __global__ __launch_bounds__(WG) void k_pack_mask(const uint8_t* __restrict__ ma
   uint32_t b = 0;
 #pragma unroll
   for (int e = 0; e < E; ++e)
-    if (mask[(size_t)(t + rowB<L>(e)) * K + c]) b |= 1u << e;
+    if (mask[(size_t)(t + rowB<PK>(e)) * K + c]) b |= 1u << e;
   bits[idx] = b;
 }
 
-template <int L, int MODE, int CWG = WG>
+template <int PK, int MODE, int CWG = WG>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
-                          amdgpu_waves_per_eu(ColShape<L, MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
+                          amdgpu_waves_per_eu(ColShape<PK, MODE>::wpe))) void k_cols(Geom g, ColArgs a) {
+  constexpr int L = plan_len(PK);
   using TR = ColTraits<MODE>;
-  constexpr bool HOIST = ColShape<L, MODE>::hoist;
+  constexpr bool HOIST = ColShape<PK, MODE>::hoist;
   // hoisted per-frame (or shared) background values; the frame mean bbar is
   // the same for every frame (an L2 hit) and is read where it is used, which
   // keeps DMB_UPD and ERR7 within their register budget
@@ -424,8 +431,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                                  MODE == COL_DMBF_LEAD || MODE == COL_DMBF);
   constexpr bool DMBF = MODE == COL_DMBF_LEAD || MODE == COL_DMBF || MODE == COL_DM1F;
   static_assert(MODE != COL_DMBF || HOIST, "the second step reads the hoisted amplitudes");
-  constexpr int T = Plan<L>::T, E = L / T, CPW = CWG / T;
-  constexpr int W = LineLds<L>::words;
+  constexpr int T = Plan<PK>::T, E = L / T, CPW = CWG / T;
+  constexpr int W = LineLds<PK>::words;
   __shared__ float2 lds[CPW * W];
   __shared__ float2 s_tw[L];
   __shared__ double red[2 * (CWG / 64)];
@@ -445,7 +452,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     const float2* Xn = a.X + (size_t)n * L * K;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int u = TR::fwd ? rowA<L>(e) : rowB<L>(e);
+      const int u = TR::fwd ? rowA<PK>(e) : rowB<PK>(e);
       nx[e] = load_stream(&(Xn + (size_t)u * K)[toff]);
     }
   };
@@ -489,11 +496,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         const float* ampn = a.amp + (size_t)nu * a.amp_stride;
 #pragma unroll
         for (int e = 0; e < E; ++e)
-          amh[e] = cv ? load_stream(&(ampn + (size_t)rowB<L>(e) * K)[toff]) : 0.f;
+          amh[e] = cv ? load_stream(&(ampn + (size_t)rowB<PK>(e) * K)[toff]) : 0.f;
         if constexpr (HBG) {
           const float* bgn = a.bg + (size_t)nu * a.bg_stride;
 #pragma unroll
-          for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)rowB<L>(e) * K)[toff] : 0.f;
+          for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)rowB<PK>(e) * K)[toff] : 0.f;
         }
       }
 #pragma unroll
@@ -507,7 +514,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           load(ni, (ni / ncb) * a.chunk);
         }
       }
-      if constexpr (TR::fwd) fft_fwd<L>(v, mid, s_tw, t);
+      if constexpr (TR::fwd) fft_fwd<PK>(v, mid, s_tw, t);
 
       if constexpr (MODE != COL_FWD && MODE != COL_INV) {
         if (cv) {
@@ -515,7 +522,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           float* bgn = a.bg ? a.bg + (size_t)nu * a.bg_stride : nullptr;
 #pragma unroll
           for (int e = 0; e < E; ++e) {
-            const int u = rowB<L>(e);
+            const int u = rowB<PK>(e);
             const size_t uo = (size_t)u * K;
             const float2 z = v[e];
             const float z2 = norm2(z);
@@ -647,7 +654,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       if constexpr (TR::store_fwd) {
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (Xn + (size_t)rowB<L>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) (Xn + (size_t)rowB<PK>(e) * K)[toff] = v[e];
         }
       }
       if constexpr (TR::inv) {
@@ -656,11 +663,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
 #pragma unroll
           for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
         }
-        fft_inv<L>(v, mid, s_tw, t);
+        fft_inv<PK>(v, mid, s_tw, t);
         float2* dst = MODE == COL_DMBF_LEAD ? a.Xo + (size_t)nu * L * K : Xn;
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (dst + (size_t)rowA<L>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) (dst + (size_t)rowA<PK>(e) * K)[toff] = v[e];
         }
       }
       if constexpr (MODE == COL_DMBF) {
@@ -669,7 +676,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         if (cv) {
 #pragma unroll
           for (int e = 0; e < E; ++e) {
-            const size_t uo = (size_t)rowB<L>(e) * K;
+            const size_t uo = (size_t)rowB<PK>(e) * K;
             const float2 fn = (Fn + uo)[toff];
             const float m = ((mbits >> e) & 1u) ? 1.f : 0.f;
             const float mg = msqrt(norm2(fn) + bnh[e] * bnh[e] + a.alpha);
@@ -682,11 +689,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
         }
         lds_barrier();
-        fft_inv<L>(v, mid, s_tw, t);
+        fft_inv<PK>(v, mid, s_tw, t);
         float2* Xon = a.Xo + (size_t)nu * L * K;
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (Xon + (size_t)rowA<L>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) (Xon + (size_t)rowA<PK>(e) * K)[toff] = v[e];
         }
       }
       lds_barrier();
@@ -696,14 +703,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       if (cv) {
         double* pb = a.partb + (size_t)chunk * L * K;
 #pragma unroll
-        for (int e = 0; e < E; ++e) (pb + (size_t)rowB<L>(e) * K)[toff] = (double)bacc[e];
+        for (int e = 0; e < E; ++e) (pb + (size_t)rowB<PK>(e) * K)[toff] = (double)bacc[e];
       }
     }
     if constexpr (TR::bsum2) {
       if (cv) {
         double* pb = a.partb2 + (size_t)chunk * L * K;
 #pragma unroll
-        for (int e = 0; e < E; ++e) (pb + (size_t)rowB<L>(e) * K)[toff] = (double)bacc2[e];
+        for (int e = 0; e < E; ++e) (pb + (size_t)rowB<PK>(e) * K)[toff] = (double)bacc2[e];
       }
     }
   }
@@ -928,11 +935,12 @@ __device__ __forceinline__ float2 add_line_range(float2 s, const float2* lds, in
 // two batches ahead, then this batch's probe row, then the next batch's
 // spectra), so no wait on the in-order load counter also waits for the
 // prefetch of the next batch.
-template <int L>
+template <int PK>
 __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
                                                     const float2* __restrict__ tw) {
-  constexpr int T = Plan<L>::T, E = L / T, LPW = WG / T;
-  constexpr int W = LineLds<L>::words;
+  constexpr int L = plan_len(PK);
+  constexpr int T = Plan<PK>::T, E = L / T, LPW = WG / T;
+  constexpr int W = LineLds<PK>::words;
   constexpr int OFF_NONE = -(1 << 29);  // column offset of a spare line: matches no column
   static_assert(W >= L, "a line's exchange area must hold the parked row");
   static_assert(64 % T == 0, "a line must lie within one wave");
@@ -956,7 +964,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
   auto load = [&](int2 m) {
     const size_t rbase = (size_t)m.x * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nx[e] = a.Y[rbase + layB<L>(e, t)];
+    for (int e = 0; e < E; ++e) nx[e] = a.Y[rbase + layB<PK>(e, t)];
   };
   int2 m_cur = make_int2(0, 0), m_next = make_int2(0, 0);  // .y = x_n << 16 | i
   if (nbatch > 0) {
@@ -969,26 +977,26 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
     const float2* Prow = probe_row(g, a.P, m_cur.x, i);
     float2 pv[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) pv[e] = Prow[layA<L>(e, t)];
+    for (int e = 0; e < E; ++e) pv[e] = Prow[layA<PK>(e, t)];
     const int2 m_after = entry(bt + 2);
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nx[e];
     if (bt + 1 < nbatch) load(m_next);
-    fft_inv<L>(v, lds + line * W, s_tw, t);
+    fft_inv<PK>(v, lds + line * W, s_tw, t);
     if (a.Fout) {
       const int xn = m_cur.y >> 16;
       if (bt * LPW + line < cnt && xn >= v0 && xn < v0 + a.segw) {
         const size_t rbase = (size_t)m_cur.x * L;
 #pragma unroll
-        for (int e = 0; e < E; ++e) store_stream(&a.Fout[rbase + layA<L>(e, t)], v[e]);
+        for (int e = 0; e < E; ++e) store_stream(&a.Fout[rbase + layA<PK>(e, t)], v[e]);
       }
     }
     // park conj(P) * psi' in this line's area: every value depends on the
     // pass-2 reads of the same wave, so no barrier is needed before it
     float2* park = lds + line * W;
 #pragma unroll
-    for (int e = 0; e < E; ++e) park[layA<L>(e, t)] = cmulc(v[e], pv[e]);
+    for (int e = 0; e < E; ++e) park[layA<PK>(e, t)] = cmulc(v[e], pv[e]);
     if (t == 0) s_oy[line] = (bt * LPW + line < cnt) ? (m_cur.y >> 16) : OFF_NONE;
     lds_barrier();
     // lane l < LPW: line l's column offset; other lanes hold OFF_NONE

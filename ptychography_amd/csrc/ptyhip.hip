@@ -53,11 +53,19 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 #define PTY_SIZES(X) X(16) X(32) X(64) X(96) X(128) X(192) X(256)
+// every line-FFT plan the row / column kernels are built for: the default
+// plan of each side and the alternatives (fft.hpp)
+// stage bits of a context's plan mask (pty_set_plan)
+constexpr int PLAN_ROWS = PTY_PLAN_ROWS, PLAN_COLS = PTY_PLAN_COLS, PLAN_ROWG = PTY_PLAN_ROWG;
+constexpr int PLAN_ALL = PLAN_ROWS | PLAN_COLS | PLAN_ROWG;
+#define PTY_PLANS(X) PTY_SIZES(X) X(plan_key(64, 1)) X(plan_key(96, 1)) X(plan_key(128, 1)) \
+  X(plan_key(192, 1)) X(plan_key(256, 1))
 
-int threads_per_line(int L) {
-  switch (L) {
+// threads per line of plan key PK (for a side L: its default plan); 0 = none
+int threads_per_line(int PK) {
+  switch (PK) {
 #define C(Lv) case Lv: return Plan<Lv>::T;
-    PTY_SIZES(C)
+    PTY_PLANS(C)
 #undef C
     default: return 0;
   }
@@ -245,6 +253,11 @@ struct pty_ctx {
   bool have_metric_state = false;
   int last_algo = -1;
 
+  // line-FFT plan of each kernel stage, a bit mask: bit PLAN_ROWS (row
+  // kernels), PLAN_COLS (column kernels), PLAN_ROWG (fused row gather); a set
+  // bit selects the side's alternative plan (sides without one keep the default)
+  int plan = 0;
+
   // packed scans (pty_set_scans): scans of one frame size whose frames form
   // this context's stack in scan order and whose objects occupy consecutive
   // row bands of its object; every stage then runs once over all of them
@@ -314,6 +327,12 @@ void prof_collect(pty_ctx* c) {
 // launch helpers
 // --------------------------------------------------------------------------
 
+// plan key of a kernel stage (PLAN_ROWS / PLAN_COLS / PLAN_ROWG) for a frame side L
+int pk(const pty_ctx* c, int L, int stage) {
+  const int alt = plan_key(L, 1);
+  return ((c->plan & stage) && threads_per_line(alt) > 0) ? alt : L;
+}
+
 // an iteration's metrics (3 doubles per scan) in the run's device buffer
 double* metrics_at(pty_ctx* c, int64_t k) { return c->metrics + 3 * (size_t)c->nscan * k; }
 
@@ -328,15 +347,15 @@ template <int SRC>
 int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows,
              float w = 0.f) {
   StageTimer tm(c, PTY_STAGE_ROWS_FWD);
-  const int T = threads_per_line(c->K), lpw = WG / T;
+  const int T = threads_per_line(pk(c, c->K, PLAN_ROWS)), lpw = WG / T;
   dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
-  switch (c->K) {
+  switch (pk(c, c->K, PLAN_ROWS)) {
 #define C(Lv)                                                                              \
   case Lv:                                                                                 \
     hipLaunchKernelGGL((k_rows_fwd<Lv, SRC>), grid, dim3(WG), 0, c->stream, c->g, O, P, F, X, \
                        (const float2*)c->twK, nrows, w);                                   \
     break;
-    PTY_SIZES(C)
+    PTY_PLANS(C)
 #undef C
     default: return fail(PTY_EUNSUP, "unsupported K");
   }
@@ -348,16 +367,16 @@ template <int DST>
 int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const float2* P,
              float scale, double* partial, int nrows, int* nparts = nullptr, HioMix hm = {}) {
   StageTimer tm(c, PTY_STAGE_ROWS_INV);
-  const int T = threads_per_line(c->K), lpw = WG / T;
+  const int T = threads_per_line(pk(c, c->K, PLAN_ROWS)), lpw = WG / T;
   dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
   if (nparts) *nparts = grid.x;
-  switch (c->K) {
+  switch (pk(c, c->K, PLAN_ROWS)) {
 #define C(Lv)                                                                               \
   case Lv:                                                                                  \
     hipLaunchKernelGGL((k_rows_inv<Lv, DST>), grid, dim3(WG), 0, c->stream, c->g, Y, Out, O, P, \
                        scale, partial, (const float2*)c->twK, nrows, hm);                   \
     break;
-    PTY_SIZES(C)
+    PTY_PLANS(C)
 #undef C
     default: return fail(PTY_EUNSUP, "unsupported K");
   }
@@ -366,7 +385,7 @@ int rows_inv(pty_ctx* c, const float2* Y, float2* Out, const float2* O, const fl
 }
 
 int row_blocks(pty_ctx* c, int nrows) {
-  const int lpw = WG / threads_per_line(c->K);
+  const int lpw = WG / threads_per_line(pk(c, c->K, PLAN_ROWS));
   return (nrows + lpw - 1) / lpw;
 }
 
@@ -391,18 +410,19 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
   int wpe = 3;
   bool hoist = false;
-  switch (c->M) {
+  const int pkm = pk(c, c->M, PLAN_COLS);
+  switch (pkm) {
 #define C(Lv)                              \
   case Lv:                                 \
     wpe = ColShape<Lv, MODE>::wpe;         \
     hoist = ColShape<Lv, MODE>::hoist;     \
     break;
-    PTY_SIZES(C)
+    PTY_PLANS(C)
 #undef C
     default: return fail(PTY_EUNSUP, "unsupported M");
   }
   const bool wide = c->M == 256 && (c->cols_wg == 512 || (c->cols_wg == 0 && hoist));
-  const int cpw = (wide ? 512 : WG) / threads_per_line(c->M);
+  const int cpw = (wide ? 512 : WG) / threads_per_line(pkm);
   const long items = (long)((c->K + cpw - 1) / cpw) * chunks;
   // default grid: Pmod_1 modes without the register prefetch take one item per
   // work-group (up to 128 per CU; measured at 4096 x 256^2: 1.04 ms against
@@ -421,14 +441,18 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
     if (grid > c->cap_col) return fail(PTY_EINVAL, "column partials exceed capacity");
   }
   if (wide) {
-    hipLaunchKernelGGL((k_cols<256, MODE, 512>), dim3(grid), dim3(512), 0, c->stream, c->g, a);
+    if (pkm == 256)
+      hipLaunchKernelGGL((k_cols<256, MODE, 512>), dim3(grid), dim3(512), 0, c->stream, c->g, a);
+    else
+      hipLaunchKernelGGL((k_cols<plan_key(256, 1), MODE, 512>), dim3(grid), dim3(512), 0, c->stream,
+                         c->g, a);
   } else {
-    switch (c->M) {
+    switch (pkm) {
 #define C(Lv)                                                                                  \
   case Lv:                                                                                     \
     hipLaunchKernelGGL((k_cols<Lv, MODE>), dim3(grid), dim3(WG), 0, c->stream, c->g, a);       \
     break;
-      PTY_SIZES(C)
+      PTY_PLANS(C)
 #undef C
       default: return fail(PTY_EUNSUP, "unsupported M");
     }
@@ -507,14 +531,23 @@ int gather(pty_ctx* c, const float2* S, const float2* P, const float2* Oold, flo
   return PTY_OK;
 }
 
-// frame sides the fused inverse-row / object-update kernel is built for
+// frame sides the fused inverse-row / object-update kernel is built for, and
+// all their plans
 #define PTY_ROWG_SIZES(X) X(64) X(96) X(128) X(192) X(256)
+#define PTY_ROWG_PLANS(X) PTY_ROWG_SIZES(X) X(plan_key(64, 1)) X(plan_key(96, 1)) \
+  X(plan_key(128, 1)) X(plan_key(192, 1)) X(plan_key(256, 1))
 
-template <int L>
-size_t rowg_static_lds() {
-  constexpr size_t lpw = WG / Plan<L>::T;
-  return sizeof(float2) * (lpw * LineLds<L>::words + 1 + L) + 16 * sizeof(double) + lpw * sizeof(int);
+template <int PK>
+constexpr size_t rowg_static_lds() {
+  constexpr size_t lpw = WG / Plan<PK>::T;
+  return sizeof(float2) * (lpw * LineLds<PK>::words + 1 + plan_len(PK)) + 16 * sizeof(double) +
+         lpw * sizeof(int);
 }
+// object rows are cut into segments for the default plan's static LDS; the
+// alternatives need no more, so the same segments fit
+#define C(Lv) static_assert(rowg_static_lds<plan_key(Lv, 1)>() <= rowg_static_lds<Lv>(), "rowg LDS");
+PTY_ROWG_SIZES(C)
+#undef C
 
 // Dynamic LDS (the row-segment accumulator) never exceeds this: rowg_segments
 // refuses layouts that would need more.
@@ -553,6 +586,9 @@ int rowg_attribute(int K) {
   case Lv:                                                                                    \
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_gather<Lv>),                 \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)ROWG_DYN_MAX);    \
+    if (e == hipSuccess)                                                                      \
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_gather<plan_key(Lv, 1)>),  \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ROWG_DYN_MAX);  \
     break;
     PTY_ROWG_SIZES(C)
 #undef C
@@ -584,13 +620,13 @@ int rows_gather(pty_ctx* c, const float2* Y, float alpha, bool econ, bool esup, 
   a.Fout = Fout;
   const size_t dyn = (size_t)c->rowg_segw * sizeof(float2);
   const int grid = c->U * c->rowg_segs;
-  switch (c->K) {
+  switch (pk(c, c->K, PLAN_ROWG)) {
 #define C(Lv)                                                                                \
   case Lv:                                                                                   \
     hipLaunchKernelGGL(k_rows_gather<Lv>, dim3(grid), dim3(WG), dyn, c->stream, c->g, a,     \
                        (const float2*)c->twK);                                               \
     break;
-    PTY_ROWG_SIZES(C)
+    PTY_ROWG_PLANS(C)
 #undef C
     default: return fail(PTY_EUNSUP, "fused row gather: unsupported K");
   }
@@ -1377,7 +1413,8 @@ std::vector<uint64_t> iter_key(pty_ctx* c, const pty_opts& o, int64_t it) {
           (uint64_t)(uintptr_t)c->X, (uint64_t)(uintptr_t)c->amp, (uint64_t)(uintptr_t)c->mask,
           (uint64_t)(uintptr_t)c->bg, (uint64_t)(uintptr_t)c->metrics,
           (uint64_t)(uintptr_t)c->slots, (uint64_t)(uintptr_t)c->pamp,
-          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->nscan, (uint64_t)c->dmf_run, (uint64_t)c->dm1_run,
+          (uint64_t)(uintptr_t)c->g.Pn, (uint64_t)c->nscan, (uint64_t)c->plan, (uint64_t)c->dmf_run,
+          (uint64_t)c->dm1_run,
           (uint64_t)c->dm1_pending, (uint64_t)c->dm1_par, (uint64_t)c->f_spec,
           (uint64_t)c->dm1_op};
 }
@@ -1935,6 +1972,13 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->cus = cus;
     const char* e_cw = getenv("PTY_COLS_WG");
     if (e_cw && (atoi(e_cw) == 256 || atoi(e_cw) == 512)) c->cols_wg = atoi(e_cw);
+    // line-FFT plans of new contexts (pty_set_plan's mask).  Measured per
+    // stage on scans of each side (ERA 1 / 3, DM 1 / 4): the alternative
+    // plans win only in the fused row gather at 192 and 256 (4-6 %: 32 or 16
+    // threads per line halve its parked rows); PTY_FFT_PLAN overrides
+    c->plan = (K == 192 || K == 256) ? PLAN_ROWG : 0;
+    const char* e_pl = getenv("PTY_FFT_PLAN");
+    if (e_pl) c->plan = std::max(0, std::min(PLAN_ALL, atoi(e_pl)));
     // The modulus step runs over the whole stack at once by default: blocks
     // sized for the 256 MB Infinity Cache (128..512 frames of 256^2) measured
     // 8..18% slower per ERA iteration at 4096 x 256^2, the persistent row and
@@ -2081,38 +2125,43 @@ int pty_prepare_amp(pty_ctx* c, const float* I, const uint8_t* mask_centred, flo
   return PTY_OK;
 }
 
+// The bound mask in the column kernels' register layout of the current plan:
+// one word per (column-thread, column), one set per scan.
+int pack_mask(pty_ctx* c) {
+  const int pkm = pk(c, c->M, PLAN_COLS);
+  const int T = threads_per_line(pkm);
+  const size_t words = (size_t)T * c->K * c->nscan;
+  if (c->cap_maskbits < words) {
+    dev_free(c->maskbits);
+    c->cap_maskbits = 0;
+    PTY_TRY(dev_alloc(&c->maskbits, words));
+    c->cap_maskbits = words;
+  }
+  const int grid = (T * c->K + WG - 1) / WG;
+  for (int sc = 0; sc < c->nscan; ++sc) {
+    const uint8_t* ms = c->mask + (size_t)sc * c->MK();
+    uint32_t* mb = c->maskbits + (size_t)sc * T * c->K;
+    switch (pkm) {
+#define C(Lv)                                                                                  \
+  case Lv:                                                                                     \
+    hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c->stream, ms, c->K, mb);     \
+    break;
+      PTY_PLANS(C)
+#undef C
+      default: return fail(PTY_EUNSUP, "unsupported M");
+    }
+    PTY_LAUNCHED("k_pack_mask");
+  }
+  return PTY_OK;
+}
+
 int pty_set_data(pty_ctx* c, const float* amp, const uint8_t* mask, double I_norm) {
   if (!c || !amp) return fail(PTY_EINVAL, "NULL argument");
   c->amp = amp;
   c->mask = mask;
   c->I_norm = I_norm;
   c->dmf_lead = false;   // the leading Pmod_7 in Xo used the previous data
-  if (mask) {
-    // one packed mask per scan (a packed context's mask is [S][M][K])
-    const int T = threads_per_line(c->M);
-    const size_t words = (size_t)T * c->K * c->nscan;
-    if (c->cap_maskbits < words) {
-      dev_free(c->maskbits);
-      c->cap_maskbits = 0;
-      PTY_TRY(dev_alloc(&c->maskbits, words));
-      c->cap_maskbits = words;
-    }
-    const int grid = (T * c->K + WG - 1) / WG;
-    for (int sc = 0; sc < c->nscan; ++sc) {
-      const uint8_t* ms = mask + (size_t)sc * c->MK();
-      uint32_t* mb = c->maskbits + (size_t)sc * T * c->K;
-      switch (c->M) {
-#define C(Lv)                                                                                  \
-  case Lv:                                                                                     \
-    hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c->stream, ms, c->K, mb);     \
-    break;
-        PTY_SIZES(C)
-#undef C
-        default: return fail(PTY_EUNSUP, "unsupported M");
-      }
-      PTY_LAUNCHED("k_pack_mask");
-    }
-  }
+  if (mask) PTY_TRY(pack_mask(c));
   return PTY_OK;
 }
 
@@ -2428,6 +2477,27 @@ int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t col
 // `o` from the bound state, after one warm-up iteration) and keep the fastest.
 // O, P, the DM exits and the background are restored afterwards, so the
 // caller's next pty_run(begin = 1) starts from the state it set.
+int pty_set_plan(pty_ctx* c, int32_t variant) {
+  if (!c) return fail(PTY_EINVAL, "ctx is NULL");
+  if (variant < 0 || variant > PLAN_ALL) return fail(PTY_EINVAL, "plan mask must be 0..7");
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  if (c->plan == variant) return PTY_OK;
+  const bool cols_change = (c->plan ^ variant) & PLAN_COLS;
+  c->plan = variant;
+  drop_graphs(c);
+  // the column kernels read the mask in their plan's register layout
+  if (c->mask && cols_change) PTY_TRY(pack_mask(c));
+  PTY_HIP(hipStreamSynchronize(c->stream));
+  return PTY_OK;
+}
+
+int pty_get_plan(pty_ctx* c, int32_t* variant, int32_t* has_alternative) {
+  if (!c || !variant || !has_alternative) return fail(PTY_EINVAL, "NULL argument");
+  *variant = c->plan;
+  *has_alternative = (threads_per_line(plan_key(c->M, 1)) > 0 || threads_per_line(plan_key(c->K, 1)) > 0) ? 1 : 0;
+  return PTY_OK;
+}
+
 int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result* out) {
   if (c && c->nscan > 1) return fail(PTY_EUNSUP, "pty_tune: not available on packed scans");
   if (!c || !o || !out || trial_iters < 1) return fail(PTY_EINVAL, "bad pty_tune arguments");
@@ -2459,29 +2529,35 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     if (snap.B) PTY_TRY(copy_d2d(c, c->bg, snap.B, NMK * sizeof(float)));
     return PTY_OK;
   };
-  struct Cfg { int fused, nt, slots, rows, cols, cwg, chunk, rowg; };
+  struct Cfg { int fused, nt, slots, rows, cols, cwg, chunk, rowg, plan; };
   std::vector<Cfg> cands;
-  const int ch = c->chunk_frames, rg = c->rowg;
-  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg});
+  const int ch = c->chunk_frames, rg = c->rowg, pl = c->plan;
+  cands.push_back({c->use_frame, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg, pl});
   if (c->rowg_ok)
-    cands.push_back({0, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, !rg});
+    cands.push_back({0, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, !rg, pl});
   for (int rows : {2, 3, 4})
-    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256, ch, rg});
+    for (int cols : {0, 2, 4}) cands.push_back({0, c->frame_nt, c->frame_slots, rows, cols, 256, ch, rg, pl});
   if (c->M == 256)
-    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512, ch, rg});
+    for (int cols : {0, 1, 3}) cands.push_back({0, c->frame_nt, c->frame_slots, 3, cols, 512, ch, rg, pl});
+  // the other line-FFT plans (radix split and threads per line) of each
+  // stage, with the current launch shapes
+  if (threads_per_line(plan_key(c->M, 1)) > 0 || threads_per_line(plan_key(c->K, 1)) > 0)
+    for (int m = 0; m <= PLAN_ALL; ++m)
+      if (m != pl)
+        cands.push_back({0, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg, m});
   {
     // frame blocks for the modulus step: whole stack, and 1/4 .. 2x the default block
     const size_t per = (size_t)c->M * c->K * 12, budget = (size_t)192 << 20;
     const int base = std::max<int>(1, (int)(budget / per));
     for (int chunk : {0, base / 4, base / 2, base, 2 * base})
-      if (chunk < c->N && chunk != ch) cands.push_back({0, c->frame_nt, c->frame_slots, 3, 0, 256, chunk, rg});
+      if (chunk < c->N && chunk != ch) cands.push_back({0, c->frame_nt, c->frame_slots, 3, 0, 256, chunk, rg, pl});
   }
   if (c->frame_ok)
     for (int nt : {512, 1024})
       for (int sl : {c->cus, 2 * c->cus})
         if (sl <= c->cap_col && sl <= c->cap_esup)
-          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg});
-  auto apply = [&](const Cfg& g) {
+          cands.push_back({1, nt, sl, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg, pl});
+  auto apply = [&](const Cfg& g) -> int {
     c->use_frame = g.fused != 0;
     c->frame_nt = g.nt;
     c->frame_slots = g.slots;
@@ -2491,11 +2567,12 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     c->chunk_frames = g.chunk;
     c->rowg = g.rowg != 0;
     drop_graphs(c);
+    return pty_set_plan(c, g.plan);
   };
   double best = 1e300, first = 0.0;
   int besti = 0;
   for (size_t i = 0; i < cands.size(); ++i) {
-    apply(cands[i]);
+    PTY_TRY(apply(cands[i]));
     PTY_TRY(restore());
     PTY_TRY(pty_run(c, o, 0, 1, 1, nullptr));
     PTY_TRY(restore());
@@ -2508,13 +2585,13 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
     if (i == 0) first = ms;
     if (ms < best) { best = ms; besti = (int)i; }
   }
-  apply(cands[besti]);
+  PTY_TRY(apply(cands[besti]));
   PTY_TRY(restore());
   PTY_HIP(hipStreamSynchronize(c->stream));
   c->H_valid = c->G_valid = false;
   c->have_metric_state = false;
   const Cfg& g = cands[besti];
-  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, g.chunk, g.rowg,
+  *out = pty_tune_result{g.fused, g.nt, g.slots, g.rows, g.cols, g.cwg, g.chunk, g.rowg, g.plan,
                          (int32_t)cands.size(), (float)best, (float)first};
   return PTY_OK;
 }

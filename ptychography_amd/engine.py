@@ -212,6 +212,20 @@ class Engine:
         call("pty_tune", self._ctx, ctypes.byref(o), int(trial_iters), ctypes.byref(r))
         return {f: getattr(r, f) for f, _ in PtyTuneResult._fields_}
 
+    PLAN_ROWS, PLAN_COLS, PLAN_ROWG = 1, 2, 4
+
+    def set_plan(self, mask):
+        """Line-FFT plans per stage (``pty_set_plan``): a set bit (PLAN_ROWS,
+        PLAN_COLS, PLAN_ROWG) selects the side's alternative radix split for
+        the row kernels, the column kernels or the fused row gather."""
+        call("pty_set_plan", self._ctx, int(mask))
+
+    def plan(self):
+        """(current plan mask, the frame has alternative plans)."""
+        v, alt = ctypes.c_int32(), ctypes.c_int32()
+        call("pty_get_plan", self._ctx, ctypes.byref(v), ctypes.byref(alt))
+        return int(v.value), bool(alt.value)
+
     def set_path(self, fused, threads=0, slots=0):
         """Select the fused per-frame kernel (True) or the staged kernels (False)."""
         call("pty_set_path", self._ctx, int(bool(fused)), int(threads), int(slots))

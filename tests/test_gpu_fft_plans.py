@@ -1,0 +1,139 @@
+"""The alternative line-FFT plans (``pty_set_plan``): another radix split of
+each frame side (and, for 64, 128 and 256, twice the threads per line),
+selectable per kernel stage and by the launch-shape tuner.
+
+The transform itself is checked against numpy at every side; whole ERA / DM
+runs with every stage on its alternative plan (mask 7) must agree with the
+default plans (mask 0) to float32 round-off (the same arithmetic in another
+order) and with the complex128 oracle; a mask bound before a column-plan
+switch is repacked for the new register layout."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import ptychography_amd as pty
+from ptychography_amd.engine import Engine
+from tests.scans import broadband_start, rel_l2, small_scan
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 64), (96, 96), (128, 128), (192, 192), (256, 256), (64, 256), (256, 96),
+         (128, 192)]
+
+
+@pytest.mark.parametrize("M,K", SIZES)
+def test_alternative_fft2_matches_numpy(M, K, gpu_device):
+    print(f"fft2 alternative plans {M}x{K}", flush=True)
+    R = -np.array([[0, 0], [2, 3], [4, 6]], dtype=np.int32)
+    eng = Engine(3, M, K, M + 4, K + 6, R, device=gpu_device)
+    eng.set_plan(7)
+    assert eng.plan() == (7, True)
+    rng = np.random.default_rng(M * 7 + K)
+    x = (rng.standard_normal((5, M, K)) + 1j * rng.standard_normal((5, M, K))).astype(np.complex64)
+    d = torch.from_numpy(x).to(gpu_device)
+    eng.fft2(d)
+    ref = np.fft.fftn(x.astype(np.complex128), axes=(-2, -1))
+    assert rel_l2(d.cpu().numpy(), ref) < 2e-6
+    eng.fft2(d, inverse=True)
+    assert rel_l2(d.cpu().numpy(), x) < 3e-6
+    eng.close()
+
+
+def _run(plan, alg, method, s, O0, P0, iters, dev, monkeypatch, **kw):
+    # plan: pty_set_plan's stage mask for every context the run creates
+    monkeypatch.setenv("PTY_FFT_PLAN", str(plan))
+    f = pty.ERA if alg == "era" else pty.DM
+    return f(s.I, s.R.copy(), P0.copy(), O0.copy(), iters, method=method, mask=s.mask,
+             device=dev, **kw)
+
+
+CASES = [("era", 1, {}), ("era", 3, {}), ("dm", 1, {}), ("dm", 3, {}), ("dm", 4, {"bg": True})]
+
+
+@pytest.mark.parametrize("frame", [64, 128, 256])
+@pytest.mark.parametrize("alg,method,extra", CASES)
+def test_alternative_plans_match_default_plans_and_the_oracle(alg, method, extra, frame, gpu_device, monkeypatch):
+    step = frame // 4
+    s = small_scan(frame=frame, grid=(3, 3), step=step, seed=40 + frame, beamstop_radius=frame / 32,
+                   dead_frac=0.04, background_frac=0.01 if extra.get("bg") else 0.0)
+    O0 = broadband_start(s.O.shape, seed=frame)
+    P0 = s.P
+    kw = {}
+    if extra.get("bg"):
+        kw["background"] = np.full(s.I.shape[1:], 0.01 * s.I.mean())
+    iters = 4
+    print(f"{alg.upper()} method {method}, 9 x {frame}^2: alternative vs default plans and the oracle", flush=True)
+    r0 = _run(0, alg, method, s, O0, P0, iters, gpu_device, monkeypatch, **kw)
+    r1 = _run(7, alg, method, s, O0, P0, iters, gpu_device, monkeypatch, **kw)
+    monkeypatch.delenv("PTY_FFT_PLAN")
+    f = oracle.ERA if alg == "era" else oracle.DM
+    rr = f(s.I, s.R.copy(), P0.copy(), O0.copy(), iters, method=method, mask=s.mask, **kw)
+    H = rr[-1]["heatmap"]
+    cov = H > 1e-3 * H.max()
+    O_0, O_1, O_r = r0[0], r1[0], rr[0]
+    d01 = rel_l2(O_1[cov], O_0[cov])
+    d1r = rel_l2(O_1[cov], O_r[cov])
+    d0r = rel_l2(O_0[cov], O_r[cov])
+    e0, e1, er = (np.asarray(r[-1]["eMod"]) for r in (r0, r1, rr))
+    print(f"  object: alternative-default {d01:.1e}, alternative-oracle {d1r:.1e}, "
+          f"default-oracle {d0r:.1e}; eMod alternative-oracle {np.max(np.abs(e1 - er) / er):.1e}",
+          flush=True)
+    # float32 round-off of one transform order against another; DM is not a
+    # contraction and grows one-ulp differences about 1.5x per iteration, so
+    # its two plans need only agree as closely as each agrees with the oracle
+    bound = 1e-5 if alg == "era" else max(1e-5, 2 * d0r)
+    assert d01 < bound
+    assert d1r < max(1e-5, 3 * d0r)
+    assert np.max(np.abs(e1 - e0) / e0) < 1e-5
+    assert np.max(np.abs(e1 - er) / er) < 1e-4
+    if method in (3,):
+        pbound = 1e-5 if alg == "era" else max(1e-5, 2 * rel_l2(r0[1], rr[1]))
+        assert rel_l2(r1[1], r0[1]) < pbound
+
+
+def test_plan_switch_repacks_the_mask(gpu_device):
+    s = small_scan(frame=128, grid=(3, 3), step=32, seed=51, beamstop_radius=6.0, dead_frac=0.1)
+    O0 = broadband_start(s.O.shape, seed=51).astype(np.complex64)
+    R = s.R - s.R.max(axis=0)
+    N, M, K = s.I.shape
+    U, V = O0.shape
+    mc = torch.from_numpy(s.mask.astype(np.uint8)).to(gpu_device)
+    ms = torch.from_numpy(np.ascontiguousarray(np.fft.ifftshift(s.mask.astype(np.uint8)))).to(gpu_device)
+    out = []
+    for plan, switch_after in ((0, False), (2, True), (2, False)):
+        e = Engine(N, M, K, U, V, R, device=gpu_device)
+        if not switch_after:
+            e.set_plan(plan)
+        amp, I_norm = e.prepare_amp(torch.from_numpy(s.I.astype(np.float32)).to(gpu_device), mc)
+        e.set_data(amp, ms, I_norm)
+        if switch_after:
+            e.set_plan(plan)          # the bound mask is repacked for the new layout
+        O_t = torch.from_numpy(O0.copy()).to(gpu_device)
+        P_t = torch.from_numpy(s.P.astype(np.complex64)).to(gpu_device)
+        e.set_state(O_t, P_t)
+        m = e.run(0, 1, 3, begin=True)
+        out.append((m, O_t.cpu().numpy()))
+        e.close()
+    (m0, O_0), (m1, O_1), (m2, O_2) = out
+    assert np.array_equal(m1, m2) and np.array_equal(O_1, O_2)
+    assert np.max(np.abs(m1[:, 0] - m0[:, 0]) / m0[:, 0]) < 1e-5
+    assert rel_l2(O_1, O_0) < 1e-5
+
+
+def test_tuner_reports_its_plan(gpu_device):
+    s = small_scan(frame=64, grid=(4, 4), step=16, seed=52)
+    O0 = broadband_start(s.O.shape, seed=52).astype(np.complex64)
+    R = s.R - s.R.max(axis=0)
+    N, M, K = s.I.shape
+    U, V = O0.shape
+    e = Engine(N, M, K, U, V, R, device=gpu_device)
+    amp, I_norm = e.prepare_amp(torch.from_numpy(s.I.astype(np.float32)).to(gpu_device), None)
+    e.set_data(amp, None, I_norm)
+    e.set_state(torch.from_numpy(O0).to(gpu_device),
+                torch.from_numpy(s.P.astype(np.complex64)).to(gpu_device))
+    r = e.tune(0, 1, trial_iters=2)
+    assert 0 <= r["fft_plan"] <= 7 and e.plan()[0] == r["fft_plan"]
+    assert r["candidates"] >= 17
+    e.close()

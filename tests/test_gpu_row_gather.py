@@ -31,6 +31,9 @@ def _run(s, dev, method, rowg, iters=(1, 6, 5), graphs=False, bg=None, obj_pad=0
     eng = Engine(N, M, K, U + obj_pad, V + 2 * obj_pad, s.R.astype(np.int32), device=dev)
     other = before_run() if before_run else None
     eng.set_path(False)
+    # both paths on the default line-FFT plans, so that only the fusion differs
+    # (the alternative plans are compared in test_gpu_fft_plans.py)
+    eng.set_plan(0)
     eng.set_row_gather(rowg)
     sup, en = eng.row_gather_status()
     assert sup and en == rowg
