@@ -553,19 +553,45 @@ PTY_ROWG_SIZES(C)
 // refuses layouts that would need more.
 constexpr size_t ROWG_DYN_MAX = 64u << 10;
 
-// Column segments per object row for the fused kernel: the fewest that let
-// three work-groups (else two, else one) share a CU's 160 KB of LDS, each
-// segment at least 512 columns wide; 0 = geometry not supported.
-int rowg_segments(int K, int V) {
+// Work-groups of the fused kernel of plan key PK that a CU's registers hold
+// (a 256-thread work-group is one wave per SIMD; 512 VGPRs per SIMD lane,
+// allocated in blocks of 8), at most 8.
+int rowg_vgpr_per_cu(int PK) {
+  hipFuncAttributes fa{};
+  hipError_t e = hipErrorInvalidValue;
+  switch (PK) {
+#define C(Lv) case Lv: e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_rows_gather<Lv>)); break;
+    PTY_ROWG_PLANS(C)
+#undef C
+    default: break;
+  }
+  if (e != hipSuccess || fa.numRegs <= 0) return 3;
+  return std::max(1, std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8)));
+}
+
+// Column segments per object row for the fused kernel of plan key PK: the
+// fewest that let as many work-groups as the registers allow (else fewer)
+// share a CU's 160 KB of LDS, each segment at least 512 columns wide; 0 =
+// geometry not supported.  Measured at 4096 x 256^2 with the alternative
+// plan (five work-groups per CU by registers): two segments per object row
+// 0.653 ms, one segment (three per CU by LDS) 0.736 ms.  PTY_ROWG_SEGS
+// forces a count (1..4).
+int rowg_segments(int PK, int V) {
   size_t st = 0;
-  switch (K) {
+  switch (PK) {
 #define C(Lv) case Lv: st = rowg_static_lds<Lv>(); break;
-    PTY_ROWG_SIZES(C)
+    PTY_ROWG_PLANS(C)
 #undef C
     default: return 0;
   }
   if (V > ROWG_VMAX) return 0;
-  for (int per_cu = 3; per_cu >= 1; --per_cu) {
+  if (const char* e_s = getenv("PTY_ROWG_SEGS")) {
+    const int f = atoi(e_s);
+    if (f >= 1 && f <= 4 && (V + f - 1) / f * sizeof(float2) <= ROWG_DYN_MAX &&
+        st + (V + f - 1) / f * sizeof(float2) <= (size_t)(160u << 10))
+      return f;
+  }
+  for (int per_cu = rowg_vgpr_per_cu(PK); per_cu >= 1; --per_cu) {
     const size_t budget = (size_t)(160u << 10) / per_cu;
     if (budget < st + 512 * sizeof(float2)) continue;
     const size_t w = std::min((budget - st) / sizeof(float2), ROWG_DYN_MAX / sizeof(float2));
@@ -1922,8 +1948,17 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
       for (int tx = off[n].y / TILE_W; tx <= (off[n].y + K - 1) / TILE_W; ++tx)
         frames[fill[ty * c->tiles_x + tx]++] = n;
 
-  // object rows -> ascending frame lists, when the fused kernel applies
-  const int segs = rowg_segments(K, V);
+  // line-FFT plans of new contexts (pty_set_plan's mask).  Measured per
+  // stage on scans of each side (ERA 1 / 3, DM 1 / 4): the alternative
+  // plans win only in the fused row gather at 192 and 256 (4-6 %: 32 or 16
+  // threads per line halve its parked rows); PTY_FFT_PLAN overrides
+  c->plan = (K == 192 || K == 256) ? PLAN_ROWG : 0;
+  if (const char* e_pl = getenv("PTY_FFT_PLAN")) c->plan = std::max(0, std::min(PLAN_ALL, atoi(e_pl)));
+
+  // object rows -> ascending frame lists, when the fused kernel applies; the
+  // segments are cut for the row gather's plan at creation (a later plan
+  // change keeps them: still correct, possibly fewer work-groups per CU)
+  const int segs = rowg_segments(pk(c, K, PLAN_ROWG), V);
   c->rowg_ok = segs > 0 && (size_t)N * M <= ((size_t)1 << 26);
   std::vector<int> rcnt;
   std::vector<int2> rents;
@@ -1972,13 +2007,6 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->cus = cus;
     const char* e_cw = getenv("PTY_COLS_WG");
     if (e_cw && (atoi(e_cw) == 256 || atoi(e_cw) == 512)) c->cols_wg = atoi(e_cw);
-    // line-FFT plans of new contexts (pty_set_plan's mask).  Measured per
-    // stage on scans of each side (ERA 1 / 3, DM 1 / 4): the alternative
-    // plans win only in the fused row gather at 192 and 256 (4-6 %: 32 or 16
-    // threads per line halve its parked rows); PTY_FFT_PLAN overrides
-    c->plan = (K == 192 || K == 256) ? PLAN_ROWG : 0;
-    const char* e_pl = getenv("PTY_FFT_PLAN");
-    if (e_pl) c->plan = std::max(0, std::min(PLAN_ALL, atoi(e_pl)));
     // The modulus step runs over the whole stack at once by default: blocks
     // sized for the 256 MB Infinity Cache (128..512 frames of 256^2) measured
     // 8..18% slower per ERA iteration at 4096 x 256^2, the persistent row and

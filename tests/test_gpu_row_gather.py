@@ -14,6 +14,8 @@ Oracle parity of the fused path is covered by the 64^2 ERA tests in
 test_gpu_solvers.py, which take it by default.
 """
 
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -25,15 +27,23 @@ pytestmark = pytest.mark.gpu
 
 
 def _run(s, dev, method, rowg, iters=(1, 6, 5), graphs=False, bg=None, obj_pad=0, before_run=None,
-         algorithm=0, OP_iters=(1, 1)):
+         algorithm=0, OP_iters=(1, 1), plan=0):
     N, M, K = s.I.shape
     U, V = s.O.shape
-    eng = Engine(N, M, K, U + obj_pad, V + 2 * obj_pad, s.R.astype(np.int32), device=dev)
+    # both paths on the same line-FFT plans (default: the default plans, also
+    # for cutting the row segments at creation), so that only the fusion
+    # differs; the plans themselves are compared in test_gpu_fft_plans.py
+    saved = os.environ.get("PTY_FFT_PLAN")
+    os.environ["PTY_FFT_PLAN"] = str(plan)
+    try:
+        eng = Engine(N, M, K, U + obj_pad, V + 2 * obj_pad, s.R.astype(np.int32), device=dev)
+    finally:
+        if saved is None:
+            del os.environ["PTY_FFT_PLAN"]
+        else:
+            os.environ["PTY_FFT_PLAN"] = saved
     other = before_run() if before_run else None
     eng.set_path(False)
-    # both paths on the default line-FFT plans, so that only the fusion differs
-    # (the alternative plans are compared in test_gpu_fft_plans.py)
-    eng.set_plan(0)
     eng.set_row_gather(rowg)
     sup, en = eng.row_gather_status()
     assert sup and en == rowg
@@ -83,6 +93,16 @@ def test_fused_segmented_rows(frame, grid, step, gpu_device):
     s = small_scan(frame=frame, grid=grid, step=step, jitter=2)
     assert s.O.shape[1] > {256: 2200, 64: 4400}[frame]
     _compare(_run(s, gpu_device, 1, True), _run(s, gpu_device, 1, False))
+
+
+def test_fused_segmented_rows_alternative_plan(gpu_device):
+    # the row gather's alternative 256 plan parks 8 lines of 265 words, so a
+    # work-group takes whole object rows up to ~4300 columns; wider objects
+    # are still cut into two segments
+    print("row gather segmented rows, alternative plan", flush=True)
+    s = small_scan(frame=256, grid=(2, 36), step=130, jitter=2)
+    assert s.O.shape[1] > 4400
+    _compare(_run(s, gpu_device, 1, True, plan=4), _run(s, gpu_device, 1, False, plan=4))
 
 
 def test_contexts_with_different_segment_widths(gpu_device):
