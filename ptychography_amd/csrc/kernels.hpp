@@ -796,7 +796,7 @@ __global__ __launch_bounds__(WG) void k_gather_O(Geom g, GatherArgs a) {
           ok[f][k] = jin && (unsigned)i < (unsigned)M;
           const size_t ro = ok[f][k] ? (size_t)i * K : 0;
           ps[f][k] = ok[f][k] ? Pj[ro] : make_float2(0.f, 0.f);
-          ss[f][k] = (ok[f][k] && Sn) ? Sn[ro] : make_float2(0.f, 0.f);
+          ss[f][k] = (ok[f][k] && Sn) ? load_stream(&Sn[ro]) : make_float2(0.f, 0.f);
         }
       }
 #pragma unroll
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(WG, 3) void k_rows_gather(Geom g, RowGatherArgs a,
   auto load = [&](int2 m) {
     const size_t rbase = (size_t)m.x * L;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nx[e] = a.Y[rbase + layB<PK>(e, t)];
+    for (int e = 0; e < E; ++e) nx[e] = load_stream(&a.Y[rbase + layB<PK>(e, t)]);
   };
   int2 m_cur = make_int2(0, 0), m_next = make_int2(0, 0);  // .y = x_n << 16 | i
   if (nbatch > 0) {
@@ -1087,8 +1087,8 @@ __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
   const int n0 = a.cstart ? a.cstart[blockIdx.y] : blockIdx.y * a.chunk;
   const int n1 = a.cstart ? a.cstart[blockIdx.y + 1] : min(g.N, n0 + a.chunk);
   double nx = 0.0, ny = 0.0, gg = 0.0;
-  // four frames' loads in flight per step; the sums take the frames in order
-  constexpr int FB = 4;
+  // eight frames' loads in flight per step; the sums take the frames in order
+  constexpr int FB = 8;
   for (int nb = n0; nb < n1; nb += FB) {
     float2 ov[FB], sv[FB];
 #pragma unroll
@@ -1096,7 +1096,8 @@ __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
       const int n = min(nb + f, n1 - 1);
       const int2 o = g.off[n];
       ov[f] = a.O[(size_t)(o.x + i) * g.V + o.y + j];
-      sv[f] = a.S[(size_t)n * MK + p];
+      // psi' is read once: streaming loads keep the reused object rows in L2
+      sv[f] = load_stream(&a.S[(size_t)n * MK + p]);
     }
 #pragma unroll
     for (int f = 0; f < FB; ++f) {
