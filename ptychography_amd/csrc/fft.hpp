@@ -44,6 +44,7 @@ __device__ __forceinline__ float2 load_stream(const float2* p) {
   return make_float2(w.x, w.y);
 }
 __device__ __forceinline__ float load_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void store_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void store_stream(float2* p, float2 v) {
   typedef float f2v __attribute__((ext_vector_type(2)));
   const f2v w = {v.x, v.y};

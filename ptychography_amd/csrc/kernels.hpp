@@ -636,14 +636,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                   acc[0] += (double)(m * d * d);
                 }
                 float2* fp = (Fn + uo) + toff;
-                const float2 f = *fp;
+                const float2 f = load_stream(fp);
                 const float2 y = make_float2(2.f * z.x - f.x, 2.f * z.y - f.y);
                 const float mg = msqrt(norm2(y)) + a.alpha;
                 const float fac = (m != 0.f) ? am * mrcp(mg) : 1.f;
                 const float2 yp = y * fac;
                 const float2 fn = make_float2(f.x - z.x + yp.x, f.y - z.y + yp.y);
                 acc[1] += (double)norm2(yp - z);
-                *fp = fn;
+                store_stream(fp, fn);
                 v[e] = fn * a.scale;
               }
             }
@@ -667,7 +667,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         float2* dst = MODE == COL_DMBF_LEAD ? a.Xo + (size_t)nu * L * K : Xn;
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (dst + (size_t)rowA<PK>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) store_stream(&(dst + (size_t)rowA<PK>(e) * K)[toff], v[e]);
         }
       }
       if constexpr (MODE == COL_DMBF) {
@@ -693,7 +693,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         float2* Xon = a.Xo + (size_t)nu * L * K;
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) (Xon + (size_t)rowA<PK>(e) * K)[toff] = v[e];
+          for (int e = 0; e < E; ++e) store_stream(&(Xon + (size_t)rowA<PK>(e) * K)[toff], v[e]);
         }
       }
       lds_barrier();
