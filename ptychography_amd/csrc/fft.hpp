@@ -206,12 +206,40 @@ __device__ __forceinline__ int idxA(int e, int t) { return (t + (e / Ra) * T) + 
 template <int Ra, int Rb, int T>
 __device__ __forceinline__ int idxB(int e, int t) { return (t + (e / Rb) * T) + (e % Rb) * Ra; }
 
+// Bank-conflict-free layout of the three-pass exchange matrices.  A 64-bit
+// LDS access serves 16 lanes per cycle over 16 two-bank slots, so it is free
+// of conflicts when the 16 lanes' float2 addresses differ mod 16.  Pass 2
+// reads mid[k1 S1 + j + Rc m] with k1 = g % Ra fastest and j = g / Ra, and
+// writes mid[j S2 + k1 + Ra kb]: with the row strides N2 + 1 / N3 + 1 those
+// addresses collide (k1 + j mod 16: the 8 x 4 x 8 plan's row gather lost ~64
+// % of its LDS cycles to conflicts).  For Ra in {2, 4, 8}, rows that are a
+// multiple of 16 long and lines of at least 16 threads (each 16-lane group
+// within one line, starting at a multiple of 16) the matrices are stored
+// unpadded with the columns XOR-swizzled: column c of row k1 (pass 1 / 2) at
+// c ^ (k1 16 / Ra), column c of row j (pass 2 / 3) at c ^ (j Ra) mod 16.
+// Every 16-lane access then covers all 16 slots, and padding-free rows
+// keep the exchange area at L words.  Other plans keep odd strides N + 1.
+constexpr bool swz3(int Ra, int N2, int N3, int T) {
+  return T % 16 == 0 && (Ra == 2 || Ra == 4 || Ra == 8) && N2 % 16 == 0 && N3 % 16 == 0;
+}
+// words of the two exchange matrices of a three-pass transform (Ra, Rb, Rc)
+constexpr int words3(int Ra, int Rb, int Rc, int T) {
+  const int N2 = Rb * Rc, N3 = Ra * Rb;
+  const bool sw = swz3(Ra, N2, N3, T);
+  const int a = Ra * (sw ? N2 : N2 + 1), b = Rc * (sw ? N3 : N3 + 1);
+  return a > b ? a : b;
+}
+
 // LDS words (float2) one line needs for its exchange buffer (both directions).
 template <int PK>
 struct LineLds {
   static constexpr int R1 = Plan<PK>::R1, R2 = Plan<PK>::R2, R3 = Plan<PK>::R3;
-  static constexpr int a = R3 > 1 ? R1 * (R2 * R3 + 1) : R2 * (R1 + 1);
-  static constexpr int b = R3 > 1 ? R3 * (R1 * R2 + 1) : R1 * (R2 + 1);
+  // three passes: forward (R1, R2, R3) and inverse (R3, R2, R1) matrices
+  static constexpr int f3 = R3 > 1 ? words3(R1, R2, R3, Plan<PK>::T) : 0;
+  static constexpr int i3 = R3 > 1 ? words3(R3, R2, R1, Plan<PK>::T) : 0;
+  static constexpr int m3 = f3 > i3 ? f3 : i3;
+  static constexpr int a = R3 > 1 ? m3 : R2 * (R1 + 1);
+  static constexpr int b = R3 > 1 ? m3 : R1 * (R2 + 1);
   static constexpr int words = (a > b ? a : b) | 1;  // odd stride between lines
 };
 
@@ -269,6 +297,10 @@ __device__ __forceinline__ void line_fft3(float2* v, float2* mid, const float2* 
   static_assert(N2 % T == 0 && (Ra * Rc) % T == 0 && N3 % T == 0, "T must divide every pass");
   constexpr int E = L / T, NQ1 = N2 / T, NQ2 = (Ra * Rc) / T, NQ3 = N3 / T;
   static_assert(NQ1 * Ra == E && NQ2 * Rb == E && NQ3 * Rc == E, "layout size mismatch");
+  constexpr bool SW = swz3(Ra, N2, N3, T);
+  constexpr int S1 = SW ? N2 : N2 + 1, S2 = SW ? N3 : N3 + 1;
+  auto x1 = [](int k1) { return SW ? (k1 * (16 / Ra)) & 15 : 0; };  // row k1 of matrix 1
+  auto x2 = [](int j) { return SW ? (j * Ra) & 15 : 0; };           // row j of matrix 2
   auto twid = [&](float2 a, int k) {
     float2 w = tw[k];
     if constexpr (INV) w.y = -w.y;
@@ -280,14 +312,14 @@ __device__ __forceinline__ void line_fft3(float2* v, float2* mid, const float2* 
     const int n2 = t + q * T;
 #pragma unroll
     for (int k1 = 0; k1 < Ra; ++k1)
-      mid[k1 * (N2 + 1) + n2] = k1 == 0 ? v[q * Ra] : twid(v[q * Ra + k1], n2 * k1);
+      mid[k1 * S1 + (n2 ^ x1(k1))] = k1 == 0 ? v[q * Ra] : twid(v[q * Ra + k1], n2 * k1);
   });
   lds_barrier();
   static_for<0, NQ2>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const int g = t + q * T, k1 = g % Ra, j = g / Ra;
 #pragma unroll
-    for (int m = 0; m < Rb; ++m) v[q * Rb + m] = mid[k1 * (N2 + 1) + j + Rc * m];
+    for (int m = 0; m < Rb; ++m) v[q * Rb + m] = mid[k1 * S1 + ((j + Rc * m) ^ x1(k1))];
   });
   lds_barrier();
   static_for<0, NQ2>([&](auto qc) {
@@ -296,14 +328,14 @@ __device__ __forceinline__ void line_fft3(float2* v, float2* mid, const float2* 
     dft<Rb, INV>(v + q * Rb);
 #pragma unroll
     for (int kb = 0; kb < Rb; ++kb)
-      mid[j * (N3 + 1) + k1 + Ra * kb] = kb == 0 ? v[q * Rb] : twid(v[q * Rb + kb], Ra * j * kb);
+      mid[j * S2 + ((k1 + Ra * kb) ^ x2(j))] = kb == 0 ? v[q * Rb] : twid(v[q * Rb + kb], Ra * j * kb);
   });
   lds_barrier();
   static_for<0, NQ3>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const int b = t + q * T;
 #pragma unroll
-    for (int j = 0; j < Rc; ++j) v[q * Rc + j] = mid[j * (N3 + 1) + b];
+    for (int j = 0; j < Rc; ++j) v[q * Rc + j] = mid[j * S2 + (b ^ x2(j))];
     dft<Rc, INV>(v + q * Rc);
   });
 }
