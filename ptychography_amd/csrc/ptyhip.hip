@@ -770,7 +770,7 @@ int sum_region(pty_ctx* c, const double* p, int n, double* host_out) {
 int bg_mean(pty_ctx* c, int nchunks, float* out, const double* partb = nullptr) {
   StageTimer tm(c, PTY_STAGE_REDUCE);
   const int MK = (int)c->MK();
-  hipLaunchKernelGGL(k_bg_mean, dim3((MK + WG - 1) / WG), dim3(WG), 0, c->stream,
+  hipLaunchKernelGGL(k_bg_mean, dim3((MK + BG_PIX - 1) / BG_PIX), dim3(WG), 0, c->stream,
                      partb ? partb : (const double*)c->partb, nchunks, MK, c->N, out);
   PTY_LAUNCHED("k_bg_mean");
   return PTY_OK;
