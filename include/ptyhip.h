@@ -217,7 +217,8 @@ int pty_tune(pty_ctx* ctx, const pty_opts* opts, int32_t trial_iters, pty_tune_r
  * whose transforms are numpy.fft).  `mask` sets, per stage, the frame side's
  * alternative radix split instead of its default: PTY_PLAN_ROWS (row
  * kernels), PTY_PLAN_COLS (column kernels), PTY_PLAN_ROWG (the fused inverse
- * row FFT + object update).  Alternatives: 64: 4x4x4 on 16 threads per line
+ * row FFT + object update), PTY_PLAN_BGERR (the model-error pass with
+ * background of DM methods 4-6, a column kernel of its own).  Alternatives: 64: 4x4x4 on 16 threads per line
  * instead of 8x8 on 8; 96: 4x4x6 instead of 6x4x4; 128: 4x8x4 on 16 threads
  * instead of 16x8 on 8; 192: 4x4x12 instead of 12x4x4; 256: 8x4x8 on 32
  * threads instead of 16x16 on 16; sides 16 and 32 keep their plan.  Stages
@@ -226,7 +227,7 @@ int pty_tune(pty_ctx* ctx, const pty_opts* opts, int32_t trial_iters, pty_tune_r
  * operations).  The default mask is the library's per-side choice (or the
  * environment's PTY_FFT_PLAN); pty_tune tries every mask.  pty_get_plan
  * reports the mask and whether the frame has alternatives. */
-enum { PTY_PLAN_ROWS = 1, PTY_PLAN_COLS = 2, PTY_PLAN_ROWG = 4 };
+enum { PTY_PLAN_ROWS = 1, PTY_PLAN_COLS = 2, PTY_PLAN_ROWG = 4, PTY_PLAN_BGERR = 8 };
 int pty_set_plan(pty_ctx* ctx, int32_t mask);
 int pty_get_plan(pty_ctx* ctx, int32_t* mask, int32_t* has_alternative);
 

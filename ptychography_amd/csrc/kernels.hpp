@@ -431,6 +431,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                                  MODE == COL_DMBF_LEAD || MODE == COL_DMBF);
   constexpr bool DMBF = MODE == COL_DMBF_LEAD || MODE == COL_DMBF || MODE == COL_DM1F;
   static_assert(MODE != COL_DMBF || HOIST, "the second step reads the hoisted amplitudes");
+  // COL_ERR7 on plans of at most 8 points per thread also loads the frame
+  // mean background with the amplitudes, ahead of the FFT (16 points per
+  // thread read it where it is used, within their 3-wave register budget)
+  constexpr bool HBB = HOIST && MODE == COL_ERR7 && (L / Plan<PK>::T) <= 8;
   constexpr int T = Plan<PK>::T, E = L / T, CPW = CWG / T;
   constexpr int W = LineLds<PK>::words;
   __shared__ float2 lds[CPW * W];
@@ -485,11 +489,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       const int nu = __builtin_amdgcn_readfirstlane(n);
       float2* Xn = a.X + (size_t)nu * L * K;
       float2 v[E];
-      float amh[HOIST ? E : 1], bgh[HBG ? E : 1];
+      float amh[HOIST ? E : 1], bgh[HBG ? E : 1], bbh[HBB ? E : 1];
       // COL_DMBF keeps b_new for its second step (re-reading it and the
       // amplitudes from cache measured 7.24 vs 6.70 ms per DM iteration at
       // 4096 x 256^2)
       float bnh[MODE == COL_DMBF ? E : 1];
+      // ... and F_new where the 2-wave register budget holds it (256 points
+      // per line on 16 threads: 255 VGPRs; the 128 and 192 plans spill, and
+      // re-read what they just stored)
+      constexpr bool KEEPF = MODE == COL_DMBF && (L == 256 || E <= 8);
+      float2 fnh[KEEPF ? E : 1];
       float2* Fn = DMBF ? a.F + (size_t)nu * L * K : nullptr;
       if constexpr (HOIST) {
         load(item, nu);
@@ -501,6 +510,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           const float* bgn = a.bg + (size_t)nu * a.bg_stride;
 #pragma unroll
           for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)rowB<PK>(e) * K)[toff] : 0.f;
+        }
+        if constexpr (HBB) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) bbh[e] = cv ? (a.bbar + (size_t)rowB<PK>(e) * K)[toff] : 0.f;
         }
       }
 #pragma unroll
@@ -553,7 +566,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 const float d = msqrt(z2) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERR7) {
-                const float b = (a.bbar + uo)[toff];
+                float b;
+                if constexpr (HBB) b = bbh[e];
+                else b = (a.bbar + uo)[toff];
                 const float d = msqrt(z2 + b * b) - am;
                 acc[0] += (double)(m * d * d);
               } else if constexpr (MODE == COL_ERAB || MODE == COL_DMB_LEAD) {
@@ -625,6 +640,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 *fp = fn;
                 *bp = bn;
                 bnh[e] = bn;
+                if constexpr (KEEPF) fnh[e] = fn;
                 bacc[e] += (BAcc)bn;
                 v[e] = fn * a.scale;
               } else if constexpr (MODE == COL_DM1F) {
@@ -671,13 +687,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         }
       }
       if constexpr (MODE == COL_DMBF) {
-        // the next iteration's leading Pmod_7 of (F_new, b_new) into Xo; F_new
-        // is re-read from what this thread just stored
+        // the next iteration's leading Pmod_7 of (F_new, b_new) into Xo
         if (cv) {
 #pragma unroll
           for (int e = 0; e < E; ++e) {
-            const size_t uo = (size_t)rowB<PK>(e) * K;
-            const float2 fn = (Fn + uo)[toff];
+            float2 fn;
+            if constexpr (KEEPF) fn = fnh[e];
+            else fn = (Fn + (size_t)rowB<PK>(e) * K)[toff];
             const float m = ((mbits >> e) & 1u) ? 1.f : 0.f;
             const float mg = msqrt(norm2(fn) + bnh[e] * bnh[e] + a.alpha);
             const float fac = m * amh[e] * mrcp(mg);

@@ -57,7 +57,8 @@ int fail(int code, const std::string& msg) {
 // plan of each side and the alternatives (fft.hpp)
 // stage bits of a context's plan mask (pty_set_plan)
 constexpr int PLAN_ROWS = PTY_PLAN_ROWS, PLAN_COLS = PTY_PLAN_COLS, PLAN_ROWG = PTY_PLAN_ROWG;
-constexpr int PLAN_ALL = PLAN_ROWS | PLAN_COLS | PLAN_ROWG;
+constexpr int PLAN_BGERR = PTY_PLAN_BGERR;
+constexpr int PLAN_ALL = PLAN_ROWS | PLAN_COLS | PLAN_ROWG | PLAN_BGERR;
 #define PTY_PLANS(X) PTY_SIZES(X) X(plan_key(64, 1)) X(plan_key(96, 1)) X(plan_key(128, 1)) \
   X(plan_key(192, 1)) X(plan_key(256, 1))
 
@@ -142,6 +143,7 @@ struct pty_ctx {
   const float* amp = nullptr;
   const uint8_t* mask = nullptr;
   uint32_t* maskbits = nullptr;  // T * K packed mask words for the column stage (owned)
+  uint32_t* maskbits_alt = nullptr;  // the same for the side's alternative plan
   double I_norm = 0.0;
   float2* O = nullptr;
   float2* P = nullptr;
@@ -277,6 +279,7 @@ struct pty_ctx {
   size_t cap_econP_all = 0;       // doubles part_econP holds
   size_t cap_esup_all = 0;        // doubles part_esup holds
   size_t cap_maskbits = 0;        // words maskbits holds
+  size_t cap_maskbits_alt = 0;    // words maskbits_alt holds
 
   size_t MK() const { return (size_t)M * K; }
   size_t NMK() const { return (size_t)N * M * K; }
@@ -410,7 +413,8 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
   const int chunks = (a.nframes + a.chunk - 1) / a.chunk;
   int wpe = 3;
   bool hoist = false;
-  const int pkm = pk(c, c->M, PLAN_COLS);
+  // the background model-error pass (COL_ERR7) has a plan bit of its own
+  const int pkm = pk(c, c->M, MODE == COL_ERR7 ? PLAN_BGERR : PLAN_COLS);
   switch (pkm) {
 #define C(Lv)                              \
   case Lv:                                 \
@@ -421,6 +425,8 @@ int cols(pty_ctx* c, ColArgs a, int* nparts = nullptr) {
 #undef C
     default: return fail(PTY_EUNSUP, "unsupported M");
   }
+  // the packed mask in this plan's register layout
+  if (a.maskbits) a.maskbits = pkm == c->M ? c->maskbits : c->maskbits_alt;
   const bool wide = c->M == 256 && (c->cols_wg == 512 || (c->cols_wg == 0 && hoist));
   const int cpw = (wide ? 512 : WG) / threads_per_line(pkm);
   const long items = (long)((c->K + cpw - 1) / cpw) * chunks;
@@ -1952,7 +1958,7 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
   // stage on scans of each side (ERA 1 / 3, DM 1 / 4): the alternative
   // plans win only in the fused row gather at 192 and 256 (4-6 %: 32 or 16
   // threads per line halve its parked rows); PTY_FFT_PLAN overrides
-  c->plan = (K == 192 || K == 256) ? PLAN_ROWG : 0;
+  c->plan = (K == 192 || K == 256) ? PLAN_ROWG | PLAN_BGERR : 0;
   if (const char* e_pl = getenv("PTY_FFT_PLAN")) c->plan = std::max(0, std::min(PLAN_ALL, atoi(e_pl)));
 
   // object rows -> ascending frame lists, when the fused kernel applies; the
@@ -2098,7 +2104,7 @@ int pty_destroy(pty_ctx* c) {
   dev_free(c->X); dev_free(c->F); dev_free(c->Xp);
   dev_free(c->Obak); dev_free(c->Pbak); dev_free(c->Os); dev_free(c->Oroll); dev_free(c->Ps);
   dev_free(c->H); dev_free(c->G); dev_free(c->bg); dev_free(c->bbar); dev_free(c->bbar_m);
-  dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->pnum); dev_free(c->pg);
+  dev_free(c->pamp); dev_free(c->partb); dev_free(c->maskbits); dev_free(c->maskbits_alt); dev_free(c->pnum); dev_free(c->pg);
   dev_free(c->cm); dev_free(c->metrics); dev_free(c->scal); dev_free(c->slots);
   dev_free(c->sub_d); dev_free(c->Pn); dev_free(c->Dy); dev_free(c->Dx); dev_free(c->pos_terms);
   dev_free(c->Xo); dev_free(c->partb2); dev_free(c->dm1_esup[0]); dev_free(c->dm1_esup[1]);
@@ -2153,32 +2159,38 @@ int pty_prepare_amp(pty_ctx* c, const float* I, const uint8_t* mask_centred, flo
   return PTY_OK;
 }
 
-// The bound mask in the column kernels' register layout of the current plan:
-// one word per (column-thread, column), one set per scan.
+// The bound mask in the column kernels' register layouts: one word per
+// (column-thread, column), one set per scan, for the side's default plan and
+// for its alternative (column modes take either, pty_set_plan).
 int pack_mask(pty_ctx* c) {
-  const int pkm = pk(c, c->M, PLAN_COLS);
-  const int T = threads_per_line(pkm);
-  const size_t words = (size_t)T * c->K * c->nscan;
-  if (c->cap_maskbits < words) {
-    dev_free(c->maskbits);
-    c->cap_maskbits = 0;
-    PTY_TRY(dev_alloc(&c->maskbits, words));
-    c->cap_maskbits = words;
-  }
-  const int grid = (T * c->K + WG - 1) / WG;
-  for (int sc = 0; sc < c->nscan; ++sc) {
-    const uint8_t* ms = c->mask + (size_t)sc * c->MK();
-    uint32_t* mb = c->maskbits + (size_t)sc * T * c->K;
-    switch (pkm) {
+  for (int v = 0; v < 2; ++v) {
+    const int pkm = plan_key(c->M, v);
+    const int T = threads_per_line(pkm);
+    if (T == 0) continue;   // no alternative plan for this side
+    uint32_t*& bits = v ? c->maskbits_alt : c->maskbits;
+    size_t& cap = v ? c->cap_maskbits_alt : c->cap_maskbits;
+    const size_t words = (size_t)T * c->K * c->nscan;
+    if (cap < words) {
+      dev_free(bits);
+      cap = 0;
+      PTY_TRY(dev_alloc(&bits, words));
+      cap = words;
+    }
+    const int grid = (T * c->K + WG - 1) / WG;
+    for (int sc = 0; sc < c->nscan; ++sc) {
+      const uint8_t* ms = c->mask + (size_t)sc * c->MK();
+      uint32_t* mb = bits + (size_t)sc * T * c->K;
+      switch (pkm) {
 #define C(Lv)                                                                                  \
   case Lv:                                                                                     \
     hipLaunchKernelGGL(k_pack_mask<Lv>, dim3(grid), dim3(WG), 0, c->stream, ms, c->K, mb);     \
     break;
-      PTY_PLANS(C)
+        PTY_PLANS(C)
 #undef C
-      default: return fail(PTY_EUNSUP, "unsupported M");
+        default: return fail(PTY_EUNSUP, "unsupported M");
+      }
+      PTY_LAUNCHED("k_pack_mask");
     }
-    PTY_LAUNCHED("k_pack_mask");
   }
   return PTY_OK;
 }
@@ -2507,14 +2519,11 @@ int pty_set_launch(pty_ctx* c, int32_t rows_mult, int32_t cols_mult, int32_t col
 // caller's next pty_run(begin = 1) starts from the state it set.
 int pty_set_plan(pty_ctx* c, int32_t variant) {
   if (!c) return fail(PTY_EINVAL, "ctx is NULL");
-  if (variant < 0 || variant > PLAN_ALL) return fail(PTY_EINVAL, "plan mask must be 0..7");
+  if (variant < 0 || variant > PLAN_ALL) return fail(PTY_EINVAL, "plan mask must be 0..15");
   PTY_HIP(hipStreamSynchronize(c->stream));
   if (c->plan == variant) return PTY_OK;
-  const bool cols_change = (c->plan ^ variant) & PLAN_COLS;
   c->plan = variant;
   drop_graphs(c);
-  // the column kernels read the mask in their plan's register layout
-  if (c->mask && cols_change) PTY_TRY(pack_mask(c));
   PTY_HIP(hipStreamSynchronize(c->stream));
   return PTY_OK;
 }
@@ -2571,7 +2580,8 @@ int pty_tune(pty_ctx* c, const pty_opts* o, int32_t trial_iters, pty_tune_result
   // stage, with the current launch shapes
   if (threads_per_line(plan_key(c->M, 1)) > 0 || threads_per_line(plan_key(c->K, 1)) > 0)
     for (int m = 0; m <= PLAN_ALL; ++m)
-      if (m != pl)
+      // (the background model-error bit matters only to DM methods 4-6)
+      if (m != pl && ((o->algorithm == 1 && o->method >= 4) || ((m ^ pl) & PLAN_BGERR) == 0))
         cands.push_back({0, c->frame_nt, c->frame_slots, c->rows_mult, c->cols_mult, c->cols_wg, ch, rg, m});
   {
     // frame blocks for the modulus step: whole stack, and 1/4 .. 2x the default block

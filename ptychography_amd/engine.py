@@ -212,12 +212,13 @@ class Engine:
         call("pty_tune", self._ctx, ctypes.byref(o), int(trial_iters), ctypes.byref(r))
         return {f: getattr(r, f) for f, _ in PtyTuneResult._fields_}
 
-    PLAN_ROWS, PLAN_COLS, PLAN_ROWG = 1, 2, 4
+    PLAN_ROWS, PLAN_COLS, PLAN_ROWG, PLAN_BGERR = 1, 2, 4, 8
 
     def set_plan(self, mask):
         """Line-FFT plans per stage (``pty_set_plan``): a set bit (PLAN_ROWS,
-        PLAN_COLS, PLAN_ROWG) selects the side's alternative radix split for
-        the row kernels, the column kernels or the fused row gather."""
+        PLAN_COLS, PLAN_ROWG, PLAN_BGERR) selects the side's alternative radix
+        split for the row kernels, the column kernels, the fused row gather or
+        DM-with-background's model-error pass."""
         call("pty_set_plan", self._ctx, int(mask))
 
     def plan(self):
