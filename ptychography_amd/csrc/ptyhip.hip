@@ -126,6 +126,13 @@ struct pty_ctx {
   bool rowg_ok = false;      // geometry supports the fused kernel
   bool rowg = false;         // currently selected
   int rowg_segs = 1, rowg_segw = 0;  // column segments per object row, their width
+  // frame rows listed per object-row segment (by where their window starts),
+  // for the forward row stage of O_win P in object-row order
+  int* rowf_ptr = nullptr;
+  int2* rowf_ent = nullptr;
+  bool rowf_ok = false;      // lists built
+  bool rowf = false;         // currently selected
+  int rowf_segs = 1, rowf_segw = 0;
   float2* twM = nullptr;
   float2* twK = nullptr;
   // column-stage launch shape
@@ -346,10 +353,80 @@ int persistent_grid(long items, long cap) {
   return (int)std::max<long>(NXCD, cap / NXCD * NXCD);
 }
 
+// static LDS of k_rows_fwd_orow<PK>: line exchange areas, twiddles, list chunk
+template <int PK>
+constexpr size_t rowf_static_lds() {
+  return sizeof(float2) * ((WG / Plan<PK>::T) * LineLds<PK>::words + plan_len(PK)) +
+         sizeof(int2) * ROWF_ENT;
+}
+size_t rowf_static(int PK) {
+  switch (PK) {
+#define C(Lv) case Lv: return rowf_static_lds<Lv>();
+    PTY_PLANS(C)
+#undef C
+    default: return 0;
+  }
+}
+// The object-row forward stage's dynamic LDS (staged object columns) never
+// exceeds this.
+constexpr size_t ROWF_DYN_MAX = 64u << 10;
+
+// Column segments of the object-row forward stage for plan key PK: the
+// fewest whose staged columns (segment + L - 1) let three work-groups share
+// a CU's 160 KB of LDS, each segment at least L wide; 0 = not supported.
+int rowf_segments(int PK, int L, int V) {
+  const size_t st = rowf_static(PK);
+  if (st == 0 || V > ROWG_VMAX) return 0;
+  const size_t budget = (size_t)(160u << 10) / 3;
+  if (budget < st + 2 * (size_t)L * sizeof(float2)) return 0;
+  const size_t cols = std::min((budget - st) / sizeof(float2), ROWF_DYN_MAX / sizeof(float2));
+  const size_t segw = cols - (L - 1);
+  return (int)((V + segw - 1) / segw);
+}
+
+int rowf_attribute() {
+  hipError_t e = hipSuccess;
+#define C(Lv)                                                                                 \
+  if (e == hipSuccess)                                                                        \
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_fwd_orow<Lv>),              \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)ROWF_DYN_MAX);
+  PTY_PLANS(C)
+#undef C
+  if (e != hipSuccess) return fail(PTY_EHIP, std::string("k_rows_fwd_orow LDS attribute: ") + hipGetErrorString(e));
+  return PTY_OK;
+}
+
+// O_win P over the whole stack, row-transformed, in object-row order
+// (k_rows_fwd_orow); false when the lists do not apply to the current plan
+bool rows_fwd_orow(pty_ctx* c, const float2* O, const float2* P, float2* X) {
+  const int PK = pk(c, c->K, PLAN_ROWS);
+  const size_t dyn = (size_t)std::min(c->V, c->rowf_segw + c->K - 1) * sizeof(float2);
+  if (dyn > ROWF_DYN_MAX || rowf_static(PK) + dyn > (size_t)(160u << 10)) return false;
+  RowFwdList a{c->rowf_segs, c->rowf_segw, c->rowf_ptr, c->rowf_ent};
+  const int grid = c->U * c->rowf_segs;
+  switch (PK) {
+#define C(Lv)                                                                                \
+  case Lv:                                                                                   \
+    hipLaunchKernelGGL(k_rows_fwd_orow<Lv>, dim3(grid), dim3(WG), dyn, c->stream, c->g, O, P, X, a, \
+                       (const float2*)c->twK);                                               \
+    break;
+    PTY_PLANS(C)
+#undef C
+    default: return false;
+  }
+  return true;
+}
+
 template <int SRC>
 int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows,
              float w = 0.f) {
   StageTimer tm(c, PTY_STAGE_ROWS_FWD);
+  if constexpr (SRC == SRC_OP) {
+    if (c->rowf && nrows == c->N * c->M && rows_fwd_orow(c, O, P, X)) {
+      PTY_LAUNCHED("k_rows_fwd_orow");
+      return PTY_OK;
+    }
+  }
   const int T = threads_per_line(pk(c, c->K, PLAN_ROWS)), lpw = WG / T;
   dim3 grid(persistent_grid((nrows + lpw - 1) / lpw, (long)c->rows_mult * c->cus));
   switch (pk(c, c->K, PLAN_ROWS)) {
@@ -1987,6 +2064,27 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
           rents[rfill[(off[n].x + i) * segs + sg]++] = make_int2(n * M + i, off[n].y << 16 | i);
   }
 
+  // frame rows -> object-row segment lists for the forward row stage, cut for
+  // the row plan at creation (a later plan change keeps them if its LDS fits)
+  const int fsegs = rowf_segments(pk(c, K, PLAN_ROWS), K, V);
+  c->rowf_ok = fsegs > 0 && (size_t)N * M <= ((size_t)1 << 26);
+  std::vector<int> fcnt;
+  std::vector<int2> fents;
+  if (c->rowf_ok) {
+    c->rowf_segs = fsegs;
+    c->rowf_segw = (V + fsegs - 1) / fsegs;
+    const int sw = c->rowf_segw, nitem = U * fsegs;
+    fcnt.assign(nitem + 1, 0);
+    for (int n = 0; n < N; ++n)
+      for (int i = 0; i < M; ++i) fcnt[(off[n].x + i) * fsegs + off[n].y / sw + 1]++;
+    for (int k = 0; k < nitem; ++k) fcnt[k + 1] += fcnt[k];
+    fents.resize(fcnt[nitem]);
+    std::vector<int> ffill(fcnt.begin(), fcnt.end() - 1);
+    for (int n = 0; n < N; ++n)
+      for (int i = 0; i < M; ++i)
+        fents[ffill[(off[n].x + i) * fsegs + off[n].y / sw]++] = make_int2(n * M + i, off[n].y << 16 | i);
+  }
+
   // launch shapes
   c->col_blocks = (K + (WG / threads_per_line(M)) - 1) / (WG / threads_per_line(M));
   c->col_chunk = std::max(1, std::min(16, (int)((long)N * c->col_blocks / 2048)));
@@ -2067,6 +2165,14 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     c->rowg = !(e_rg && atoi(e_rg) == 0);
   }
 
+  if (c->rowf_ok) {
+    if ((r = dev_alloc(&c->rowf_ptr, fcnt.size())) != PTY_OK) return cleanup(r);
+    if ((r = dev_alloc(&c->rowf_ent, fents.size())) != PTY_OK) return cleanup(r);
+    if ((r = rowf_attribute()) != PTY_OK) return cleanup(r);
+    const char* e_rf = getenv("PTY_ROW_FWD_OROW");
+    c->rowf = !(e_rf && atoi(e_rf) == 0);
+  }
+
   c->cap_pchunks = (size_t)c->probe_chunks;
   c->cap_G = 1;
   c->cap_econP_all = 2 * (size_t)c->cap_econP;
@@ -2087,6 +2193,10 @@ int pty_create(pty_ctx** out, int32_t N, int32_t M, int32_t K, int32_t U, int32_
     e = hipMemcpy(c->row_ptr, rcnt.data(), rcnt.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess && c->rowg_ok)
     e = hipMemcpy(c->row_ent, rents.data(), rents.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->rowf_ok)
+    e = hipMemcpy(c->rowf_ptr, fcnt.data(), fcnt.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && c->rowf_ok)
+    e = hipMemcpy(c->rowf_ent, fents.data(), fents.size() * sizeof(int2), hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(fail(PTY_EHIP, std::string("upload: ") + hipGetErrorString(e)));
   c->g.N = N; c->g.M = M; c->g.K = K; c->g.U = U; c->g.V = V;
   c->g.off = c->off;
@@ -2100,6 +2210,7 @@ int pty_destroy(pty_ctx* c) {
   else (void)hipDeviceSynchronize();
   dev_free(c->off); dev_free(c->tile_ptr); dev_free(c->tile_frames);
   dev_free(c->row_ptr); dev_free(c->row_ent);
+  dev_free(c->rowf_ptr); dev_free(c->rowf_ent);
   dev_free(c->twM); dev_free(c->twK);
   dev_free(c->X); dev_free(c->F); dev_free(c->Xp);
   dev_free(c->Obak); dev_free(c->Pbak); dev_free(c->Os); dev_free(c->Oroll); dev_free(c->Ps);
