@@ -188,26 +188,31 @@ struct RowFwdList {
   const int2* __restrict__ ent;     // (n*M + i, x_n << 16 | i)
 };
 
-// The forward row stage of O_win P in object-row order: one work-group per
-// (object row u, column segment).  The object columns the segment's windows
-// reach ([v0, v0 + segw + L - 1), clipped to V) and the segment's list of
-// frame rows (ROWF_ENT entries at a time) are staged in LDS; the listed frame
-// rows are then formed LPW at a time from LDS and the probe row (L2) and
-// transformed as k_rows_fwd<PK, SRC_OP> does, with the same arithmetic, so
-// the spectra are bit-identical.  Each object row is read from memory once
+// The forward row stage of the sources that read the object (SRC_OP, SRC_DM,
+// SRC_HIO) in object-row order: one work-group per (object row u, column
+// segment).  The object columns the segment's windows reach ([v0, v0 + segw
+// + L - 1), clipped to V) and the segment's list of frame rows (ROWF_ENT
+// entries at a time) are staged in LDS; the listed frame rows are then formed
+// LPW at a time from LDS, the probe row (L2) and, for DM / HIO, the frame row
+// of f (read a batch ahead), and transformed as k_rows_fwd<PK, SRC> does,
+// with the same arithmetic, so the spectra are bit-identical (X may alias F:
+// a row is read and written by the same batch).  Each object row is read from memory once
 // per segment instead of once by every frame whose window covers it (about
 // 40 frames at a 40-pixel step), and no global load but the probe row's sits
 // between one batch and the next (list entries read from memory a batch
 // ahead stalled the batches whenever other traffic had pushed the lists out
 // of the Infinity Cache: 0.40 vs 0.56-0.60 ms at 4096 x 256^2).
 constexpr int ROWF_ENT = 256;
-template <int PK>
-__global__ __launch_bounds__(WG, 3) void k_rows_fwd_orow(Geom g, const float2* __restrict__ O,
-                                                         const float2* P, float2* X, RowFwdList a,
-                                                         const float2* __restrict__ tw) {
+template <int PK, int SRC>
+__global__ __launch_bounds__(WG, (SRC == SRC_DM || SRC == SRC_HIO) ? 2 : 3)
+void k_rows_fwd_orow(Geom g, const float2* __restrict__ O, const float2* P, const float2* F,
+                     float2* X, RowFwdList a, const float2* __restrict__ tw, float w) {
   constexpr int L = plan_len(PK);
   constexpr int T = Plan<PK>::T, E = L / T, LPW = WG / T;
   constexpr int W = LineLds<PK>::words;
+  constexpr bool HAS_F = (SRC == SRC_DM || SRC == SRC_HIO);
+  constexpr int E2 = HAS_F ? E : 1;  // f values (DM, HIO)
+  static_assert(SRC == SRC_OP || HAS_F, "sources that read the object");
   static_assert(ROWF_ENT % LPW == 0, "whole batches per list chunk");
   __shared__ float2 lds[LPW * W];
   __shared__ float2 s_tw[L];
@@ -221,11 +226,16 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd_orow(Geom g, const float2* _
   const int span = min(g.V, v0 + a.segw + L - 1) - v0;
   const float2* Ou = O + (size_t)u * g.V + v0;
   for (int v = tid; v < span; v += WG) orow[v] = Ou[v];
-  float2 pv[E];
+  float2 pv[E], fv[E2];
   auto load_probe = [&](int2 m) {
     const float2* Prow = probe_row(g, P, m.x, m.y & 0xffff);
 #pragma unroll
     for (int e = 0; e < E; ++e) pv[e] = Prow[layA<PK>(e, t)];
+    if constexpr (HAS_F) {
+      const size_t rbase = (size_t)m.x * L;
+#pragma unroll
+      for (int e = 0; e < E; ++e) fv[e] = load_stream(&F[rbase + layA<PK>(e, t)]);
+    }
   };
   for (int c0 = 0; c0 < cnt; c0 += ROWF_ENT) {
     const int nc = min(ROWF_ENT, cnt - c0);
@@ -241,7 +251,12 @@ __global__ __launch_bounds__(WG, 3) void k_rows_fwd_orow(Geom g, const float2* _
       const float2* Ow = orow + ((m.y >> 16) - v0);
       float2 v[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = cmul(Ow[layA<PK>(e, t)], pv[e]);
+      for (int e = 0; e < E; ++e) {
+        float2 psi = cmul(Ow[layA<PK>(e, t)], pv[e]);
+        if constexpr (SRC == SRC_DM) psi = make_float2(2.f * psi.x - fv[e].x, 2.f * psi.y - fv[e].y);
+        if constexpr (SRC == SRC_HIO) psi = make_float2(fv[e].x + w * psi.x, fv[e].y + w * psi.y);
+        v[e] = psi;
+      }
       const int2 mn = s_ent[min((bt + 1) * LPW + line, nc - 1)];
       if (bt + 1 < nbatch) load_probe(mn);
       fft_fwd<PK>(v, lds + line * W, s_tw, t);

@@ -386,19 +386,23 @@ int rowf_segments(int PK, int L, int V) {
 
 int rowf_attribute() {
   hipError_t e = hipSuccess;
-#define C(Lv)                                                                                 \
+#define A(Lv, SRC)                                                                            \
   if (e == hipSuccess)                                                                        \
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_fwd_orow<Lv>),              \
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rows_fwd_orow<Lv, SRC>),         \
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)ROWF_DYN_MAX);
+#define C(Lv) A(Lv, SRC_OP) A(Lv, SRC_DM) A(Lv, SRC_HIO)
   PTY_PLANS(C)
 #undef C
+#undef A
   if (e != hipSuccess) return fail(PTY_EHIP, std::string("k_rows_fwd_orow LDS attribute: ") + hipGetErrorString(e));
   return PTY_OK;
 }
 
-// O_win P over the whole stack, row-transformed, in object-row order
-// (k_rows_fwd_orow); false when the lists do not apply to the current plan
-bool rows_fwd_orow(pty_ctx* c, const float2* O, const float2* P, float2* X) {
+// The forward row stage of a source that reads the object, over the whole
+// stack, in object-row order (k_rows_fwd_orow); false when the lists do not
+// apply to the current plan
+template <int SRC>
+bool rows_fwd_orow(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, float w) {
   const int PK = pk(c, c->K, PLAN_ROWS);
   const size_t dyn = (size_t)std::min(c->V, c->rowf_segw + c->K - 1) * sizeof(float2);
   if (dyn > ROWF_DYN_MAX || rowf_static(PK) + dyn > (size_t)(160u << 10)) return false;
@@ -407,8 +411,8 @@ bool rows_fwd_orow(pty_ctx* c, const float2* O, const float2* P, float2* X) {
   switch (PK) {
 #define C(Lv)                                                                                \
   case Lv:                                                                                   \
-    hipLaunchKernelGGL(k_rows_fwd_orow<Lv>, dim3(grid), dim3(WG), dyn, c->stream, c->g, O, P, X, a, \
-                       (const float2*)c->twK);                                               \
+    hipLaunchKernelGGL((k_rows_fwd_orow<Lv, SRC>), dim3(grid), dim3(WG), dyn, c->stream, c->g, O, P, \
+                       F, X, a, (const float2*)c->twK, w);                                   \
     break;
     PTY_PLANS(C)
 #undef C
@@ -421,8 +425,8 @@ template <int SRC>
 int rows_fwd(pty_ctx* c, const float2* O, const float2* P, const float2* F, float2* X, int nrows,
              float w = 0.f) {
   StageTimer tm(c, PTY_STAGE_ROWS_FWD);
-  if constexpr (SRC == SRC_OP) {
-    if (c->rowf && nrows == c->N * c->M && rows_fwd_orow(c, O, P, X)) {
+  if constexpr (SRC != SRC_F) {
+    if (c->rowf && nrows == c->N * c->M && rows_fwd_orow<SRC>(c, O, P, F, X, w)) {
       PTY_LAUNCHED("k_rows_fwd_orow");
       return PTY_OK;
     }

@@ -1,19 +1,20 @@
-"""The forward row stage of O_win P in object-row order (k_rows_fwd_orow)
-against the frame-order kernel (k_rows_fwd<SRC_OP>, selected with
-PTY_ROW_FWD_OROW=0 at context creation).
+"""The forward row stage in object-row order (k_rows_fwd_orow) against the
+frame-order kernel (k_rows_fwd, selected with PTY_ROW_FWD_OROW=0 at context
+creation), for the three sources that read the object: O_win P (ERA, DM on
+spectra, the metric passes), 2 O_win P - f (DM in real space) and
+f + w O_win P (HIO).
 
-Both form every frame row as O[y_n + i, x_n + c] * P[i, c] with the same
-complex product and transform it with the same line FFT; only where the
-object values come from differs (LDS staging of the object-row segment vs
-global loads of each window).  So every result is bit-identical: ERA and DM
-iterates, metrics and info, for objects of one and of several column
-segments, on the alternative row plan, with packed scans and with sub-pixel
-(per-frame) probes."""
+Both form every frame row with the same complex arithmetic and transform it
+with the same line FFT; only where the object values come from differs (LDS
+staging of the object-row segment vs global loads of each window).  So every
+result is bit-identical: ERA, DM and HIO iterates, metrics and info, for
+objects of one and of several column segments, on the alternative row plan,
+with packed scans and with sub-pixel (per-frame) probes."""
 
 import numpy as np
 import pytest
 
-from ptychography_amd import DM, ERA, ERA_many
+from ptychography_amd import DM, ERA, ERA_many, HIO
 from tests.scans import broadband_start, small_scan
 
 pytestmark = pytest.mark.gpu
@@ -33,7 +34,7 @@ def _same(a, b):
     assert len(a) == len(b)
     for x, y in zip(a[:-1], b[:-1]):
         assert np.array_equal(x, y)
-    for k in ("eMod", "eCon", "eSup"):
+    for k in ("eMod", "eCon", "eSup", "exits", "I", "heatmap"):
         if k in a[-1]:
             assert np.array_equal(np.asarray(a[-1][k]), np.asarray(b[-1][k])), k
 
@@ -74,6 +75,32 @@ def test_dm_object_row_forward_is_bit_identical(method, gpu_device, monkeypatch)
     kw = {"background": np.full(s.I.shape[1:], 0.01 * s.I.mean())} if bg else {}
     run = lambda: DM(s.I, s.R.copy(), s.P.copy(), O0.copy(), 6, method=method, mask=s.mask,
                      device=gpu_device, **kw)
+    _same(*_both(run, monkeypatch))
+
+
+@pytest.mark.parametrize("method", [1, 3, 4])
+def test_dm_real_space_is_bit_identical(method, gpu_device, monkeypatch):
+    # DM on f in real space: the row pass forms 2 O_win P - f
+    print(f"row forward DM method {method} in real space", flush=True)
+    monkeypatch.setenv("PTY_DM_FOURIER", "0")
+    bg = method == 4
+    s = small_scan(frame=64, grid=(4, 4), step=16, seed=76, beamstop_radius=3.0, dead_frac=0.05,
+                   background_frac=0.01 if bg else 0.0)
+    O0 = broadband_start(s.O.shape, seed=76)
+    kw = {"background": np.full(s.I.shape[1:], 0.01 * s.I.mean())} if bg else {}
+    run = lambda: DM(s.I, s.R.copy(), s.P.copy(), O0.copy(), 6, method=method, mask=s.mask,
+                     device=gpu_device, **kw)
+    _same(*_both(run, monkeypatch))
+
+
+@pytest.mark.parametrize("update", ["sample", "both"])
+def test_hio_is_bit_identical(update, gpu_device, monkeypatch):
+    # HIO's row pass forms f + w O_win P
+    print(f"row forward HIO update={update}", flush=True)
+    s = small_scan(frame=64, grid=(5, 5), step=12, seed=77)
+    O0 = broadband_start(s.O.shape, seed=77)
+    run = lambda: HIO(s.I, s.R.copy(), s.P.copy(), O0.copy(), 6, beta=0.9, update=update,
+                      device=gpu_device)
     _same(*_both(run, monkeypatch))
 
 
