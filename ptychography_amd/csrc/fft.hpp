@@ -51,6 +51,32 @@ __device__ __forceinline__ void store_stream(float2* p, float2 v) {
   __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
 }
 
+// Raw buffer accesses relative to a wave-uniform base: the buffer resource
+// (the base, no range limit) is held in SGPRs, the per-lane byte offset in
+// one VGPR and a uniform byte offset in an SGPR, so an access costs no
+// per-lane 64-bit address arithmetic.  `nt`: non-temporal (streaming).
+// Offsets are bytes and must stay below 2^32.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+constexpr int BUF_NT = 2;  // cache-policy bits of a non-temporal access
+template <int AUX = BUF_NT>
+__device__ __forceinline__ float2 buf_load2(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v w = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+  return make_float2(w.x, w.y);
+}
+template <int AUX = BUF_NT>
+__device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+}
+template <int AUX = BUF_NT>
+__device__ __forceinline__ void buf_store2(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef unsigned u2v __attribute__((ext_vector_type(2)));
+  const u2v w = {__builtin_bit_cast(unsigned, v.x), __builtin_bit_cast(unsigned, v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, voff, soff, AUX);
+}
+
 // multiply by -i (forward sign) or +i (inverse sign)
 template <bool INV>
 __device__ __forceinline__ float2 mul_negi(float2 a) {

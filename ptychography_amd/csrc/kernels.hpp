@@ -545,11 +545,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
   auto load = [&](int item, int n) {
     const int c = min((item % ncb) * CPW + l, K - 1);  // clamped: spare columns re-read a real one
     const uint32_t toff = (uint32_t)(t * K + c);
-    const float2* Xn = a.X + (size_t)n * L * K;
+    const auto rx = buf_rsrc(a.X + (size_t)n * L * K);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int u = TR::fwd ? rowA<PK>(e) : rowB<PK>(e);
-      nx[e] = load_stream(&(Xn + (size_t)u * K)[toff]);
+      nx[e] = buf_load2(rx, toff * 8u, (uint32_t)(u * K) * 8u);
     }
   };
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -594,10 +594,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       float2* Fn = DMBF ? a.F + (size_t)nu * L * K : nullptr;
       if constexpr (HOIST) {
         load(item, nu);
-        const float* ampn = a.amp + (size_t)nu * a.amp_stride;
+        // spare columns (not cv) read a real one; their values are never used
+        const auto ra = buf_rsrc(a.amp + (size_t)nu * a.amp_stride);
+        const uint32_t toffc = (uint32_t)(t * K + min(c, K - 1));
 #pragma unroll
-        for (int e = 0; e < E; ++e)
-          amh[e] = cv ? load_stream(&(ampn + (size_t)rowB<PK>(e) * K)[toff]) : 0.f;
+        for (int e = 0; e < E; ++e) amh[e] = buf_load1(ra, toffc * 4u, (uint32_t)(rowB<PK>(e) * K) * 4u);
         if constexpr (HBG) {
           const float* bgn = a.bg + (size_t)nu * a.bg_stride;
 #pragma unroll
@@ -621,6 +622,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       }
       if constexpr (TR::fwd) fft_fwd<PK>(v, mid, s_tw, t);
 
+      // COL_ERA1: the frame's E terms of each error sum in FP32 (relative
+      // rounding <= E 2^-24), then added to the FP64 partials
+      float fr0 = 0.f, fr1 = 0.f;
       if constexpr (MODE != COL_FWD && MODE != COL_INV) {
         if (cv) {
           const float* ampn = a.amp ? a.amp + (size_t)nu * a.amp_stride : nullptr;
@@ -648,11 +652,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 const float mg = msqrt(z2) + a.alpha;
                 if constexpr (MODE == COL_ERA1) {
                   const float d = mg - am;
-                  acc[0] += (double)(m * d * d);
+                  fr0 += m * d * d;
                 }
                 const float fac = (m != 0.f) ? am * mrcp(mg) : 1.f;
                 const float2 zn = z * fac;
-                if constexpr (MODE == COL_ERA1) acc[1] += (double)norm2(zn);
+                if constexpr (MODE == COL_ERA1) fr1 += norm2(zn);
                 v[e] = zn * a.scale;
               } else if constexpr (MODE == COL_ERR1) {
                 const float d = msqrt(z2) - am;
@@ -759,6 +763,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         }
       }
 
+      if constexpr (MODE == COL_ERA1) {
+        acc[0] += (double)fr0;
+        acc[1] += (double)fr1;
+      }
       if constexpr (TR::store_fwd) {
         if (cv) {
 #pragma unroll
@@ -772,10 +780,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           for (int e = 0; e < E; ++e) v[e] = v[e] * a.scale;
         }
         fft_inv<PK>(v, mid, s_tw, t);
-        float2* dst = MODE == COL_DMBF_LEAD ? a.Xo + (size_t)nu * L * K : Xn;
+        const auto rd = buf_rsrc(MODE == COL_DMBF_LEAD ? a.Xo + (size_t)nu * L * K : Xn);
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) store_stream(&(dst + (size_t)rowA<PK>(e) * K)[toff], v[e]);
+          for (int e = 0; e < E; ++e) buf_store2(v[e], rd, toff * 8u, (uint32_t)(rowA<PK>(e) * K) * 8u);
         }
       }
       if constexpr (MODE == COL_DMBF) {
@@ -1189,11 +1197,15 @@ struct ProbeArgs {
 
 __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
   const int MK = g.M * g.K;
-  const int p = blockIdx.x * WG + threadIdx.x;
+  // XCD-aware order: each die takes consecutive probe rows of one frame chunk,
+  // whose object-window rows overlap (frame rows a jitter apart)
+  const int bl = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int bx = bl % gridDim.x, by = bl / gridDim.x;
+  const int p = bx * WG + threadIdx.x;
   if (p >= MK) return;
   const int i = p / g.K, j = p % g.K;
-  const int n0 = a.cstart ? a.cstart[blockIdx.y] : blockIdx.y * a.chunk;
-  const int n1 = a.cstart ? a.cstart[blockIdx.y + 1] : min(g.N, n0 + a.chunk);
+  const int n0 = a.cstart ? a.cstart[by] : by * a.chunk;
+  const int n1 = a.cstart ? a.cstart[by + 1] : min(g.N, n0 + a.chunk);
   double nx = 0.0, ny = 0.0, gg = 0.0;
   // eight frames' loads in flight per step; the sums take the frames in order
   constexpr int FB = 8;
@@ -1217,8 +1229,8 @@ __global__ __launch_bounds__(WG) void k_probe_partial(Geom g, ProbeArgs a) {
       }
     }
   }
-  a.pnum[(size_t)blockIdx.y * MK + p] = make_double2(nx, ny);
-  if (a.fresh) a.pg[(size_t)blockIdx.y * MK + p] = gg;
+  a.pnum[(size_t)by * MK + p] = make_double2(nx, ny);
+  if (a.fresh) a.pg[(size_t)by * MK + p] = gg;
 }
 
 struct ProbeFinalArgs {
