@@ -71,6 +71,10 @@ __device__ __forceinline__ float buf_load1(__amdgpu_buffer_rsrc_t r, uint32_t vo
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
 }
 template <int AUX = BUF_NT>
+__device__ __forceinline__ void buf_store1(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, AUX);
+}
+template <int AUX = BUF_NT>
 __device__ __forceinline__ void buf_store2(float2 v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
   typedef unsigned u2v __attribute__((ext_vector_type(2)));
   const u2v w = {__builtin_bit_cast(unsigned, v.x), __builtin_bit_cast(unsigned, v.y)};

@@ -591,6 +591,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       // re-read what they just stored)
       constexpr bool KEEPF = MODE == COL_DMBF && (L == 256 || E <= 8);
       float2 fnh[KEEPF ? E : 1];
+      // ... and loads the iterate F ahead of the forward transform (its
+      // registers then hold F_new)
+      constexpr bool HF = MODE == COL_DMBF && L == 256;
+      float2 fh[HF ? E : 1];
       float2* Fn = DMBF ? a.F + (size_t)nu * L * K : nullptr;
       if constexpr (HOIST) {
         load(item, nu);
@@ -600,9 +604,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
 #pragma unroll
         for (int e = 0; e < E; ++e) amh[e] = buf_load1(ra, toffc * 4u, (uint32_t)(rowB<PK>(e) * K) * 4u);
         if constexpr (HBG) {
-          const float* bgn = a.bg + (size_t)nu * a.bg_stride;
+          const auto rb = buf_rsrc(a.bg + (size_t)nu * a.bg_stride);
 #pragma unroll
-          for (int e = 0; e < E; ++e) bgh[e] = cv ? (bgn + (size_t)rowB<PK>(e) * K)[toff] : 0.f;
+          for (int e = 0; e < E; ++e) bgh[e] = buf_load1<0>(rb, toffc * 4u, (uint32_t)(rowB<PK>(e) * K) * 4u);
+        }
+        if constexpr (HF) {
+          const auto rF = buf_rsrc(Fn);
+#pragma unroll
+          for (int e = 0; e < E; ++e) fh[e] = buf_load2<0>(rF, toffc * 8u, (uint32_t)(rowB<PK>(e) * K) * 8u);
         }
         if constexpr (HBB) {
 #pragma unroll
@@ -714,17 +723,18 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 // b0 = 2 bbar - b1 (bbar = frame mean of b1), Pmod_7(y, b0),
                 // F_new = f1 - S + y', b_new = b1 - bbar + b0'; the step size
                 // sum |y' - S|^2 / (M K) = sum |f_new - f1|^2 in real space
-                float* bp = (bgn + uo) + toff;
-                float b;
-                if constexpr (HBG) b = bgh[e];
-                else b = *bp;
-                float2* fp = (Fn + uo) + toff;
-                const float2 f = *fp;
+                static_assert(HBG, "COL_DMBF reads its hoisted background");
+                const auto rF = buf_rsrc(Fn), rb = buf_rsrc(bgn), rbb = buf_rsrc(a.bbar);
+                const uint32_t so = (uint32_t)u * (uint32_t)K;
+                const float b = bgh[e];
+                float2 f;
+                if constexpr (HF) f = fh[e];
+                else f = buf_load2<0>(rF, toff * 8u, so * 8u);
                 const float mg1 = msqrt(norm2(f) + b * b + a.alpha);
                 const float fac1 = m * am * mrcp(mg1);
                 const float b1 = b * fac1;
                 const float2 f1 = f * fac1;
-                const float bb = (a.bbar + uo)[toff];
+                const float bb = buf_load1<0>(rbb, toff * 4u, so * 4u);
                 const float2 y = make_float2(2.f * z.x - f1.x, 2.f * z.y - f1.y);
                 const float b0 = 2.f * bb - b1;
                 const float mg2 = msqrt(norm2(y) + b0 * b0 + a.alpha);
@@ -733,8 +743,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                 const float2 fn = make_float2(f1.x - z.x + yp.x, f1.y - z.y + yp.y);
                 const float bn = (b1 - bb) + b0 * fac2;
                 acc[0] += (double)norm2(yp - z);
-                *fp = fn;
-                *bp = bn;
+                buf_store2<0>(fn, rF, toff * 8u, so * 8u);
+                buf_store1<0>(bn, rb, toff * 4u, so * 4u);
                 bnh[e] = bn;
                 if constexpr (KEEPF) fnh[e] = fn;
                 bacc[e] += (BAcc)bn;
@@ -793,7 +803,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
           for (int e = 0; e < E; ++e) {
             float2 fn;
             if constexpr (KEEPF) fn = fnh[e];
-            else fn = (Fn + (size_t)rowB<PK>(e) * K)[toff];
+            else fn = buf_load2<0>(buf_rsrc(Fn), toff * 8u, (uint32_t)(rowB<PK>(e) * K) * 8u);
             const float m = ((mbits >> e) & 1u) ? 1.f : 0.f;
             const float mg = msqrt(norm2(fn) + bnh[e] * bnh[e] + a.alpha);
             const float fac = m * amh[e] * mrcp(mg);
@@ -806,10 +816,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         }
         lds_barrier();
         fft_inv<PK>(v, mid, s_tw, t);
-        float2* Xon = a.Xo + (size_t)nu * L * K;
+        const auto rXo = buf_rsrc(a.Xo + (size_t)nu * L * K);
         if (cv) {
 #pragma unroll
-          for (int e = 0; e < E; ++e) store_stream(&(Xon + (size_t)rowA<PK>(e) * K)[toff], v[e]);
+          for (int e = 0; e < E; ++e) buf_store2(v[e], rXo, toff * 8u, (uint32_t)(rowA<PK>(e) * K) * 8u);
         }
       }
       lds_barrier();
