@@ -470,6 +470,7 @@ template <int PK, int MODE> struct ColShape {
   // waves more; COL_DMBF spills at 4)
   static constexpr int wpe =
       MODE == COL_DMBF ? (E <= 8 ? 3 : 2)
+      : (MODE == COL_DM1F && L == 256 && E == 16) ? 3
       : hoist ? (bgvals ? (E <= 8 ? 4 : (MODE == COL_DMB_UPD && L == 128) ? 2 : 3) : 4)
               : ColTraits<MODE>::heavy ? 2 : 3;  // waves per SIMD
 };
@@ -593,7 +594,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
       float2 fnh[KEEPF ? E : 1];
       // ... and loads the iterate F ahead of the forward transform (its
       // registers then hold F_new)
-      constexpr bool HF = MODE == COL_DMBF && L == 256;
+      constexpr bool HF = (MODE == COL_DMBF || MODE == COL_DM1F) && L == 256;
       float2 fh[HF ? E : 1];
       float2* Fn = DMBF ? a.F + (size_t)nu * L * K : nullptr;
       if constexpr (HOIST) {
@@ -611,7 +612,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
         if constexpr (HF) {
           const auto rF = buf_rsrc(Fn);
 #pragma unroll
-          for (int e = 0; e < E; ++e) fh[e] = buf_load2<0>(rF, toffc * 8u, (uint32_t)(rowB<PK>(e) * K) * 8u);
+          for (int e = 0; e < E; ++e)
+            fh[e] = buf_load2<MODE == COL_DM1F ? BUF_NT : 0>(rF, toffc * 8u, (uint32_t)(rowB<PK>(e) * K) * 8u);
         }
         if constexpr (HBB) {
 #pragma unroll
@@ -758,7 +760,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
                   acc[0] += (double)(m * d * d);
                 }
                 float2* fp = (Fn + uo) + toff;
-                const float2 f = load_stream(fp);
+                float2 f;
+                if constexpr (HF) f = fh[e];
+                else f = load_stream(fp);
                 const float2 y = make_float2(2.f * z.x - f.x, 2.f * z.y - f.y);
                 const float mg = msqrt(norm2(y)) + a.alpha;
                 const float fac = (m != 0.f) ? am * mrcp(mg) : 1.f;
