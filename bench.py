@@ -346,23 +346,32 @@ def cpu_baseline(scan, nframes):
     """Oracle ERA (method 1, complex128, reference-faithful make_exits) on the first
     `nframes` frames with the full-size object; per-iteration time scaled to all frames."""
     import oracle
-    I = scan["I"][:nframes].double().cpu().numpy()
     R = scan["R"][:nframes].astype(np.int64)
-    O0 = np.ones((scan["U"], scan["V"]), np.complex128)
-    stamps = []
-    t0 = time.perf_counter()
-    oracle.ERA(I, R.copy(), scan["P"].astype(np.complex128), O0, 2, method=1,
-               mask=scan["mask"], full_output=False,
-               callback=lambda i, O, P: stamps.append(time.perf_counter()))
-    per_iter = stamps[-1] - stamps[-2]
-    total = time.perf_counter() - t0
-    per_iter_full = per_iter * scan["N"] / nframes
-    return {"value": 1.0 / per_iter_full, "unit": "iters/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle.ERA method 1, complex128, {nframes} of {scan['N']} frames "
-                       f"(256^2) with the full {scan['U']}x{scan['V']} object, 2 iterations "
-                       f"({total:.1f} s wall); second iteration {per_iter:.3f} s scaled by "
-                       f"{scan['N']}/{nframes}; host {cpu_model()[0]}, {cpu_model()[1]} logical "
-                       f"CPUs, one used (NumPy's FFT is single-threaded)")}
+
+    def timed(real, cplx):
+        # the oracle follows the reference's dtype rule: float64 I -> complex128
+        # arithmetic, float32 I -> complex64
+        I = scan["I"][:nframes].cpu().numpy().astype(real)
+        O0 = np.ones((scan["U"], scan["V"]), cplx)
+        stamps = []
+        t0 = time.perf_counter()
+        oracle.ERA(I, R.copy(), scan["P"].astype(cplx), O0, 2, method=1,
+                   mask=scan["mask"], full_output=False,
+                   callback=lambda i, O, P: stamps.append(time.perf_counter()))
+        return stamps[-1] - stamps[-2], time.perf_counter() - t0
+
+    per_iter, total = timed(np.float64, np.complex128)
+    per_iter64, total64 = timed(np.float32, np.complex64)
+    scale = scan["N"] / nframes
+    return {"value": 1.0 / (per_iter * scale), "unit": "iters/s", "cores": 1, "kind": "port",
+            "value_complex64": 1.0 / (per_iter64 * scale),
+            "sample": (f"oracle.ERA method 1, complex128 (value) and complex64 (value_complex64), "
+                       f"{nframes} of {scan['N']} frames (256^2) with the full "
+                       f"{scan['U']}x{scan['V']} object, 2 iterations each ({total:.1f} s and "
+                       f"{total64:.1f} s wall); second iteration {per_iter:.3f} s / "
+                       f"{per_iter64:.3f} s scaled by {scan['N']}/{nframes}; host "
+                       f"{cpu_model()[0]}, {cpu_model()[1]} logical CPUs, one used (NumPy's FFT "
+                       f"is single-threaded)")}
 
 
 def main():
