@@ -132,3 +132,22 @@ def test_subpixel_positions(gpu_device, monkeypatch):
     O0 = broadband_start((U, V), seed=75)
     run = lambda: ERA(I, R.copy(), P.copy(), O0.copy(), 4, method=1, device=gpu_device)
     _same(*_both(run, monkeypatch))
+
+
+def test_object_wider_than_the_row_lists_take(gpu_device, monkeypatch):
+    # V > 8192: no object-row lists (the packed column offset holds 13 bits)
+    # and no fused row gather; the frame-order kernels run and match the oracle
+    import oracle
+    from tests.scans import rel_l2
+    print("object wider than 8192 columns", flush=True)
+    s = small_scan(frame=128, grid=(2, 66), step=126, jitter=1, seed=78)
+    assert s.O.shape[1] > 8192
+    O0 = broadband_start(s.O.shape, seed=78)
+    O_g, info = ERA(s.I, s.R.copy(), s.P.copy(), O0.copy(), 3, method=1, device=gpu_device)
+    O_r, info_r = oracle.ERA(s.I, s.R.copy(), s.P.copy(), O0.copy(), 3, method=1)
+    H = info_r["heatmap"]
+    cov = H > 1e-3 * H.max()
+    print(f"  object rel L2 {rel_l2(O_g[cov], O_r[cov]):.1e}", flush=True)
+    assert rel_l2(O_g[cov], O_r[cov]) < 1e-5
+    e, er = np.asarray(info["eMod"]), np.asarray(info_r["eMod"])
+    assert np.max(np.abs(e - er) / er) < 1e-5
