@@ -576,13 +576,21 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
     for (int e = 0; e < (TR::bsum ? E : 1); ++e) bacc[e] = 0;
 #pragma unroll
     for (int e = 0; e < (TR::bsum2 ? E : 1); ++e) bacc2[e] = 0;
+    // the frame-mean background is the same for every frame of the item:
+    // loaded once per item (per frame it was re-fetched past L2 for part of
+    // the frames)
+    float bbh[HBB ? E : 1];
+    if constexpr (HBB) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) bbh[e] = cv ? (a.bbar + (size_t)rowB<PK>(e) * K)[toff] : 0.f;
+    }
 
     for (int n = n0; n < n1; ++n) {
       // opaque, wave-uniform frame index: keeps the frame base in SGPRs
       const int nu = __builtin_amdgcn_readfirstlane(n);
       float2* Xn = a.X + (size_t)nu * L * K;
       float2 v[E];
-      float amh[HOIST ? E : 1], bgh[HBG ? E : 1], bbh[HBB ? E : 1];
+      float amh[HOIST ? E : 1], bgh[HBG ? E : 1];
       // COL_DMBF keeps b_new for its second step (re-reading it and the
       // amplitudes from cache measured 7.24 vs 6.70 ms per DM iteration at
       // 4096 x 256^2)
@@ -614,10 +622,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, CWG),
 #pragma unroll
           for (int e = 0; e < E; ++e)
             fh[e] = buf_load2<MODE == COL_DM1F ? BUF_NT : 0>(rF, toffc * 8u, (uint32_t)(rowB<PK>(e) * K) * 8u);
-        }
-        if constexpr (HBB) {
-#pragma unroll
-          for (int e = 0; e < E; ++e) bbh[e] = cv ? (a.bbar + (size_t)rowB<PK>(e) * K)[toff] : 0.f;
         }
       }
 #pragma unroll
