@@ -22,7 +22,9 @@ to eMod <= 1e-3 of ERA method 3 on the held-out 32 x 32-position scan of
 "multi_scan" reports the batched path: the 16 seeded scans of
 ``sim.c4_specs()`` (frames 64-192^2, 240-1980 positions, 334 M frame pixels
 per iteration), ERA method 3, as sequential single-scan calls and as one batch
-with one hipGraph per iteration, inputs resident.
+with one hipGraph per iteration, inputs resident: scans of one frame size
+packed into one engine (the default, "speedup") and unpacked
+("speedup_unpacked").
 
 ``--copy-calibration`` only copies 1 GiB three times (the plain copy kernel
 whose FETCH_SIZE / WRITE_SIZE counters scale the kernel traffic figures).
@@ -201,20 +203,23 @@ def multi_scan(dev, iters):
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    def batched():
+    def batched(pack=True):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ERA_many([job(sc) for sc in scans], iters, device=dev, full_output=False, graphs=True)
+        ERA_many([job(sc) for sc in scans], iters, device=dev, full_output=False, graphs=True,
+                 pack=pack)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    sequential(), batched()
+    sequential(), batched(), batched(False)
     t_seq = min(sequential() for _ in range(2))
     t_bat = min(batched() for _ in range(2))
+    t_unp = min(batched(False) for _ in range(2))
     return {"workload": "c4-16-scans-method3", "scans": len(scans), "frame_px_per_iter": px,
             "iters": iters, "sequential_s": t_seq, "batched_graphs_s": t_bat,
             "sequential_gpx_iter_per_s": px * iters / t_seq / 1e9,
-            "batched_gpx_iter_per_s": px * iters / t_bat / 1e9, "speedup": t_seq / t_bat}
+            "batched_gpx_iter_per_s": px * iters / t_bat / 1e9, "speedup": t_seq / t_bat,
+            "batched_unpacked_graphs_s": t_unp, "speedup_unpacked": t_seq / t_unp}
 
 
 def learned_init(dev, target=1e-3, max_iters=1500, reps=5):
