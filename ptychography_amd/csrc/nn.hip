@@ -318,23 +318,49 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in,
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
   const uint4 zero = make_uint4(0, 0, 0, 0);
+  // staging: every global load of a chunk is issued before the first LDS
+  // store, so the chunk pays one load latency (a loop of load -> store paid
+  // it once per element); XL covers the halo tile up to W = 128.  (Issuing
+  // the next chunk's loads before this chunk's MFMA loop as well took the
+  // kernel to 456 VGPRs + AGPRs and one wave per SIMD.)
+  constexpr int XL = 8, WL = (9 * BN * (CK / 8) + 255) / 256;
+  const int nx = HW2 * (CK / 8);
   for (int c0 = 0; c0 < C; c0 += CK) {
-    for (int e = tid; e < HW2 * (CK / 8); e += 256) {
+    uint4 xv[XL], wv[WL];
+#pragma unroll
+    for (int k = 0; k < XL; ++k) {
+      const int e = tid + 256 * k;
       const int q = e & (CK / 8 - 1), p = e / (CK / 8);
       const int hr = p / W2, hc = p - hr * W2;
       const int g = g0 - 1 + hr, xx = hc - 1;
-      uint4 v = zero;
-      if (g >= 0 && g < rows && xx >= 0 && xx < W)
-        v = *reinterpret_cast<const uint4*>(in + ((size_t)g * W + xx) * C + c0 + 8 * q);
-      *reinterpret_cast<uint4*>(sX + p * CS + 8 * q) = v;
+      xv[k] = zero;
+      if (e < nx && g >= 0 && g < rows && xx >= 0 && xx < W)
+        xv[k] = *reinterpret_cast<const uint4*>(in + ((size_t)g * W + xx) * C + c0 + 8 * q);
     }
-    for (int e = tid; e < 9 * BN * (CK / 8); e += 256) {
+#pragma unroll
+    for (int k = 0; k < WL; ++k) {
+      const int e = tid + 256 * k;
       const int q = e & (CK / 8 - 1), r = e / (CK / 8);
       const int tap = r / BN, n = r - tap * BN;
-      uint4 v = zero;
-      if (n0 + n < N)
-        v = *reinterpret_cast<const uint4*>(wt + ((size_t)tap * N + n0 + n) * C + c0 + 8 * q);
-      *reinterpret_cast<uint4*>(sW + r * CS + 8 * q) = v;
+      wv[k] = zero;
+      if (e < 9 * BN * (CK / 8) && n0 + n < N)
+        wv[k] = *reinterpret_cast<const uint4*>(wt + ((size_t)tap * N + n0 + n) * C + c0 + 8 * q);
+    }
+#pragma unroll
+    for (int k = 0; k < XL; ++k) {
+      const int e = tid + 256 * k;
+      if (e < nx) {
+        const int q = e & (CK / 8 - 1), p = e / (CK / 8);
+        *reinterpret_cast<uint4*>(sX + p * CS + 8 * q) = xv[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WL; ++k) {
+      const int e = tid + 256 * k;
+      if (e < 9 * BN * (CK / 8)) {
+        const int q = e & (CK / 8 - 1), r = e / (CK / 8);
+        *reinterpret_cast<uint4*>(sW + r * CS + 8 * q) = wv[k];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -531,24 +557,24 @@ __device__ __forceinline__ float c1_input(const void* __restrict__ x, size_t i, 
   }
 }
 
-// first layer: one input channel, N <= 64 outputs.  N % 8 == 0: a thread
-// computes 8 channels of one pixel and stores them as one 16-byte vector
-// (consecutive threads -> consecutive channel groups); otherwise one pixel
-// per thread.
+// first layer: one input channel, N <= 64 outputs, one pixel per thread (its
+// nine inputs formed once).  N % 8 == 0: the channels are stored in 16-byte
+// groups of 8, weights and bias read from LDS as float4.
 template <bool FROM_I>
 __global__ __launch_bounds__(256) void k_conv_c1(const void* __restrict__ x, float scale,
                                                  const uint16_t* __restrict__ w, int B, int H,
                                                  int W, int N, ConvOut o) {
-  __shared__ float sw[9 * 64];
+  __shared__ float4 sw4[9 * 64 / 4];
+  __shared__ float4 sb4[64 / 4];
+  float* sw = reinterpret_cast<float*>(sw4);
+  float* sb = reinterpret_cast<float*>(sb4);
   for (int i = threadIdx.x; i < 9 * N; i += blockDim.x) sw[i] = bf2f(w[i]);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) sb[i] = o.bias ? o.bias[i] : 0.f;
   __syncthreads();
   const size_t P = (size_t)B * H * W;
   const bool vec = (N % 8) == 0 && o.y && !o.yf && !o.mask && ((uintptr_t)o.y & 15) == 0;
-  const int G = vec ? N / 8 : 1;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < P * G;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const size_t pix = e / G;
-    const int grp = e - pix * G;
+  for (size_t pix = (size_t)blockIdx.x * blockDim.x + threadIdx.x; pix < P;
+       pix += (size_t)gridDim.x * blockDim.x) {
     const int xx = pix % W, y = (pix / W) % H;
     float v[9];
 #pragma unroll
@@ -559,16 +585,23 @@ __global__ __launch_bounds__(256) void k_conv_c1(const void* __restrict__ x, flo
                  : 0.f;
     }
     if (vec) {
-      float r[8];
+      for (int grp = 0; grp < N / 8; ++grp) {
+        float r[8];
+        const float4 b0 = sb4[2 * grp], b1 = sb4[2 * grp + 1];
+        r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[3] = b0.w;
+        r[4] = b1.x; r[5] = b1.y; r[6] = b1.z; r[7] = b1.w;
 #pragma unroll
-      for (int h = 0; h < 8; ++h) {
-        const int n = 8 * grp + h;
-        float acc = o.bias ? o.bias[n] : 0.f;
+        for (int t = 0; t < 9; ++t) {
+          const float4 w0 = sw4[(t * N + 8 * grp) / 4], w1 = sw4[(t * N + 8 * grp) / 4 + 1];
+          r[0] += v[t] * w0.x; r[1] += v[t] * w0.y; r[2] += v[t] * w0.z; r[3] += v[t] * w0.w;
+          r[4] += v[t] * w1.x; r[5] += v[t] * w1.y; r[6] += v[t] * w1.z; r[7] += v[t] * w1.w;
+        }
+        if (o.relu) {
 #pragma unroll
-        for (int t = 0; t < 9; ++t) acc += v[t] * sw[t * N + n];
-        r[h] = o.relu ? fmaxf(acc, 0.f) : acc;
+          for (int h = 0; h < 8; ++h) r[h] = fmaxf(r[h], 0.f);
+        }
+        *reinterpret_cast<uint4*>(o.y + pix * N + 8 * grp) = pack8(r);
       }
-      *reinterpret_cast<uint4*>(o.y + pix * N + 8 * grp) = pack8(r);
     } else {
       for (int n = 0; n < N; ++n) {
         float acc = 0.f;
@@ -651,7 +684,10 @@ __global__ __launch_bounds__(256) void k_wgrad_thin(const uint16_t* __restrict__
   }
 }
 
-// few output channels (N <= 8), C % 8 == 0, C <= 64: one pixel per thread
+// few output channels (N <= NN <= 8), C % 8 == 0, C <= 64: one pixel per
+// thread; NN bounds the unrolled output loop (2 for the network's last layer:
+// with 8 the masked iterations cost as many LDS reads as the live ones)
+template <int NN>
 __global__ __launch_bounds__(256) void k_conv_nsmall(const uint16_t* __restrict__ x,
                                                      const uint16_t* __restrict__ w, int B, int H,
                                                      int W, int C, int N, ConvOut o) {
@@ -662,9 +698,9 @@ __global__ __launch_bounds__(256) void k_conv_nsmall(const uint16_t* __restrict_
   for (size_t pix = (size_t)blockIdx.x * blockDim.x + threadIdx.x; pix < P;
        pix += (size_t)gridDim.x * blockDim.x) {
     const int xx = pix % W, y = (pix / W) % H;
-    float s[8];
+    float s[NN];
 #pragma unroll
-    for (int n = 0; n < 8; ++n) s[n] = 0.f;
+    for (int n = 0; n < NN; ++n) s[n] = 0.f;
     for (int t = 0; t < 9; ++t) {
       const int yy = y + t / 3 - 1, x2 = xx + t % 3 - 1;
       if ((unsigned)yy >= (unsigned)H || (unsigned)x2 >= (unsigned)W) continue;
@@ -677,13 +713,13 @@ __global__ __launch_bounds__(256) void k_conv_nsmall(const uint16_t* __restrict_
           const float a = bf2f((uint16_t)(u[h / 2] >> (16 * (h & 1))));
           const float* wr = sw + (t * C + c0 + h) * N;
 #pragma unroll
-          for (int n = 0; n < 8; ++n)
+          for (int n = 0; n < NN; ++n)
             if (n < N) s[n] += a * wr[n];
         }
       }
     }
 #pragma unroll
-    for (int n = 0; n < 8; ++n)
+    for (int n = 0; n < NN; ++n)
       if (n < N) conv_store(o, pix, N, n, s[n]);
   }
 }
@@ -1188,8 +1224,12 @@ int pty_nn_conv3x3_fwd(const uint16_t* x, const uint16_t* w, const uint16_t* wT,
   // the vector kernels read and write 16-byte groups of channels
   const bool al = aligned16(x) && aligned16(wT) && (!y || aligned16(y));
   if (Cout <= 8 && Cin % 8 == 0 && Cin <= 64 && aligned16(x)) {
-    hipLaunchKernelGGL(k_conv_nsmall, dim3(grid_for(P)), dim3(256), 0, s, x, w, B, H, W, Cin,
-                       Cout, o);
+    if (Cout <= 2)
+      hipLaunchKernelGGL(k_conv_nsmall<2>, dim3(grid_for(P)), dim3(256), 0, s, x, w, B, H, W, Cin,
+                         Cout, o);
+    else
+      hipLaunchKernelGGL(k_conv_nsmall<8>, dim3(grid_for(P)), dim3(256), 0, s, x, w, B, H, W, Cin,
+                         Cout, o);
     NN_LAUNCHED("k_conv_nsmall");
     return PTY_OK;
   }
