@@ -10,7 +10,7 @@ times a global phase).  The patches' unknown phases are synchronised globally
 from the pairwise overlap products of neighbouring windows (projected power
 iterations, ``sync_phases``), and O0 is the engine's overlap projection of the
 exits P * patch * phase (the |P|^2-weighted average).  The plain start stays
-the fallback and the judge: ``learned_start`` keeps the learned O0 only if its
+the fallback and the arbiter: ``learned_start`` keeps the learned O0 only if its
 first-iteration eMod beats the plain start's.
 
 Layers (NHWC, bf16 activations and weights, FP32 accumulate):
